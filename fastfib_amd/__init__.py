@@ -1,0 +1,184 @@
+"""fastfib_amd: exact big-integer products and F(n) on MI355X (gfx950).
+
+Python access to libfastfib_amd.so over its C-ABI (include/fastfib_amd.h).
+The library does all arithmetic on the GPU; this module only moves limbs.
+It raises at import-time use if the library is not built (there is no CPU
+fallback): build it with ``python -m fastfib_amd.build``.
+
+Numbers cross as Python ints or numpy uint64 limb arrays (little-endian).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfastfib_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "fastfib_amd.h")
+
+if hasattr(sys, "set_int_max_str_digits"):
+    sys.set_int_max_str_digits(0)
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class Big(ctypes.Structure):
+    _fields_ = [("d", _u64p), ("n", ctypes.c_size_t)]
+
+
+class DBig(ctypes.Structure):
+    _fields_ = [("d", ctypes.c_void_p), ("n", ctypes.c_size_t), ("cap", ctypes.c_size_t),
+                ("bits", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m fastfib_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.ff_init.argtypes = [ctypes.c_int]
+    L.ff_init.restype = ctypes.c_int
+    L.ff_stream.restype = ctypes.c_void_p
+    L.ff_sync.restype = ctypes.c_int
+    L.ff_last_error.restype = ctypes.c_char_p
+    L.ff_ntt_init.argtypes = [ctypes.c_size_t]
+    for name in ("ff_big_mul", "ff_big_mul_ntt"):
+        getattr(L, name).argtypes = [ctypes.POINTER(Big), ctypes.POINTER(Big)]
+        getattr(L, name).restype = Big
+    L.ff_big_sqr.argtypes = [ctypes.POINTER(Big)]
+    L.ff_big_sqr.restype = Big
+    L.ff_fibonacci.argtypes = [ctypes.c_uint64]
+    L.ff_fibonacci.restype = Big
+    L.ff_big_free.argtypes = [ctypes.POINTER(Big)]
+    L.ff_dbig_alloc.argtypes = [ctypes.POINTER(DBig), ctypes.c_size_t]
+    L.ff_dbig_release.argtypes = [ctypes.POINTER(DBig)]
+    L.ff_dbig_upload.argtypes = [ctypes.POINTER(DBig), _u64p, ctypes.c_size_t]
+    L.ff_dbig_download.argtypes = [ctypes.POINTER(DBig), _u64p, ctypes.c_size_t]
+    L.ff_dmul.argtypes = [ctypes.POINTER(DBig), ctypes.POINTER(DBig), ctypes.POINTER(DBig)]
+    L.ff_dfibonacci.argtypes = [ctypes.c_uint64, ctypes.POINTER(DBig)]
+    L.ff_set_option.argtypes = [ctypes.c_char_p, ctypes.c_longlong]
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError(f"fastfib_amd error {rc}: {lib().ff_last_error().decode()}")
+
+
+def to_limbs(x: int) -> np.ndarray:
+    n = max(1, (x.bit_length() + 63) // 64)
+    return np.frombuffer(x.to_bytes(8 * n, "little"), dtype=np.uint64).copy()
+
+
+def from_limbs(d) -> int:
+    return int.from_bytes(np.ascontiguousarray(d, dtype=np.uint64).tobytes(), "little")
+
+
+def _as_big(a) -> tuple[Big, np.ndarray]:
+    arr = to_limbs(a) if isinstance(a, int) else np.ascontiguousarray(a, dtype=np.uint64)
+    if arr.size == 0:
+        arr = np.zeros(1, dtype=np.uint64)
+    return Big(arr.ctypes.data_as(_u64p), arr.size), arr
+
+
+def _take(b: Big) -> np.ndarray:
+    out = np.ctypeslib.as_array(b.d, shape=(b.n,)).copy()
+    lib().ff_big_free(ctypes.byref(b))
+    return out
+
+
+def init(device: int = -1) -> None:
+    _check(lib().ff_init(device))
+
+
+def sync() -> None:
+    _check(lib().ff_sync())
+
+
+def set_option(name: str, value: int) -> None:
+    _check(lib().ff_set_option(name.encode(), int(value)))
+
+
+def big_mul(a, b) -> np.ndarray:
+    """ff_big_mul (general product; pass the same object twice to square)."""
+    A, ka = _as_big(a)
+    if b is a:
+        return _take(lib().ff_big_mul(ctypes.byref(A), ctypes.byref(A)))
+    B, kb = _as_big(b)
+    return _take(lib().ff_big_mul(ctypes.byref(A), ctypes.byref(B)))
+
+
+def big_sqr(a) -> np.ndarray:
+    A, ka = _as_big(a)
+    return _take(lib().ff_big_sqr(ctypes.byref(A)))
+
+
+def big_mul_ntt(a, b=None) -> np.ndarray:
+    """ff_big_mul_ntt; b None squares (the reference's a == b)."""
+    A, ka = _as_big(a)
+    if b is None:
+        return _take(lib().ff_big_mul_ntt(ctypes.byref(A), ctypes.byref(A)))
+    B, kb = _as_big(b)
+    return _take(lib().ff_big_mul_ntt(ctypes.byref(A), ctypes.byref(B)))
+
+
+def fibonacci(n: int) -> np.ndarray:
+    return _take(lib().ff_fibonacci(n))
+
+
+class Device:
+    """A number held in device memory (ff_dbig)."""
+
+    def __init__(self, value=None):
+        self.h = DBig()
+        if value is not None:
+            arr = to_limbs(value) if isinstance(value, int) else np.ascontiguousarray(value, dtype=np.uint64)
+            _check(lib().ff_dbig_upload(ctypes.byref(self.h), arr.ctypes.data_as(_u64p), arr.size))
+
+    @property
+    def n(self) -> int:
+        return int(self.h.n)
+
+    @property
+    def bits(self) -> int:
+        return int(self.h.bits)
+
+    def limbs(self) -> np.ndarray:
+        out = np.zeros(max(1, self.h.n), dtype=np.uint64)
+        _check(lib().ff_dbig_download(ctypes.byref(self.h), out.ctypes.data_as(_u64p), out.size))
+        return out
+
+    def value(self) -> int:
+        return from_limbs(self.limbs())
+
+    def release(self) -> None:
+        lib().ff_dbig_release(ctypes.byref(self.h))
+
+    def __del__(self):
+        try:
+            if self.h.d and _lib is not None:
+                self.release()
+        except Exception:
+            pass
+
+
+def dmul(a: Device, b: Device | None = None) -> Device:
+    out = Device()
+    _check(lib().ff_dmul(ctypes.byref(out.h), ctypes.byref(a.h), None if b is None else ctypes.byref(b.h)))
+    return out
+
+
+def dfibonacci(n: int) -> Device:
+    out = Device()
+    _check(lib().ff_dfibonacci(n, ctypes.byref(out.h)))
+    return out

@@ -1,0 +1,97 @@
+// Modular arithmetic for the three NTT primes on gfx950 (32-bit lanes).
+//
+// The primes and generators are those of ntt_arith.c:96-100 (reference):
+// p = c * 2^k + 1 < 2^31, with 3 * 2^25 | p - 1, so every transform length
+// 2^k and 3 * 2^k up to 2^25 has its roots of unity. Values are canonical in
+// [0, p). Multiplication is Montgomery with R = 2^32 computed from the high
+// halves of 32x32 products (v_mul_hi_u32): with m = lo(a*b) * p^-1 mod 2^32,
+// a*b - m*p is divisible by 2^32 and the low halves cancel exactly, so
+//   mont(a, b) = hi(a*b) - hi(m*p)   in (-p, p)
+// followed by one conditional add. It accepts any 32-bit a (b < p), which
+// lets callers pass x - y + p without reducing it first.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FF_P1 2113929217u  // 63 * 2^25 + 1, generator 5
+#define FF_P2 2013265921u  // 15 * 2^27 + 1, generator 31
+#define FF_P3 1811939329u  // 27 * 2^26 + 1, generator 13
+
+// Every supported length divides FF_NBASE = 3 * 2^25, so w_L = w_NBASE^(NBASE/L).
+#define FF_NBASE (3u << 25)
+// Two-level table of w_NBASE^x: x = hi * FF_TL_SIZE + lo.
+#define FF_TL_BITS 14
+#define FF_TL_SIZE (1u << FF_TL_BITS)
+#define FF_TH_SIZE (FF_NBASE >> FF_TL_BITS)
+// Longest LDS-resident sub-transform (radix-2 levels with table twiddles).
+#define FF_TW 4096
+
+struct FFPrime {
+    uint32_t p;     // the prime
+    uint32_t pinv;  // p^-1 mod 2^32
+    uint32_t r2;    // R^2 mod p (mont(h, r2) = h * 2^32 mod p)
+    uint32_t w3;    // primitive cube root of unity (Montgomery form)
+    uint32_t w3p;   // w3 * pinv
+    uint32_t iw3;   // w3^-1 = w3^2 (Montgomery form)
+    uint32_t iw3p;
+    uint32_t one;   // R mod p: 1 in Montgomery form
+};
+
+// Device tables, filled once by ff_tables_init (all in Montgomery form):
+//   rt[pi][h + j]  = w_2h^j      for h = 1, 2, .., FF_TW/2, j < h
+//   irt[pi][h + j] = w_2h^-j
+//   rtp / irtp     = the same times pinv (mod 2^32), for mont_w
+//   tl[pi][x]      = w_NBASE^x        x < FF_TL_SIZE
+//   th[pi][x]      = w_NBASE^(x * FF_TL_SIZE)
+struct FFTables {
+    uint32_t rt[3][FF_TW], rtp[3][FF_TW], irt[3][FF_TW], irtp[3][FF_TW];
+    uint32_t tl[3][FF_TL_SIZE];
+    uint32_t th[3][FF_TH_SIZE];
+};
+
+__device__ __forceinline__ uint32_t ff_mont_fix(uint32_t hi, uint32_t mh, uint32_t p) {
+    uint32_t r = hi - mh;
+    return hi < mh ? r + p : r;
+}
+
+// a * b * 2^-32 mod p, any 32-bit a, b < p.
+__device__ __forceinline__ uint32_t ff_mont(uint32_t a, uint32_t b, uint32_t p, uint32_t pinv) {
+    uint32_t lo = a * b;
+    uint32_t hi = __umulhi(a, b);
+    uint32_t m = lo * pinv;
+    return ff_mont_fix(hi, __umulhi(m, p), p);
+}
+
+// The same with wp = w * pinv precomputed: m = a * wp needs no lo(a*w).
+__device__ __forceinline__ uint32_t ff_mont_w(uint32_t a, uint32_t w, uint32_t wp, uint32_t p) {
+    uint32_t hi = __umulhi(a, w);
+    uint32_t m = a * wp;
+    return ff_mont_fix(hi, __umulhi(m, p), p);
+}
+
+__device__ __forceinline__ uint32_t ff_add(uint32_t x, uint32_t y, uint32_t p) {
+    uint32_t s = x + y;  // < 2p < 2^32
+    uint32_t t = s - p;
+    return t < s ? t : s;  // min(s, s - p) as unsigned
+}
+
+__device__ __forceinline__ uint32_t ff_sub(uint32_t x, uint32_t y, uint32_t p) {
+    uint32_t d = x - y;
+    uint32_t e = d + p;
+    return e < d ? e : d;  // min(d, d + p)
+}
+
+// Reduces any 32-bit value to [0, p) (p > 2^30, so at most 3 steps).
+__device__ __forceinline__ uint32_t ff_red32(uint32_t x, uint32_t p) {
+    uint32_t t = x - p;
+    x = t < x ? t : x;
+    t = x - p;
+    x = t < x ? t : x;
+    t = x - p;
+    return t < x ? t : x;
+}
+
+// w_NBASE^x (Montgomery form) for x < FF_NBASE.
+__device__ __forceinline__ uint32_t ff_root(const FFTables *T, int pi, uint32_t x, uint32_t p, uint32_t pinv) {
+    return ff_mont(T->th[pi][x >> FF_TL_BITS], T->tl[pi][x & (FF_TL_SIZE - 1)], p, pinv);
+}

@@ -1,0 +1,110 @@
+// Host/device shared declarations of the fastfib_amd engine (not part of the
+// public C-ABI in include/fastfib_amd.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ff_arith.h"
+
+// One transform operand or array for the pass kernels. The forward top pass
+// reads `src` (an operand's limbs as 32-bit words, nw of them) and writes
+// `data`; later passes work on `data` ([3][L] words, prime-major) in place.
+struct FFXform {
+    const uint32_t *src;
+    uint32_t nw;
+    uint32_t *data;
+};
+
+// One product for the inner (pointwise) kernels: fa holds a's forward
+// transform and receives the product's; fb = nullptr squares.
+struct FFConv {
+    uint32_t *fa;
+    const uint32_t *fb;
+};
+
+// Parameters of one outer (strided, four-step) pass over blocks of Lb words.
+struct FFPass {
+    uint32_t L;         // full transform length (words per prime)
+    uint32_t Lb;        // length of one block this pass transforms
+    uint32_t R;         // rows = Lb / Sb
+    uint32_t Sb;        // column stride inside the block
+    uint32_t nblocks;   // L / Lb
+    uint32_t tpb;       // column tiles per block = Sb / C
+    uint32_t rootstep;  // FF_NBASE / Lb
+    uint32_t rstep3;    // FF_NBASE / R (radix-3 layer twiddles)
+    int rho;            // R = 2^rho or 3 * 2^rho
+    int r3;
+    int C, logC, ld;
+    int top;            // forward: read limbs; inverse: apply the CRT scale
+    int bw;             // coefficient width (forward top pass)
+    uint32_t sc[3];     // R^2 / L mod p (inverse top pass)
+};
+
+// Parameters of the inner kernels: contiguous blocks of S = 2^s words, or
+// the whole transform in one block when small = 1 (L <= FF_TW, may be 3*2^k).
+struct FFInner {
+    uint32_t L;
+    uint32_t S;
+    int s;
+    int small;
+    int r3;         // small only
+    uint32_t rstep3;
+    int bw;         // small only: coefficient width
+    uint32_t sc[3]; // small only: CRT scale
+};
+
+// A product's full schedule.
+struct FFPlan {
+    uint32_t L;
+    int bw;
+    int r3;
+    int lg;          // L = 2^lg or 3 * 2^lg
+    uint64_t conv;   // product coefficients
+    int npass;       // outer passes (0: one small kernel)
+    FFPass pass[3];
+    FFInner inner;
+};
+
+// Per-chunk carry record written by the CRT pack kernel.
+struct FFChunk {
+    uint64_t h_lo, h_hi;  // carry out of the chunk's bw limbs (bits above)
+    uint32_t ones;        // limbs 2 .. bw-1 of the chunk are all ones
+    uint32_t pad_;
+};
+
+extern "C" {
+// ff_ntt.hip
+int ff_tables_init(void);
+const FFTables *ff_tables_dev(void);
+void ff_prime_consts(FFPrime out[3]);
+int ff_plan_build(FFPlan *pl, uint32_t L, int bw, int lg, int r3, uint64_t conv, int S_log, int tile_words);
+hipError_t ff_launch_forward(const FFPlan *pl, const FFXform *x_dev, int nx, hipStream_t st);
+hipError_t ff_launch_conv(const FFPlan *pl, const FFXform *a_dev, const FFXform *b_dev, const FFConv *c_dev,
+                          int nc, hipStream_t st);
+hipError_t ff_launch_inverse(const FFPlan *pl, const FFXform *x_dev, int nx, hipStream_t st);
+
+// ff_crt.hip
+struct FFCrtJob {
+    const uint32_t *y;   // residues [3][L]
+    uint64_t *out;       // limbs: room for max(nout, nchunks * bw + 2)
+    FFChunk *chunks;     // nchunks records
+    unsigned __int128 *cin;  // nchunks + 1 incoming carries
+    uint64_t nout;
+    uint64_t conv;
+    uint32_t L;
+    int bw;
+};
+hipError_t ff_launch_crt(const FFCrtJob *jobs_dev, int njobs, uint64_t max_chunks, hipStream_t st);
+hipError_t ff_launch_normlen(const uint64_t *d, uint64_t n, uint64_t *len_out, hipStream_t st);
+}
+
+#define FF_CHECK(expr)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            ff_fail(__FILE__, __LINE__, #expr, e_);                                         \
+        }                                                                                   \
+    } while (0)
+
+extern "C" void ff_fail(const char *file, int line, const char *what, hipError_t e);

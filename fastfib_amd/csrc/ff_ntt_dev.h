@@ -1,0 +1,215 @@
+// Device building blocks of the transforms: radix-2 DIF/DIT levels on
+// groups of up to 16 elements held in registers, run as "rounds" over an LDS
+// tile, and the radix-3 layer.
+//
+// Tile layout: nsub stacked sub-transforms of M = 2^rho rows each, C columns;
+// element (sub s, row v, column c) lives at lds[(s * M + v) * ld + c]. A
+// round covers the row bits [b_lo, b_lo + NL): each group is the 2^NL rows
+// that differ only in those bits, for one (s, c) and one value of the other
+// row bits. Forward rounds run the DIF levels b = rho-1 .. 0 (Gentleman-Sande,
+// natural order in, bit-reversed order out); inverse rounds run DIT levels
+// b = 0 .. rho-1 with inverse roots (bit-reversed in, natural out), so that
+// inverse(forward(x)) = M * x. Twiddles of level b are rt[2^b + j], j = row
+// mod 2^b, the table layout of ntt_tables.c:204-209 (reference).
+#pragma once
+#include "ff_arith.h"
+
+template <int NL>
+__device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, const uint32_t *rt, const uint32_t *rtp,
+                                             uint32_t p) {
+#pragma unroll
+    for (int lb = NL - 1; lb >= 0; lb--) {
+        const int b = b_lo + lb;
+        const int h = 1 << b;
+#pragma unroll
+        for (int j = 0; j < (1 << NL); j++) {
+            if (j & (1 << lb)) continue;
+            const int k = j | (1 << lb);
+            uint32_t x = v[j], y = v[k];
+            v[j] = ff_add(x, y, p);
+            if (b == 0) {
+                v[k] = ff_sub(x, y, p);
+            } else {
+                const int t = low | ((j & ((1 << lb) - 1)) << b_lo);
+                v[k] = ff_mont_w(x - y + p, rt[h + t], rtp[h + t], p);
+            }
+        }
+    }
+}
+
+template <int NL>
+__device__ __forceinline__ void ff_dit_group(uint32_t *v, int low, int b_lo, const uint32_t *irt, const uint32_t *irtp,
+                                             uint32_t p) {
+#pragma unroll
+    for (int lb = 0; lb < NL; lb++) {
+        const int b = b_lo + lb;
+        const int h = 1 << b;
+#pragma unroll
+        for (int j = 0; j < (1 << NL); j++) {
+            if (j & (1 << lb)) continue;
+            const int k = j | (1 << lb);
+            uint32_t x = v[j], y = v[k];
+            if (b != 0) {
+                const int t = low | ((j & ((1 << lb) - 1)) << b_lo);
+                y = ff_mont_w(y, irt[h + t], irtp[h + t], p);
+            }
+            v[j] = ff_add(x, y, p);
+            v[k] = ff_sub(x, y, p);
+        }
+    }
+}
+
+struct FFTile {
+    uint32_t *s;  // LDS base
+    int ld;       // row stride in words
+    int C;        // columns (power of 2)
+    int logC;
+    int rho;      // rows per sub-transform = 2^rho
+    int nsub;     // stacked sub-transforms (1, or 3 under a radix-3 layer)
+    int pad;      // 1: one spare word after every 16 rows (single-column tiles)
+};
+
+// LDS index of row position pos (= sub * 2^rho + row), column c.
+__device__ __forceinline__ int ff_pos_addr(const FFTile &t, int pos, int c) {
+    return pos * t.ld + c + (t.pad ? (pos >> 4) : 0);
+}
+
+__device__ __forceinline__ int ff_tile_addr(const FFTile &t, int sub, int row, int c) {
+    return ff_pos_addr(t, (sub << t.rho) + row, c);
+}
+
+// One round of NL levels over the whole tile (all threads of the block).
+template <int NL, bool INV>
+__device__ void ff_round(const FFTile &t, int b_lo, const uint32_t *rt, const uint32_t *rtp, uint32_t p) {
+    const int G = 1 << NL;
+    const int per_sub = (1 << (t.rho - NL)) << t.logC;  // groups per sub-transform
+    const int ngroups = per_sub * t.nsub;
+    for (int gi = threadIdx.x; gi < ngroups; gi += blockDim.x) {
+        const int sub = gi / per_sub;
+        const int g = gi - sub * per_sub;
+        const int c = g & (t.C - 1);
+        const int rest = g >> t.logC;
+        const int low = rest & ((1 << b_lo) - 1);
+        const int base = low | ((rest >> b_lo) << (b_lo + NL));
+        uint32_t v[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) v[j] = t.s[ff_tile_addr(t, sub, base | (j << b_lo), c)];
+        if (INV) ff_dit_group<NL>(v, low, b_lo, rt, rtp, p);
+        else ff_dif_group<NL>(v, low, b_lo, rt, rtp, p);
+#pragma unroll
+        for (int j = 0; j < G; j++) t.s[ff_tile_addr(t, sub, base | (j << b_lo), c)] = v[j];
+    }
+}
+
+// All DIF levels (forward) or DIT levels (inverse) of the tile's columns.
+template <bool INV>
+__device__ void ff_tile_transform(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, uint32_t p) {
+    if (t.rho == 0) return;
+    if (!INV) {
+        // DIF: top bits first, 4 levels per round, the remainder last
+        int b = t.rho;
+        while (b > 0) {
+            const int nl = b >= 4 ? 4 : b;
+            b -= nl;
+            __syncthreads();
+            switch (nl) {
+            case 4: ff_round<4, false>(t, b, rt, rtp, p); break;
+            case 3: ff_round<3, false>(t, b, rt, rtp, p); break;
+            case 2: ff_round<2, false>(t, b, rt, rtp, p); break;
+            default: ff_round<1, false>(t, b, rt, rtp, p); break;
+            }
+        }
+    } else {
+        // DIT: bottom bits first, mirror of the DIF rounds
+        int b = 0;
+        const int first = t.rho % 4 == 0 ? 4 : t.rho % 4;
+        int nl = first;
+        while (b < t.rho) {
+            __syncthreads();
+            switch (nl) {
+            case 4: ff_round<4, true>(t, b, rt, rtp, p); break;
+            case 3: ff_round<3, true>(t, b, rt, rtp, p); break;
+            case 2: ff_round<2, true>(t, b, rt, rtp, p); break;
+            default: ff_round<1, true>(t, b, rt, rtp, p); break;
+            }
+            b += nl;
+            nl = 4;
+        }
+    }
+    __syncthreads();
+}
+
+// Radix-3 layer over a tile of nsub = 3 stacked thirds (ntt_radix3.c:8-22,
+// reference): forward, for n1 < M with a_r = (third r, row n1):
+//   y0 = a0 + a1 + a2, d = w3 (a1 - a2),
+//   y1 = (a0 - a2 + d) wR^n1, y2 = (a0 - a1 - d) wR^2n1
+// Inverse (roots inverted): t1 = z1 wR^-n1, t2 = z2 wR^-2n1,
+//   x0 = z0 + t1 + t2, d = w3^-1 (t1 - t2), x1 = z0 - t2 + d, x2 = z0 - t1 - d.
+// rstep = FF_NBASE / R, so wR^e = w_NBASE^(e * rstep).
+template <bool INV>
+__device__ void ff_radix3_layer(const FFTile &t, const FFTables *T, int pi, const FFPrime &P, uint32_t rstep) {
+    const uint32_t p = P.p;
+    const int M = 1 << t.rho;
+    const int n = M << t.logC;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int c = i & (t.C - 1);
+        const int n1 = i >> t.logC;
+        const int a0i = ff_tile_addr(t, 0, n1, c), a1i = ff_tile_addr(t, 1, n1, c), a2i = ff_tile_addr(t, 2, n1, c);
+        uint32_t e1 = (uint32_t)n1 * rstep;
+        uint32_t e2 = 2u * (uint32_t)n1 * rstep;
+        if (INV) {
+            e1 = e1 ? FF_NBASE - e1 : 0;
+            e2 = e2 ? FF_NBASE - e2 : 0;
+        }
+        const uint32_t tw1 = ff_root(T, pi, e1, p, P.pinv), tw2 = ff_root(T, pi, e2, p, P.pinv);
+        const uint32_t a0 = t.s[a0i], a1 = t.s[a1i], a2 = t.s[a2i];
+        if (!INV) {
+            const uint32_t y0 = ff_add(ff_add(a0, a1, p), a2, p);
+            const uint32_t d = ff_mont_w(a1 - a2 + p, P.w3, P.w3p, p);
+            const uint32_t y1 = ff_add(ff_sub(a0, a2, p), d, p);
+            const uint32_t y2 = ff_sub(ff_sub(a0, a1, p), d, p);
+            t.s[a0i] = y0;
+            t.s[a1i] = ff_mont(y1, tw1, p, P.pinv);
+            t.s[a2i] = ff_mont(y2, tw2, p, P.pinv);
+        } else {
+            const uint32_t t1 = ff_mont(a1, tw1, p, P.pinv), t2 = ff_mont(a2, tw2, p, P.pinv);
+            const uint32_t d = ff_mont_w(t1 - t2 + p, P.iw3, P.iw3p, p);
+            t.s[a0i] = ff_add(ff_add(a0, t1, p), t2, p);
+            t.s[a1i] = ff_add(ff_sub(a0, t2, p), d, p);
+            t.s[a2i] = ff_sub(ff_sub(a0, t1, p), d, p);
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int ff_bitrev(int x, int bits) {
+    return bits ? (int)(__brev((unsigned)x) >> (32 - bits)) : 0;
+}
+
+// Frequency held by tile row position pos after the forward row transform:
+// rev(v) for a power-of-2 R; 3 rev(v) + r for R = 3M (pos = r M + v).
+__device__ __forceinline__ uint32_t ff_row_freq(int pos, int rho, int r3) {
+    if (!r3) return (uint32_t)ff_bitrev(pos, rho);
+    const int r = pos >> rho, v = pos & ((1 << rho) - 1);
+    return 3u * (uint32_t)ff_bitrev(v, rho) + (uint32_t)r;
+}
+
+// b-bit coefficient k of an operand stored as nw 32-bit words (the limbs,
+// little-endian), reduced mod p; zero past the words (src_load8,
+// ntt_kernels.c:40-74, reference).
+__device__ __forceinline__ uint32_t ff_load_coef(const uint32_t *w, uint32_t nw, uint64_t k, int bw, const FFPrime &P) {
+    const uint64_t bit = k * (uint64_t)bw;
+    const uint64_t wi = bit >> 5;
+    const int sh = (int)(bit & 31);
+    if (wi >= nw) return 0;
+    uint64_t lo = w[wi];
+    if (wi + 1 < nw) lo |= (uint64_t)w[wi + 1] << 32;
+    uint64_t v = lo >> sh;
+    if (sh + bw > 64 && wi + 2 < nw) v |= (uint64_t)w[wi + 2] << (64 - sh);
+    v &= (bw >= 64 ? ~0ull : ((1ull << bw) - 1));
+    const uint32_t lo32 = ff_red32((uint32_t)v, P.p);
+    const uint32_t hi = (uint32_t)(v >> 32);
+    if (!hi) return lo32;
+    return ff_add(lo32, ff_mont(hi, P.r2, P.p, P.pinv), P.p);
+}

@@ -1,0 +1,125 @@
+/*
+ * fastfib_amd: exact big-integer multiplication and F(n) on AMD Instinct
+ * MI355X (gfx950) - the C-ABI of libfastfib_amd.so.
+ *
+ * The first group keeps the reference's C surface (shwwwa/fastfib, a unity
+ * build whose functions are all static: common.h:8-11, fastfib.c:16-21), so
+ * a caller of big_mul / big_sqr / big_mul_ntt / fibonacci can switch by
+ * name: same Big layout, same squaring signal (a == b pointer identity,
+ * mul_basic.c:177, mul_ntt.c:36,125), same ownership (results are new and
+ * freed with ff_big_free), and the same failure behaviour (a message on
+ * stderr and exit(1), as alloc.c:142-146). All arithmetic runs on the GPU.
+ *
+ * The second group is new: device-resident numbers, batched products and
+ * status codes, for callers that keep operands in HBM between products.
+ */
+#ifndef FASTFIB_AMD_H
+#define FASTFIB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define FF_API __attribute__((visibility("default")))
+#else
+#define FF_API
+#endif
+
+/* value = sum d[i] * 2^(64 i); normalized: n >= 1, no leading zero limb,
+   zero is {0} with n = 1. Layout-identical to the reference's Big
+   (src/common.h:29-33). */
+typedef struct {
+    uint64_t *d;
+    size_t n;
+} Big;
+
+/* ------------------------------------------------------------------------
+ * Drop-in surface. Host pointers in and out; failures abort the process.
+ * --------------------------------------------------------------------- */
+
+/* Replaces ntt_init (src/ntt_tables.c:268): prepares the device and the
+   root tables for products of up to max_limbs limbs. Idempotent and
+   grow-only; every other entry point also initializes on first use. */
+FF_API void ff_ntt_init(size_t max_limbs);
+
+/* Replaces big_mul (src/mul_basic.c:64): a * b as a new Big; a == b squares. */
+FF_API Big ff_big_mul(const Big *a, const Big *b);
+
+/* Replaces big_sqr (src/mul_basic.c:109): a * a as a new Big. */
+FF_API Big ff_big_sqr(const Big *a);
+
+/* Replaces big_mul_ntt (src/mul_ntt.c:97): a * b by the three-prime NTT
+   with the reference's width and length rule (mul_ntt.c:103-118); a == b
+   squares with one forward transform. */
+FF_API Big ff_big_mul_ntt(const Big *a, const Big *b);
+
+/* Replaces fibonacci (src/fib.c:211): F(n) as a new Big, by fast doubling
+   with every product and linear step on the GPU. */
+FF_API Big ff_fibonacci(uint64_t n);
+
+/* Replaces big_free (src/bigint.c:26): frees x's limbs, sets d = NULL, n = 0. */
+FF_API void ff_big_free(Big *x);
+
+/* ------------------------------------------------------------------------
+ * Device-resident API (no counterpart in the reference). Returns FF_OK or a
+ * negative status; ff_last_error() describes the last failure.
+ * --------------------------------------------------------------------- */
+
+#define FF_OK 0
+#define FF_ERR_DEVICE (-1)      /* a HIP call failed */
+#define FF_ERR_NOMEM (-2)       /* device memory exhausted */
+#define FF_ERR_ARG (-3)         /* invalid argument */
+#define FF_ERR_UNSUPPORTED (-4) /* size beyond what this build handles */
+
+/* A number in device memory: d has room for cap limbs, n is its
+   normalized length and bits its bit length (0 for zero). */
+typedef struct {
+    uint64_t *d;
+    size_t n;
+    size_t cap;
+    uint64_t bits;
+} ff_dbig;
+
+/* Selects the device (-1: the current one), creates the library's stream
+   and uploads the root tables. */
+FF_API int ff_init(int device);
+
+/* The HIP stream (hipStream_t) all device work of the library is queued on. */
+FF_API void *ff_stream(void);
+
+/* Waits for all queued device work. */
+FF_API int ff_sync(void);
+
+FF_API const char *ff_last_error(void);
+
+FF_API int ff_dbig_alloc(ff_dbig *x, size_t cap);
+FF_API void ff_dbig_release(ff_dbig *x);
+/* Copies n host limbs in (normalizes n and computes bits). */
+FF_API int ff_dbig_upload(ff_dbig *x, const uint64_t *limbs, size_t n);
+/* Copies x's n limbs out to host memory with room for cap limbs. */
+FF_API int ff_dbig_download(const ff_dbig *x, uint64_t *limbs, size_t cap);
+
+/* out = a * b on the device (b == a or b == NULL squares); out is
+   (re)allocated as needed. Same schedule rule as ff_big_mul_ntt. */
+FF_API int ff_dmul(ff_dbig *out, const ff_dbig *a, const ff_dbig *b);
+
+/* out = F(n), computed and left in device memory. */
+FF_API int ff_dfibonacci(uint64_t n, ff_dbig *out);
+
+/* Schedule knobs (the reference's compile-time tunables, Appendix of its
+   README: -DNAME=value, here set at run time). Names:
+     "inner_log"   log2 of the LDS-resident inner block (4..12, default 12)
+     "tile_words"  LDS words of an outer-pass tile (1024..16384, default 16384)
+     "force_len"   transform length to use instead of the shortest (0: off)
+   Returns FF_ERR_ARG for an unknown name or value. */
+FF_API int ff_set_option(const char *name, long long value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -66,6 +66,13 @@ def lib() -> ctypes.CDLL:
     L.ff_dmul.argtypes = [ctypes.POINTER(DBig), ctypes.POINTER(DBig), ctypes.POINTER(DBig)]
     L.ff_dfibonacci.argtypes = [ctypes.c_uint64, ctypes.POINTER(DBig)]
     L.ff_set_option.argtypes = [ctypes.c_char_p, ctypes.c_longlong]
+    L.ff_fib_step_linear.argtypes = [ctypes.POINTER(Big), ctypes.POINTER(Big), ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(Big), ctypes.POINTER(Big)]
+    L.ff_last_factors.argtypes = [ctypes.POINTER(Big), ctypes.POINTER(Big), ctypes.POINTER(Big),
+                                  ctypes.POINTER(Big)]
+    L.ff_prof_enable.argtypes = [ctypes.c_int]
+    L.ff_prof_read.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_long),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 
@@ -134,6 +141,41 @@ def big_mul_ntt(a, b=None) -> np.ndarray:
 
 def fibonacci(n: int) -> np.ndarray:
     return _take(lib().ff_fibonacci(n))
+
+
+def fib_step_linear(s1, s2, k_odd: int, bit: int) -> tuple[np.ndarray, np.ndarray]:
+    """ff_fib_step_linear: (lo, hi) from s1 = F(k)^2, s2 = F(k-1)^2."""
+    A, ka = _as_big(s1)
+    B, kb = _as_big(s2)
+    lo, hi = Big(), Big()
+    lib().ff_fib_step_linear(ctypes.byref(A), ctypes.byref(B), int(k_odd), int(bit), ctypes.byref(lo),
+                             ctypes.byref(hi))
+    return _take(lo), _take(hi)
+
+
+def last_factors(u, v, with_y: bool = True):
+    """ff_last_factors: x = 2u + v and, if with_y, y = 2u - v."""
+    U, ku = _as_big(u)
+    V, kv = _as_big(v)
+    x, y = Big(), Big()
+    lib().ff_last_factors(ctypes.byref(U), ctypes.byref(V), ctypes.byref(x), ctypes.byref(y) if with_y else None)
+    return (_take(x), _take(y)) if with_y else (_take(x), None)
+
+
+def kernel_stats() -> dict:
+    """Per-kernel-kind launches, milliseconds and algorithmic bytes booked by
+    the live timer (enable with lib().ff_prof_enable(1))."""
+    out = {}
+    k = 0
+    while True:
+        name, n = ctypes.c_char_p(), ctypes.c_long()
+        ms, b = ctypes.c_double(), ctypes.c_double()
+        if lib().ff_prof_read(k, ctypes.byref(name), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)) != 0:
+            break
+        if n.value:
+            out[name.value.decode()] = {"launches": n.value, "ms": ms.value, "bytes": b.value}
+        k += 1
+    return out
 
 
 class Device:

@@ -1,202 +1,27 @@
-// Host side of fastfib_amd: device memory, product schedules, the product
-// pipeline (forward passes -> inner convolution -> inverse passes -> CRT ->
-// carries), device-resident fast doubling, and the C-ABI of
-// include/fastfib_amd.h.
-#include <stdarg.h>
+// Host side of fastfib_amd: product schedules, the product pipeline
+// (forward passes -> inner convolution -> inverse passes -> CRT + carries),
+// device-resident fast doubling, and the C-ABI of include/fastfib_amd.h.
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
-#include <map>
 #include <mutex>
-#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "../../include/fastfib_amd.h"
 #include "ff_internal.h"
 #include "ff_limbs.h"
+#include "ff_runtime.h"
 
-// ------------------------------------------------------------------ state
-
-static int g_inited = 0;
-static int g_device = 0;
-static hipStream_t g_st = nullptr;
-static std::recursive_mutex g_mu;  // the drop-in calls may come from several threads
-static char g_err[512];
-
-static int g_inner_log = 12;
-static int g_tile_words = 16384;
-static uint64_t g_force_len = 0;
-
-static void set_err(const char *fmt, ...) {
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof g_err, fmt, ap);
-    va_end(ap);
-}
-
-extern "C" const char *ff_last_error(void) { return g_err; }
-
-// Drop-in failure behaviour: a message and exit(1) (alloc.c:142-146).
-extern "C" void ff_fail(const char *file, int line, const char *what, hipError_t e) {
-    fprintf(stderr, "fastfib_amd: %s failed at %s:%d: %s\n", what, file, line, hipGetErrorString(e));
-    exit(1);
-}
-
-// ------------------------------------------------------------------ device memory
-// Freed blocks are kept by size and reused (stream-ordered: every user of a
-// block runs on g_st, so a block freed after enqueueing its last use can be
-// handed to the next caller at once). hipMalloc only on a miss.
-
-static std::multimap<size_t, void *> g_free;
-static std::unordered_map<void *, size_t> g_sizes;
-static size_t g_pool_bytes = 0;
-
-static size_t size_class(size_t b) {
-    if (b < 4096) return 4096;
-    // round up to 1/8 of the next power of two
-    size_t p = 1;
-    while (p < b) p <<= 1;
-    size_t step = p / 8;
-    return (b + step - 1) / step * step;
-}
-
-static void *dalloc(size_t bytes) {
-    const size_t sz = size_class(bytes);
-    auto it = g_free.lower_bound(sz);
-    if (it != g_free.end() && it->first <= sz + sz / 2) {
-        void *p = it->second;
-        g_free.erase(it);
-        return p;
-    }
-    void *p = nullptr;
-    hipError_t e = hipMalloc(&p, sz);
-    if (e != hipSuccess) {
-        // drop the cache and retry once
-        (void)hipStreamSynchronize(g_st);
-        for (auto &kv : g_free) {
-            (void)hipFree(kv.second);
-            g_pool_bytes -= kv.first;
-            g_sizes.erase(kv.second);
-        }
-        g_free.clear();
-        (void)hipGetLastError();
-        e = hipMalloc(&p, sz);
-        if (e != hipSuccess) {
-            fprintf(stderr, "out of memory (%zu bytes)\n", bytes);
-            exit(1);
-        }
-    }
-    g_sizes[p] = sz;
-    g_pool_bytes += sz;
-    return p;
-}
-
-static void dfree(void *p) {
-    if (!p) return;
-    auto it = g_sizes.find(p);
-    if (it == g_sizes.end()) return;
-    g_free.emplace(it->second, p);
-}
-
-// Pinned staging ring for the small descriptor arrays the kernels read.
-static char *g_ring = nullptr;
-static const size_t kRingBytes = 8u << 20;
-static size_t g_ring_pos = 0;
-static uint64_t *g_res_h = nullptr;  // pinned: normalized lengths read back
-static const int kResMax = 4096;
-
-template <class T>
-static T *stage(const T *src, size_t n) {
-    const size_t bytes = (n * sizeof(T) + 255) / 256 * 256;
-    if (bytes > kRingBytes) {
-        set_err("descriptor array too large");
-        return nullptr;
-    }
-    if (g_ring_pos + bytes > kRingBytes) {
-        FF_CHECK(hipStreamSynchronize(g_st));
-        g_ring_pos = 0;
-    }
-    memcpy(g_ring + g_ring_pos, src, n * sizeof(T));
-    T *d = (T *)dalloc(bytes);
-    FF_CHECK(hipMemcpyAsync(d, g_ring + g_ring_pos, bytes, hipMemcpyHostToDevice, g_st));
-    g_ring_pos += bytes;
-    return d;
-}
-
-extern "C" int ff_init(int device) {
-    std::lock_guard<std::recursive_mutex> lk(g_mu);
-    if (g_inited) return FF_OK;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-        set_err("no HIP device");
-        return FF_ERR_DEVICE;
-    }
-    if (device >= 0) {
-        if (hipSetDevice(device) != hipSuccess) {
-            set_err("hipSetDevice(%d) failed", device);
-            return FF_ERR_DEVICE;
-        }
-    }
-    (void)hipGetDevice(&g_device);
-    if (hipStreamCreateWithFlags(&g_st, hipStreamNonBlocking) != hipSuccess) {
-        set_err("stream creation failed");
-        return FF_ERR_DEVICE;
-    }
-    if (hipHostMalloc((void **)&g_ring, kRingBytes, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&g_res_h, kResMax * 2 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
-        set_err("pinned staging allocation failed");
-        return FF_ERR_NOMEM;
-    }
-    if (ff_tables_init() != 0) {
-        set_err("table upload failed");
-        return FF_ERR_DEVICE;
-    }
-    g_inited = 1;
-    return FF_OK;
-}
-
-static void ensure_init(void) {
-    if (!g_inited && ff_init(-1) != FF_OK) {
-        fprintf(stderr, "fastfib_amd: %s\n", g_err);
-        exit(1);
-    }
-}
-
-extern "C" void *ff_stream(void) {
-    ensure_init();
-    return (void *)g_st;
-}
-
-extern "C" int ff_sync(void) {
-    ensure_init();
-    return hipStreamSynchronize(g_st) == hipSuccess ? FF_OK : FF_ERR_DEVICE;
-}
-
-extern "C" int ff_set_option(const char *name, long long v) {
-    std::lock_guard<std::recursive_mutex> lk(g_mu);
-    if (!strcmp(name, "inner_log")) {
-        if (v < 4 || v > 12) return FF_ERR_ARG;
-        g_inner_log = (int)v;
-    } else if (!strcmp(name, "tile_words")) {
-        if (v < 1024 || v > 16384) return FF_ERR_ARG;
-        g_tile_words = (int)v;
-    } else if (!strcmp(name, "force_len")) {
-        if (v < 0) return FF_ERR_ARG;
-        g_force_len = (uint64_t)v;
-    } else {
-        return FF_ERR_ARG;
-    }
-    return FF_OK;
-}
+typedef unsigned __int128 u128;
 
 // ------------------------------------------------------------------ schedule
 // The reference's choice of coefficient width and transform length
 // (mul_ntt.c:55-118): widths 32..35 while min(ca, cb) * 2^(2w) < p1 p2 p3,
 // the shortest L in {2^k, 3 * 2^k} >= ca + cb - 1 (at least 16), the
 // narrowest width among equal lengths.
-
-typedef unsigned __int128 u128;
 
 static int width_ok(uint64_t bits_a, uint64_t bits_b, unsigned w) {
     const u128 M = (u128)FF_P1 * FF_P2 * FF_P3;
@@ -234,9 +59,9 @@ static ProdShape product_shape(uint64_t bits_a, uint64_t bits_b) {
     if (s.L == 0 || s.L > (1u << 25)) s.split = 1;
     if (!s.split && g_force_len) {
         // a forced length must be supported (2^k or 3*2^k <= 2^25) and long enough
-        uint64_t F = g_force_len;
-        int r3 = (F % 3 == 0);
-        uint64_t m = r3 ? F / 3 : F;
+        const uint64_t F = g_force_len;
+        const int r3 = (F % 3 == 0);
+        const uint64_t m = r3 ? F / 3 : F;
         if ((m & (m - 1)) == 0 && F >= s.conv && F >= 16 && F <= (1u << 25)) {
             s.L = F;
             s.r3 = r3;
@@ -254,6 +79,77 @@ static uint64_t out_room(const ProdShape &s, uint64_t na, uint64_t nb) {
     return a > b ? a : b;
 }
 
+// ------------------------------------------------------------------ length slots
+// Device words where kernels book a result's normalized length and top-limb
+// bit count as ((n << 8) | bits of the top limb), by atomicMax.
+
+static unsigned long long *g_slots = nullptr;
+static int g_slot_cap = 0, g_slot_used = 0;
+
+static void slots_reset(int need) {
+    if (need > g_slot_cap) {
+        if (g_slots) ff_dfree(g_slots);
+        g_slot_cap = need < 1024 ? 1024 : need;
+        g_slots = (unsigned long long *)ff_dalloc(g_slot_cap * sizeof(unsigned long long));
+    }
+    g_slot_used = 0;
+}
+
+static int slot_new(void) {
+    if (g_slot_used >= g_slot_cap) {
+        fprintf(stderr, "fastfib_amd: length slots exhausted\n");
+        exit(1);
+    }
+    return g_slot_used++;
+}
+
+static void slots_read(int s0, int n, uint64_t *out) {
+    uint64_t *h = ff_pinned_results((size_t)n);
+    FF_CHECK(hipMemcpyAsync(h, g_slots + s0, (size_t)n * 8, hipMemcpyDeviceToHost, g_st));
+    FF_CHECK(hipStreamSynchronize(g_st));
+    memcpy(out, h, (size_t)n * 8);
+}
+
+static uint64_t key_of_bits(uint64_t bits) {
+    if (bits == 0) return 0;
+    const uint64_t n = (bits + 63) / 64;
+    return (n << 8) | (bits - 64 * (n - 1));
+}
+
+static void key_decode(uint64_t key, uint64_t *n, uint64_t *bits) {
+    if (key == 0) {
+        *n = 1, *bits = 0;
+        return;
+    }
+    *n = key >> 8;
+    *bits = 64 * (*n - 1) + (key & 0xff);
+}
+
+// ------------------------------------------------------------------ numbers
+
+struct DNum {
+    uint64_t *d;
+    uint64_t n, bits, cap;  // n limbs in use (top may be zero only if bits says so)
+    int slot;               // where a kernel booked its length (-1: none)
+};
+
+static DNum dnum_alloc(uint64_t cap) {
+    DNum x{(uint64_t *)ff_dalloc(cap * 8), 1, 0, cap, -1};
+    return x;
+}
+
+static DNum dnum_u64(uint64_t v) {
+    DNum x = dnum_alloc(1);
+    FF_CHECK(ff_launch_fill(x.d, 1, v, g_st));
+    x.bits = v ? 64 - __builtin_clzll(v) : 0;
+    return x;
+}
+
+static void dnum_free(DNum &x) {
+    ff_dfree(x.d);
+    x.d = nullptr;
+}
+
 // ------------------------------------------------------------------ products
 
 struct MulItem {
@@ -263,94 +159,80 @@ struct MulItem {
     uint64_t nb, bits_b;
     uint64_t *out;      // room for out_room() limbs
     uint64_t conv;      // product coefficients at the group's width
+    int slot;
 };
 
-// Runs products that share one schedule, as one launch per pass.
-static void mul_group(MulItem *it, int cnt, const ProdShape &s) {
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Runs up to FF_MAXB products that share one schedule, one launch per pass.
+static void mul_group(const MulItem *it, int cnt, const ProdShape &s) {
     FFPlan pl;
     if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, g_inner_log, g_tile_words) != 0) {
         fprintf(stderr, "fastfib_amd: no schedule for L=%llu\n", (unsigned long long)s.L);
         exit(1);
     }
     const size_t words = 3 * (size_t)s.L;
-    std::vector<FFXform> fwd, xa(cnt), xb(cnt), inv(cnt);
-    std::vector<FFConv> cv(cnt);
-    std::vector<FFCrtJob> crt(cnt);
+    FFXformSet fwd[2] = {}, inv = {};
+    FFConvSet cv = {};
+    FFCrtSet crt = {};
+    int nfwd = 0;
     std::vector<void *> tmp;
     uint64_t max_chunks = 0;
+    double opbytes = 0, conv_total = 0, out_limbs = 0;
+    int ngeneral = 0;
     for (int i = 0; i < cnt; i++) {
-        uint32_t *fa = (uint32_t *)dalloc(words * 4);
-        uint32_t *fb = it[i].b ? (uint32_t *)dalloc(words * 4) : nullptr;
-        tmp.push_back(fa);
-        if (fb) tmp.push_back(fb);
-        xa[i] = FFXform{(const uint32_t *)it[i].a, (uint32_t)(2 * it[i].na), fa};
-        xb[i] = FFXform{it[i].b ? (const uint32_t *)it[i].b : nullptr, (uint32_t)(2 * it[i].nb), fb};
-        fwd.push_back(xa[i]);
-        if (fb) fwd.push_back(xb[i]);
-        cv[i] = FFConv{fa, fb};
-        inv[i] = FFXform{nullptr, 0, fa};
         const uint64_t nch = (it[i].conv + 63) / 64;
+        const uint64_t nwg = (nch + 63) / 64;
+        const size_t o_fb = align256(words * 4);
+        const size_t o_ch = o_fb + (it[i].b ? align256(words * 4) : 0);
+        const size_t o_cin = o_ch + align256(nch * sizeof(FFChunk));
+        const size_t o_wg = o_cin + align256((nch + 1) * sizeof(u128));
+        const size_t o_mode = o_wg + align256(nwg * 4);
+        char *base = (char *)ff_dalloc(o_mode + 256);
+        tmp.push_back(base);
+        uint32_t *fa = (uint32_t *)base;
+        uint32_t *fb = it[i].b ? (uint32_t *)(base + o_fb) : nullptr;
+        const uint32_t *asrc = (const uint32_t *)it[i].a, *bsrc = (const uint32_t *)it[i].b;
+        fwd[nfwd / FF_MAXB].x[nfwd % FF_MAXB] = FFXform{asrc, (uint32_t)(2 * it[i].na), fa};
+        nfwd++;
+        if (fb) {
+            fwd[nfwd / FF_MAXB].x[nfwd % FF_MAXB] = FFXform{bsrc, (uint32_t)(2 * it[i].nb), fb};
+            nfwd++;
+        }
+        cv.c[i] = FFConv{asrc, bsrc, (uint32_t)(2 * it[i].na), (uint32_t)(2 * it[i].nb), fa, fb};
+        inv.x[i] = FFXform{nullptr, 0, fa};
         max_chunks = nch > max_chunks ? nch : max_chunks;
-        FFChunk *ch = (FFChunk *)dalloc(nch * sizeof(FFChunk));
-        u128 *cin = (u128 *)dalloc((nch + 1) * sizeof(u128));
-        tmp.push_back(ch);
-        tmp.push_back(cin);
         const uint64_t nout = it[i].na + (it[i].b ? it[i].nb : it[i].na) + 1;
-        crt[i] = FFCrtJob{fa, it[i].out, ch, cin, nout, it[i].conv, (uint32_t)s.L, (int)s.bw};
+        crt.j[i] = FFCrtJob{fa,
+                            it[i].out,
+                            (FFChunk *)(base + o_ch),
+                            (u128 *)(base + o_cin),
+                            (uint32_t *)(base + o_wg),
+                            (uint32_t *)(base + o_mode),
+                            g_slots + it[i].slot,
+                            nout,
+                            it[i].conv,
+                            (uint32_t)s.L,
+                            (int)s.bw};
+        opbytes += 8.0 * (it[i].na + (it[i].b ? it[i].nb : 0));
+        ngeneral += it[i].b ? 1 : 0;
+        conv_total += (double)it[i].conv;
+        out_limbs += (double)(nch * s.bw);
     }
-    FFXform *d_fwd = stage(fwd.data(), fwd.size());
-    FFXform *d_xa = stage(xa.data(), cnt);
-    FFXform *d_xb = stage(xb.data(), cnt);
-    FFXform *d_inv = stage(inv.data(), cnt);
-    FFConv *d_cv = stage(cv.data(), cnt);
-    FFCrtJob *d_crt = stage(crt.data(), cnt);
-    FF_CHECK(ff_launch_forward(&pl, d_fwd, (int)fwd.size(), g_st));
-    FF_CHECK(ff_launch_conv(&pl, d_xa, d_xb, d_cv, cnt, g_st));
-    FF_CHECK(ff_launch_inverse(&pl, d_inv, cnt, g_st));
-    FF_CHECK(ff_launch_crt(d_crt, cnt, max_chunks, g_st));
-    for (void *p : tmp) dfree(p);
-    dfree(d_fwd), dfree(d_xa), dfree(d_xb), dfree(d_inv), dfree(d_cv), dfree(d_crt);
+    for (int f = 0; f * FF_MAXB < nfwd; f++) {
+        const int nx = nfwd - f * FF_MAXB < FF_MAXB ? nfwd - f * FF_MAXB : FF_MAXB;
+        FF_CHECK(ff_launch_forward(&pl, &fwd[f], nx, opbytes * nx / nfwd, g_st));
+    }
+    FF_CHECK(ff_launch_conv(&pl, &cv, cnt, ngeneral, opbytes, g_st));
+    FF_CHECK(ff_launch_inverse(&pl, &inv, cnt, g_st));
+    FF_CHECK(ff_launch_crt(&crt, cnt, max_chunks, conv_total, out_limbs, g_st));
+    for (void *p : tmp) ff_dfree(p);
 }
 
-// Reads back the normalized lengths and bit counts of n arrays.
-static void read_lengths(const FFNormJob *jobs, int n, uint64_t *res) {
-    FFNormJob *dj = stage(jobs, n);
-    uint64_t *dres = (uint64_t *)dalloc(2 * n * sizeof(uint64_t));
-    FF_CHECK(ff_launch_normlen_multi(dj, n, dres, g_st));
-    FF_CHECK(hipMemcpyAsync(g_res_h, dres, 2 * n * sizeof(uint64_t), hipMemcpyDeviceToHost, g_st));
-    FF_CHECK(hipStreamSynchronize(g_st));
-    memcpy(res, g_res_h, 2 * n * sizeof(uint64_t));
-    dfree(dj);
-    dfree(dres);
-}
-
-struct DNum {
-    uint64_t *d;
-    uint64_t n, bits, cap;
-};
-
-static DNum dnum_alloc(uint64_t cap) {
-    DNum x{(uint64_t *)dalloc(cap * 8), 1, 0, cap};
-    return x;
-}
-
-static DNum dnum_u64(uint64_t v) {
-    DNum x = dnum_alloc(1);
-    FF_CHECK(hipMemcpyAsync(x.d, &v, 8, hipMemcpyHostToDevice, g_st));
-    FF_CHECK(hipStreamSynchronize(g_st));  // v lives on this stack frame
-    x.n = 1;
-    x.bits = v ? 64 - __builtin_clzll(v) : 0;
-    return x;
-}
-
-static void dnum_free(DNum &x) {
-    dfree(x.d);
-    x.d = nullptr;
-}
-
-// Products of device numbers: groups the items by schedule, runs each group,
-// and reads every result's length. Zero operands give zero.
-static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int cnt) {
+// out[i] = A[i] * B[i] (B[i] == nullptr or == A[i]: square). Results get a
+// length slot each; with pred (bit lengths known in advance) nothing waits
+// for the device, otherwise the lengths are read back.
+static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int cnt, const uint64_t *pred) {
     std::vector<MulItem> items(cnt);
     std::vector<ProdShape> shapes(cnt);
     std::vector<int> zero(cnt, 0);
@@ -358,11 +240,14 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
         const DNum *a = A[i], *b = B[i];
         const bool sq = (b == nullptr || b == a);
         const uint64_t bb = sq ? a->bits : b->bits;
+        out[i].slot = slot_new();
         if (a->bits == 0 || bb == 0) {
             zero[i] = 1;
-            out[i] = dnum_alloc(1);
-            FF_CHECK(hipMemsetAsync(out[i].d, 0, 8, g_st));
-            out[i].n = 1, out[i].bits = 0;
+            DNum z = dnum_u64(0);
+            z.slot = out[i].slot;
+            out[i] = z;
+            // the slot must read "zero": book nothing, but clear it
+            FF_CHECK(ff_launch_fill((uint64_t *)(g_slots + z.slot), 1, 0, g_st));
             continue;
         }
         shapes[i] = product_shape(a->bits, bb);
@@ -371,141 +256,285 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
             exit(1);
         }
         const uint64_t nb = sq ? a->n : b->n;
-        const uint64_t room = out_room(shapes[i], a->n, nb);
-        out[i] = dnum_alloc(room);
-        items[i] = MulItem{a->d, a->n, a->bits, sq ? nullptr : b->d, nb, bb, out[i].d, shapes[i].conv};
+        const int slot = out[i].slot;
+        out[i] = dnum_alloc(out_room(shapes[i], a->n, nb));
+        out[i].slot = slot;
+        items[i] = MulItem{a->d, a->n, a->bits, sq ? nullptr : b->d, nb, bb, out[i].d, shapes[i].conv, slot};
     }
-    // one group per (length, width): the passes only depend on those, the
-    // CRT reads each product's own conv
+    // one group per (length, width), up to FF_MAXB products per launch
     std::vector<int> done(cnt, 0);
     for (int i = 0; i < cnt; i++) {
         if (zero[i] || done[i]) continue;
         std::vector<MulItem> grp;
         for (int j = i; j < cnt; j++) {
             if (zero[j] || done[j]) continue;
-            if (shapes[j].L == shapes[i].L && shapes[j].bw == shapes[i].bw) {
+            if (shapes[j].L == shapes[i].L && shapes[j].bw == shapes[i].bw && (int)grp.size() < FF_MAXB) {
                 grp.push_back(items[j]);
                 done[j] = 1;
             }
         }
         mul_group(grp.data(), (int)grp.size(), shapes[i]);
     }
-    std::vector<FFNormJob> nj;
-    std::vector<int> which;
+    if (pred) {
+        for (int i = 0; i < cnt; i++) {
+            if (zero[i]) continue;
+            out[i].bits = pred[i];
+            out[i].n = pred[i] ? (pred[i] + 63) / 64 : 1;
+        }
+        return;
+    }
+    std::vector<uint64_t> keys(cnt);
     for (int i = 0; i < cnt; i++) {
-        if (zero[i]) continue;
-        const uint64_t nb = items[i].b ? items[i].nb : items[i].na;
-        nj.push_back(FFNormJob{out[i].d, items[i].na + nb + 1});
-        which.push_back(i);
+        uint64_t k;
+        slots_read(out[i].slot, 1, &k);
+        key_decode(k, &out[i].n, &out[i].bits);
     }
-    if (!nj.empty()) {
-        std::vector<uint64_t> res(2 * nj.size());
-        read_lengths(nj.data(), (int)nj.size(), res.data());
-        for (size_t k = 0; k < which.size(); k++) out[which[k]].n = res[2 * k], out[which[k]].bits = res[2 * k + 1];
-    }
-}
-
-static void mul2(const DNum *a, const DNum *b, DNum *out) {
-    const DNum *A[1] = {a}, *B[1] = {b};
-    mul_many(A, B, out, 1);
 }
 
 // ------------------------------------------------------------------ fast doubling
 
 // One linear step (fib.c:78-109): from s1 = F(k)^2, s2 = F(k-1)^2 makes
 // (lo, hi) = bit ? (F(2k), F(2k+1)) : (F(2k-1), F(2k)).
-static void fib_step(const DNum &s1, const DNum &s2, int k_odd, int bit, DNum *lo, DNum *hi) {
+static void fib_step(const DNum &s1, const DNum &s2, int k_odd, int bit, DNum *lo, DNum *hi, const uint64_t *pred) {
     const uint64_t len = s1.n + 1;  // s2 <= s1, and each result is < 4 s1 + 2
-    DNum A{}, B{}, C = dnum_alloc(len);
-    if (bit) B = dnum_alloc(len);
-    else A = dnum_alloc(len);
+    DNum K = dnum_alloc(len), C = dnum_alloc(len);
+    K.slot = slot_new();
+    C.slot = slot_new();
     const uint64_t nch = (len + FF_CH - 1) / FF_CH;
-    FFCarrySum *sums = (FFCarrySum *)dalloc(2 * nch * sizeof(FFCarrySum));
-    int8_t *cin = (int8_t *)dalloc(2 * nch);
-    FFStepArgs a{s1.d, s2.d, s1.n, s2.n, len, bit ? nullptr : A.d, bit ? B.d : nullptr, C.d, k_odd ? -2 : 2, sums};
+    const uint64_t nwg = (nch + FF_LIN_THREADS - 1) / FF_LIN_THREADS;
+    char *scr = (char *)ff_dalloc(align256(2 * nch * sizeof(FFCarrySum)) + align256(2 * nch) + align256(nwg * 4) + 256);
+    FFCarrySum *sums = (FFCarrySum *)scr;
+    int8_t *cin = (int8_t *)(scr + align256(2 * nch * sizeof(FFCarrySum)));
+    uint32_t *wgflag = (uint32_t *)((char *)cin + align256(2 * nch));
+    uint32_t *mode = (uint32_t *)((char *)wgflag + align256(nwg * 4));
+    FFStepArgs a{s1.d, s2.d, s1.n, s2.n, len, bit ? nullptr : K.d, bit ? K.d : nullptr, C.d, k_odd ? -2 : 2, sums,
+                 wgflag};
     FF_CHECK(ff_launch_step(&a, g_st));
-    FFCarryStreams cs{{bit ? B.d : A.d, C.d}, len, nch, 2, sums, cin};
+    const uint64_t lo_top = len >= 3 ? len - 3 : 0;
+    FFCarryStreams cs{{K.d, C.d}, len, nch, 2, sums, cin, wgflag, mode, {g_slots + K.slot, g_slots + C.slot},
+                      {lo_top, lo_top}};
     FF_CHECK(ff_launch_carry(&cs, g_st));
-    dfree(sums);
-    dfree(cin);
-    FFNormJob nj[2] = {{bit ? B.d : A.d, len}, {C.d, len}};
-    uint64_t res[4];
-    read_lengths(nj, 2, res);
-    DNum &K = bit ? B : A;
-    K.n = res[0], K.bits = res[1];
-    C.n = res[2], C.bits = res[3];
-    if (bit) *lo = C, *hi = B;
-    else *lo = A, *hi = C;
+    ff_dfree(scr);
+    if (pred) {
+        // pred[0], pred[1]: bit lengths of (lo, hi)
+        const uint64_t bk = bit ? pred[1] : pred[0], bc = bit ? pred[0] : pred[1];
+        K.bits = bk, K.n = (bk + 63) / 64;
+        C.bits = bc, C.n = (bc + 63) / 64;
+    } else {
+        uint64_t k[2];
+        slots_read(K.slot, 2, k);
+        key_decode(k[0], &K.n, &K.bits);
+        key_decode(k[1], &C.n, &C.bits);
+    }
+    if (bit) *lo = C, *hi = K;
+    else *lo = K, *hi = C;
 }
 
 // X = 2u + v and, if Y, Y = 2u - v (fib.c:145-161).
-static void last_factors(const DNum &u, const DNum &v, DNum *X, DNum *Y) {
+static void last_factors(const DNum &u, const DNum &v, DNum *X, DNum *Y, const uint64_t *pred) {
     const uint64_t len = (u.n > v.n ? u.n : v.n) + 1;
     *X = dnum_alloc(len);
-    if (Y) *Y = dnum_alloc(len);
+    X->slot = slot_new();
+    if (Y) {
+        *Y = dnum_alloc(len);
+        Y->slot = slot_new();
+    }
     const uint64_t nch = (len + FF_CH - 1) / FF_CH;
-    FFCarrySum *sums = (FFCarrySum *)dalloc(2 * nch * sizeof(FFCarrySum));
-    int8_t *cin = (int8_t *)dalloc(2 * nch);
-    FFLastArgs a{u.d, v.d, u.n, v.n, len, X->d, Y ? Y->d : nullptr, sums};
+    const uint64_t nwg = (nch + FF_LIN_THREADS - 1) / FF_LIN_THREADS;
+    char *scr = (char *)ff_dalloc(align256(2 * nch * sizeof(FFCarrySum)) + align256(2 * nch) + align256(nwg * 4) + 256);
+    FFCarrySum *sums = (FFCarrySum *)scr;
+    int8_t *cin = (int8_t *)(scr + align256(2 * nch * sizeof(FFCarrySum)));
+    uint32_t *wgflag = (uint32_t *)((char *)cin + align256(2 * nch));
+    uint32_t *mode = (uint32_t *)((char *)wgflag + align256(nwg * 4));
+    FFLastArgs a{u.d, v.d, u.n, v.n, len, X->d, Y ? Y->d : nullptr, sums, wgflag};
     FF_CHECK(ff_launch_last(&a, g_st));
-    FFCarryStreams cs{{X->d, Y ? Y->d : nullptr}, len, nch, Y ? 2 : 1, sums, cin};
+    const uint64_t lo_top = len >= 3 ? len - 3 : 0;
+    FFCarryStreams cs{{X->d, Y ? Y->d : nullptr}, len, nch, Y ? 2 : 1, sums, cin, wgflag, mode,
+                      {g_slots + X->slot, Y ? g_slots + Y->slot : nullptr}, {lo_top, lo_top}};
     FF_CHECK(ff_launch_carry(&cs, g_st));
-    dfree(sums);
-    dfree(cin);
-    FFNormJob nj[2] = {{X->d, len}, {Y ? Y->d : X->d, len}};
-    uint64_t res[4];
-    read_lengths(nj, Y ? 2 : 1, res);
-    X->n = res[0], X->bits = res[1];
-    if (Y) Y->n = res[2], Y->bits = res[3];
+    ff_dfree(scr);
+    if (pred) {
+        X->bits = pred[0], X->n = (pred[0] + 63) / 64;
+        if (Y) Y->bits = pred[1], Y->n = (pred[1] + 63) / 64;
+    } else {
+        uint64_t k[2];
+        slots_read(X->slot, Y ? 2 : 1, k);
+        key_decode(k[0], &X->n, &X->bits);
+        if (Y) key_decode(k[1], &Y->n, &Y->bits);
+    }
 }
+
+// Bit lengths of Fibonacci and Lucas numbers, known in advance so that the
+// whole doubling chain can be queued without waiting for the device:
+//   F(m) = round(phi^m / sqrt 5), L(m) = phi^m + psi^m.
+// Exact integers up to m = 93; above, log2 values in long double (about
+// 1e-9 absolute error at m = 1e10), refused (-1) within 1e-7 of an
+// integer, where the caller falls back to reading lengths back.
+enum { FB_F = 0, FB_F2 = 1, FB_L = 2 };
+
+static int fib_bits_pred(uint64_t m, int kind, uint64_t *bits) {
+    if (m <= 93) {
+        uint64_t a = 0, b = 1;  // F(0), F(1)
+        for (uint64_t i = 0; i < m; i++) {
+            const uint64_t t = a + b;
+            a = b, b = t;
+        }
+        // a = F(m), b = F(m+1)
+        u128 v;
+        if (kind == FB_F) v = a;
+        else if (kind == FB_F2) v = (u128)a * a;
+        else v = (m == 0) ? 2 : (u128)a + 2 * (u128)(b - a);  // L(m) = F(m) + 2F(m-1)
+        uint64_t r = 0;
+        while (v) v >>= 1, r++;
+        *bits = r;
+        return 0;
+    }
+    const long double LOG2PHI = 0.69424191363061730173879026689859522L;
+    const long double LOG2SQRT5 = 1.16096404744368117393515971474469509L;
+    long double x = (long double)m * LOG2PHI;
+    if (kind == FB_F) x -= LOG2SQRT5;
+    else if (kind == FB_F2) x = 2 * (x - LOG2SQRT5);
+    const long double f = floorl(x);
+    const long double frac = x - f;
+    if (frac < 1e-7L || frac > 1 - 1e-7L) return -1;
+    *bits = (uint64_t)f + 1;
+    return 0;
+}
+
+struct FibPlanVal {
+    int slot;
+    uint64_t bits;
+};
 
 // fibonacci (fib.c:211-261) with every product and pass on the device: the
 // pair (F(k-1), F(k)) stays in HBM; each bit costs two squarings run as one
-// launch per pass (same schedule) and one linear step.
-static DNum dfib(uint64_t n) {
-    if (n == 0) return dnum_u64(0);
+// launch per pass (same schedule) and one linear step. With predict, all
+// lengths come from fib_bits_pred and nothing waits for the device until
+// the end, where the lengths the kernels booked are checked against the
+// predictions (a mismatch returns false; the caller reruns reading lengths).
+static bool dfib(uint64_t n, bool predict, DNum *res) {
+    slots_reset(8 * 64 + 16);
+    if (n == 0) {
+        *res = dnum_u64(0);
+        return true;
+    }
+    std::vector<FibPlanVal> checks;
     DNum fkm1 = dnum_u64(0), fk = dnum_u64(1);
+    uint64_t k = 1;
     int k_odd = 1;
     const int top = 63 - __builtin_clzll(n);
+    auto P = [&](uint64_t m, int kind) {
+        uint64_t b = 0;
+        if (fib_bits_pred(m, kind, &b) != 0) predict = false;
+        return b;
+    };
+    if (predict) {
+        // check every prediction up front; any refusal switches to reading lengths
+        uint64_t kk = 1, tmp;
+        for (int i = top - 1; i >= 1 && predict; i--) {
+            const int bit = (int)((n >> i) & 1);
+            if (fib_bits_pred(kk, FB_F2, &tmp) || fib_bits_pred(kk - 1, FB_F2, &tmp) ||
+                fib_bits_pred(2 * kk - 1 + bit, FB_F, &tmp) || fib_bits_pred(2 * kk + bit, FB_F, &tmp))
+                predict = false;
+            kk = 2 * kk + bit;
+        }
+        if (predict && top > 0) {
+            if (fib_bits_pred(kk + 2, FB_F, &tmp) || fib_bits_pred(kk - 1, FB_L, &tmp) ||
+                fib_bits_pred(kk, FB_L, &tmp) || fib_bits_pred(n, FB_F, &tmp))
+                predict = false;
+        }
+    }
     for (int i = top - 1; i >= 1; i--) {
         DNum sq[2];
         const DNum *A[2] = {&fk, &fkm1}, *B[2] = {nullptr, nullptr};
-        mul_many(A, B, sq, 2);
+        uint64_t ps[2] = {0, 0};
+        if (predict) ps[0] = P(k, FB_F2), ps[1] = P(k - 1, FB_F2);
+        mul_many(A, B, sq, 2, predict ? ps : nullptr);
+        if (predict) {
+            checks.push_back({sq[0].slot, ps[0]});
+            if (k > 1) checks.push_back({sq[1].slot, ps[1]});
+        }
         dnum_free(fk);
         dnum_free(fkm1);
         const int bit = (int)((n >> i) & 1);
-        fib_step(sq[0], sq[1], k_odd, bit, &fkm1, &fk);
+        uint64_t pl[2] = {0, 0};
+        if (predict) pl[0] = P(2 * k - 1 + bit, FB_F), pl[1] = P(2 * k + bit, FB_F);
+        fib_step(sq[0], sq[1], k_odd, bit, &fkm1, &fk, predict ? pl : nullptr);
+        if (predict) {
+            checks.push_back({fkm1.slot, pl[0]});
+            checks.push_back({fk.slot, pl[1]});
+        }
         dnum_free(sq[0]);
         dnum_free(sq[1]);
+        k = 2 * k + bit;
         k_odd = bit;
     }
     if (top == 0) {
         dnum_free(fkm1);
-        return fk;
+        *res = fk;
+        return true;
     }
     DNum result;
     if (n & 1) {
+        // F(2k+1) = (2F(k) + F(k-1)) (2F(k) - F(k-1)) + 2(-1)^k, the
+        // factors being F(k+2) and L(k-1)
         DNum x, y;
-        last_factors(fk, fkm1, &x, &y);
+        uint64_t pf[2] = {0, 0};
+        if (predict) pf[0] = P(k + 2, FB_F), pf[1] = P(k - 1, FB_L);
+        last_factors(fk, fkm1, &x, &y, predict ? pf : nullptr);
+        if (predict) checks.push_back({x.slot, pf[0]}), checks.push_back({y.slot, pf[1]});
         dnum_free(fk);
         dnum_free(fkm1);
-        mul2(&x, &y, &result);
+        const DNum *A[1] = {&x}, *B[1] = {&y};
+        uint64_t pr = predict ? P(n, FB_F) : 0;
+        mul_many(A, B, &result, 1, predict ? &pr : nullptr);
         dnum_free(x);
         dnum_free(y);
-        FF_CHECK(ff_launch_add_small(result.d, result.n + 1, k_odd ? -2 : 2, g_st));
-        FFNormJob nj = {result.d, result.n + 1};
-        uint64_t res[2];
-        read_lengths(&nj, 1, res);
-        result.n = res[0], result.bits = res[1];
+        const uint64_t len = result.n + 1 <= result.cap ? result.n + 1 : result.cap;
+        FF_CHECK(ff_launch_add_small(result.d, len, k_odd ? -2 : 2, g_slots + result.slot,
+                                     len >= 3 ? len - 3 : 0, g_st));
+        if (predict) {
+            checks.push_back({result.slot, pr});
+        } else {
+            uint64_t kk;
+            slots_read(result.slot, 1, &kk);
+            key_decode(kk, &result.n, &result.bits);
+        }
     } else {
+        // F(2k) = F(k) (F(k) + 2F(k-1)), the factor being L(k)
         DNum x;
-        last_factors(fkm1, fk, &x, nullptr);
+        uint64_t pf[1] = {0};
+        if (predict) pf[0] = P(k, FB_L);
+        last_factors(fkm1, fk, &x, nullptr, predict ? pf : nullptr);
+        if (predict) checks.push_back({x.slot, pf[0]});
         dnum_free(fkm1);
-        mul2(&fk, &x, &result);
+        const DNum *A[1] = {&fk}, *B[1] = {&x};
+        uint64_t pr = predict ? P(n, FB_F) : 0;
+        mul_many(A, B, &result, 1, predict ? &pr : nullptr);
+        if (predict) checks.push_back({result.slot, pr});
         dnum_free(x);
         dnum_free(fk);
     }
-    return result;
+    *res = result;
+    if (!predict || checks.empty()) return true;
+    std::vector<uint64_t> keys(g_slot_used);
+    slots_read(0, g_slot_used, keys.data());
+    for (const auto &c : checks) {
+        if (keys[c.slot] != key_of_bits(c.bits)) return false;
+    }
+    return true;
+}
+
+static DNum dfib_checked(uint64_t n) {
+    DNum r;
+    if (n >= 200 && dfib(n, true, &r)) return r;
+    if (n >= 200) {
+        dnum_free(r);
+        fprintf(stderr, "fastfib_amd: predicted lengths disagreed for F(%llu); recomputing with read-back\n",
+                (unsigned long long)n);
+    }
+    dfib(n, false, &r);
+    return r;
 }
 
 // ------------------------------------------------------------------ C-ABI
@@ -543,16 +572,18 @@ static Big download(DNum &x) {
 extern "C" void ff_ntt_init(size_t max_limbs) {
     (void)max_limbs;  // the device tables cover every length up to 2^25 at once
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    ensure_init();
+    ff_ensure_init();
 }
 
 static Big host_mul(const Big *a, const Big *b, bool square) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    ensure_init();
+    ff_ensure_init();
+    slots_reset(16);
     DNum A = upload(a->d, a->n), B{};
     if (!square) B = upload(b->d, b->n);
+    const DNum *pa[1] = {&A}, *pb[1] = {square ? nullptr : &B};
     DNum R;
-    mul2(&A, square ? nullptr : &B, &R);
+    mul_many(pa, pb, &R, 1, nullptr);
     Big r = download(R);
     dnum_free(R);
     dnum_free(A);
@@ -566,8 +597,8 @@ extern "C" Big ff_big_mul_ntt(const Big *a, const Big *b) { return host_mul(a, b
 
 extern "C" Big ff_fibonacci(uint64_t n) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    ensure_init();
-    DNum R = dfib(n);
+    ff_ensure_init();
+    DNum R = dfib_checked(n);
     Big r = download(R);
     dnum_free(R);
     return r;
@@ -579,7 +610,33 @@ extern "C" void ff_big_free(Big *x) {
     x->n = 0;
 }
 
-static DNum from_dbig(const ff_dbig *x) { return DNum{x->d, x->n, x->bits, x->cap}; }
+extern "C" void ff_fib_step_linear(const Big *s1, const Big *s2, int k_odd, int bit, Big *lo, Big *hi) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    ff_ensure_init();
+    slots_reset(16);
+    DNum S1 = upload(s1->d, s1->n), S2 = upload(s2->d, s2->n), L, H;
+    fib_step(S1, S2, k_odd, bit, &L, &H, nullptr);
+    *lo = download(L);
+    *hi = download(H);
+    dnum_free(S1), dnum_free(S2), dnum_free(L), dnum_free(H);
+}
+
+extern "C" void ff_last_factors(const Big *u, const Big *v, Big *x, Big *y) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    ff_ensure_init();
+    slots_reset(16);
+    DNum U = upload(u->d, u->n), V = upload(v->d, v->n), X, Y;
+    last_factors(U, V, &X, y ? &Y : nullptr, nullptr);
+    *x = download(X);
+    dnum_free(X);
+    if (y) {
+        *y = download(Y);
+        dnum_free(Y);
+    }
+    dnum_free(U), dnum_free(V);
+}
+
+static DNum from_dbig(const ff_dbig *x) { return DNum{x->d, x->n, x->bits, x->cap, -1}; }
 
 static void to_dbig(const DNum &x, ff_dbig *o) {
     o->d = x.d;
@@ -591,9 +648,9 @@ static void to_dbig(const DNum &x, ff_dbig *o) {
 extern "C" int ff_dbig_alloc(ff_dbig *x, size_t cap) {
     if (!x || cap == 0) return FF_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    ensure_init();
+    ff_ensure_init();
     DNum d = dnum_alloc(cap);
-    FF_CHECK(hipMemsetAsync(d.d, 0, 8, g_st));
+    FF_CHECK(ff_launch_fill(d.d, 1, 0, g_st));
     to_dbig(d, x);
     return FF_OK;
 }
@@ -601,7 +658,7 @@ extern "C" int ff_dbig_alloc(ff_dbig *x, size_t cap) {
 extern "C" void ff_dbig_release(ff_dbig *x) {
     if (!x || !x->d) return;
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    dfree(x->d);
+    ff_dfree(x->d);
     x->d = nullptr;
     x->n = x->cap = 0;
     x->bits = 0;
@@ -610,7 +667,7 @@ extern "C" void ff_dbig_release(ff_dbig *x) {
 extern "C" int ff_dbig_upload(ff_dbig *x, const uint64_t *limbs, size_t n) {
     if (!x || (!limbs && n)) return FF_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    ensure_init();
+    ff_ensure_init();
     ff_dbig_release(x);
     DNum d = upload(limbs, n);
     to_dbig(d, x);
@@ -628,11 +685,13 @@ extern "C" int ff_dbig_download(const ff_dbig *x, uint64_t *limbs, size_t cap) {
 extern "C" int ff_dmul(ff_dbig *out, const ff_dbig *a, const ff_dbig *b) {
     if (!out || !a || !a->d) return FF_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    ensure_init();
+    ff_ensure_init();
+    slots_reset(16);
     DNum A = from_dbig(a), B = b ? from_dbig(b) : DNum{};
     const bool sq = (b == nullptr || b == a || b->d == a->d);
+    const DNum *pa[1] = {&A}, *pb[1] = {sq ? nullptr : &B};
     DNum R;
-    mul2(&A, sq ? nullptr : &B, &R);
+    mul_many(pa, pb, &R, 1, nullptr);
     if (out->d && out->d != a->d && (!b || out->d != b->d)) ff_dbig_release(out);
     to_dbig(R, out);
     return FF_OK;
@@ -641,8 +700,8 @@ extern "C" int ff_dmul(ff_dbig *out, const ff_dbig *a, const ff_dbig *b) {
 extern "C" int ff_dfibonacci(uint64_t n, ff_dbig *out) {
     if (!out) return FF_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    ensure_init();
-    DNum R = dfib(n);
+    ff_ensure_init();
+    DNum R = dfib_checked(n);
     to_dbig(R, out);
     return FF_OK;
 }

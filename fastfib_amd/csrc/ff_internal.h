@@ -7,20 +7,33 @@
 
 #include "ff_arith.h"
 
-// One transform operand or array for the pass kernels. The forward top pass
-// reads `src` (an operand's limbs as 32-bit words, nw of them) and writes
-// `data`; later passes work on `data` ([3][L] words, prime-major) in place.
+// Items per launch passed by value in the kernel arguments (no descriptor
+// uploads); bigger batches are cut into launches of this many.
+#define FF_MAXB 4
+
+// One transform array for the outer passes. The forward top pass reads
+// `src` (an operand's limbs as 32-bit words, nw of them) and writes `data`
+// ([3][L] words, prime-major); later passes work on `data` in place.
 struct FFXform {
     const uint32_t *src;
     uint32_t nw;
     uint32_t *data;
 };
+struct FFXformSet {
+    FFXform x[FF_MAXB];
+};
 
-// One product for the inner (pointwise) kernels: fa holds a's forward
-// transform and receives the product's; fb = nullptr squares.
+// One product for the inner kernels: fa holds a's forward transform (or, for
+// one-block transforms, receives it from asrc) and gets the product's; fb /
+// bsrc = nullptr squares.
 struct FFConv {
+    const uint32_t *asrc, *bsrc;
+    uint32_t anw, bnw;
     uint32_t *fa;
     const uint32_t *fb;
+};
+struct FFConvSet {
+    FFConv c[FF_MAXB];
 };
 
 // Parameters of one outer (strided, four-step) pass over blocks of Lb words.
@@ -73,30 +86,49 @@ struct FFChunk {
     uint32_t pad_;
 };
 
-extern "C" {
-// ff_ntt.hip
-int ff_tables_init(void);
-const FFTables *ff_tables_dev(void);
-void ff_prime_consts(FFPrime out[3]);
-int ff_plan_build(FFPlan *pl, uint32_t L, int bw, int lg, int r3, uint64_t conv, int S_log, int tile_words);
-hipError_t ff_launch_forward(const FFPlan *pl, const FFXform *x_dev, int nx, hipStream_t st);
-hipError_t ff_launch_conv(const FFPlan *pl, const FFXform *a_dev, const FFXform *b_dev, const FFConv *c_dev,
-                          int nc, hipStream_t st);
-hipError_t ff_launch_inverse(const FFPlan *pl, const FFXform *x_dev, int nx, hipStream_t st);
-
-// ff_crt.hip
+// One product's CRT: residues [3][L] -> limbs.
 struct FFCrtJob {
-    const uint32_t *y;   // residues [3][L]
-    uint64_t *out;       // limbs: room for max(nout, nchunks * bw + 2)
-    FFChunk *chunks;     // nchunks records
-    unsigned __int128 *cin;  // nchunks + 1 incoming carries
+    const uint32_t *y;       // residues [3][L]
+    uint64_t *out;           // limbs: room for max(nout, nchunks * bw + 2)
+    FFChunk *chunks;         // nchunks records
+    unsigned __int128 *cin;  // nchunks + 1 incoming carries (exact path)
+    uint32_t *wgflag;        // per pack workgroup: one of its chunks could propagate a carry
+    uint32_t *mode;          // written by the scan: 1 = exact carries in cin, 0 = the H's
+    unsigned long long *lenkey;  // (top nonzero limb + 1) << 8 | its bit length, by atomicMax
     uint64_t nout;
     uint64_t conv;
     uint32_t L;
     int bw;
 };
-hipError_t ff_launch_crt(const FFCrtJob *jobs_dev, int njobs, uint64_t max_chunks, hipStream_t st);
-hipError_t ff_launch_normlen(const uint64_t *d, uint64_t n, uint64_t *len_out, hipStream_t st);
+struct FFCrtSet {
+    FFCrtJob j[FF_MAXB];
+};
+
+// Kernel kinds for the live per-launch timer (ff_prof_*).
+enum FFKernelKind {
+    FK_FWD_TOP, FK_FWD, FK_INNER, FK_SMALL, FK_INV, FK_INV_TOP,
+    FK_CRT_PACK, FK_CRT_SCAN, FK_CRT_APPLY, FK_STEP, FK_CARRY, FK_NORM, FK_COUNT
+};
+
+extern "C" {
+// ff_engine.hip: when enabled, brackets each launch with HIP events on its
+// stream and books its algorithmic bytes under its kind.
+void ff_prof_begin(int kind, hipStream_t st);
+void ff_prof_end(int kind, double bytes, hipStream_t st);
+
+// ff_ntt.hip
+int ff_tables_init(void);
+const FFTables *ff_tables_dev(void);
+void ff_prime_consts(FFPrime out[3]);
+int ff_plan_build(FFPlan *pl, uint32_t L, int bw, int lg, int r3, uint64_t conv, int S_log, int tile_words);
+hipError_t ff_launch_forward(const FFPlan *pl, const FFXformSet *xs, int nx, double operand_bytes, hipStream_t st);
+hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int nc, int ngeneral, double operand_bytes,
+                          hipStream_t st);
+hipError_t ff_launch_inverse(const FFPlan *pl, const FFXformSet *xs, int nx, hipStream_t st);
+
+// ff_crt.hip
+hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_chunks, double conv_total,
+                         double out_limbs_total, hipStream_t st);
 }
 
 #define FF_CHECK(expr)                                                                      \

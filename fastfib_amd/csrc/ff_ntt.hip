@@ -121,11 +121,11 @@ extern "C" int ff_tables_init(void) {
 
 // Outer pass over a tile of R rows x C columns of one block.
 template <bool INV>
-__global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, const FFXform *xs, const FFTables *T,
+__global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, const FFTables *T,
                                                       FFPrimes PR) {
     extern __shared__ uint32_t lds[];
     const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
-    const FFXform X = xs[item];
+    const FFXform X = xs.x[item];
     const FFPrime P = PR.p[pi];
     const uint32_t p = P.p;
     const uint32_t blk = blockIdx.x / ps.tpb, ct = blockIdx.x - blk * ps.tpb;
@@ -134,96 +134,123 @@ __global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, const FFXform *
     FFTile t{lds, ps.ld, ps.C, ps.logC, ps.rho, ps.r3 ? 3 : 1, 0};
     const int n = (int)ps.R << ps.logC;
 
-    uint32_t scm = 0;
-    if (INV && ps.top) scm = ps.sc[pi];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int pos = i >> ps.logC, c = i & (ps.C - 1);
+    const uint32_t pinv = P.pinv;
+    const uint32_t scm = (INV && ps.top) ? ps.sc[pi] : 0;
+    const bool top = ps.top;
+    const int bw = ps.bw;
+    const uint32_t Sb = ps.Sb, rootstep = ps.rootstep;
+    const int rho = ps.rho, r3 = ps.r3;
+    const uint32_t *src = X.src;
+    const uint32_t nw = X.nw;
+    // element (row position pos, tile column c) <-> data[pos * Sb + c0 + c]
+    auto load_fwd = [&](int pos, int c) -> uint32_t {
+        const size_t g = (size_t)pos * Sb + c0 + c;
+        return top ? ff_load_coef(src, nw, g, bw, P) : data[g];
+    };
+    auto store_fwd = [&](int pos, int c, uint32_t v) {
         const uint32_t col = c0 + c;
-        const size_t g = (size_t)pos * ps.Sb + col;
-        uint32_t v;
-        if (!INV) {
-            v = ps.top ? ff_load_coef(X.src, X.nw, g, ps.bw, P) : data[g];
-        } else {
-            uint32_t e = col * ff_row_freq(pos, ps.rho, ps.r3) * ps.rootstep;
-            e = e ? FF_NBASE - e : 0;
-            uint32_t tw = ff_root(T, pi, e, p, P.pinv);
-            if (ps.top) tw = ff_mont(tw, scm, p, P.pinv);
-            v = ff_mont(data[g], tw, p, P.pinv);
-        }
-        lds[ff_pos_addr(t, pos, c)] = v;
-    }
+        const uint32_t e = col * ff_row_freq(pos, rho, r3) * rootstep;
+        data[(size_t)pos * Sb + col] = e ? ff_mont(v, ff_root(T, pi, e, p, pinv), p, pinv) : v;
+    };
+    auto load_inv = [&](int pos, int c) -> uint32_t {
+        const uint32_t col = c0 + c;
+        uint32_t e = col * ff_row_freq(pos, rho, r3) * rootstep;
+        e = e ? FF_NBASE - e : 0;
+        uint32_t tw = ff_root(T, pi, e, p, pinv);
+        if (top) tw = ff_mont(tw, scm, p, pinv);
+        return ff_mont(data[(size_t)pos * Sb + col], tw, p, pinv);
+    };
+    auto store_inv = [&](int pos, int c, uint32_t v) { data[(size_t)pos * Sb + c0 + c] = v; };
+    auto lds_load = [&](int pos, int c) -> uint32_t { return lds[ff_pos_addr(t, pos, c)]; };
+    auto lds_store = [&](int pos, int c, uint32_t v) { lds[ff_pos_addr(t, pos, c)] = v; };
+
     if (!INV) {
-        if (ps.r3) ff_radix3_layer<false>(t, T, pi, P, ps.rstep3);
-        ff_tile_transform<false>(t, T->rt[pi], T->rtp[pi], p);
-    } else {
-        ff_tile_transform<true>(t, T->irt[pi], T->irtp[pi], p);
-        if (ps.r3) ff_radix3_layer<true>(t, T, pi, P, ps.rstep3);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int pos = i >> ps.logC, c = i & (ps.C - 1);
-        const uint32_t col = c0 + c;
-        const size_t g = (size_t)pos * ps.Sb + col;
-        uint32_t v = lds[ff_pos_addr(t, pos, c)];
-        if (!INV) {
-            const uint32_t e = col * ff_row_freq(pos, ps.rho, ps.r3) * ps.rootstep;
-            v = ff_mont(v, ff_root(T, pi, e, p, P.pinv), p, P.pinv);
+        if (!r3) {
+            ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, true, true, load_fwd, store_fwd);
+        } else {
+            // the radix-3 layer works on the LDS tile
+            for (int i = threadIdx.x; i < n; i += blockDim.x) lds_store(i >> ps.logC, i & (ps.C - 1),
+                                                                        load_fwd(i >> ps.logC, i & (ps.C - 1)));
+            ff_radix3_layer<false>(t, T, pi, P, ps.rstep3);
+            ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, false, true, lds_load, store_fwd);
         }
-        data[g] = v;
+    } else {
+        if (!r3) {
+            ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, true, load_inv, store_inv);
+        } else {
+            ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, false, load_inv, lds_store);
+            ff_radix3_layer<true>(t, T, pi, P, ps.rstep3);
+            for (int i = threadIdx.x; i < n; i += blockDim.x) store_inv(i >> ps.logC, i & (ps.C - 1),
+                                                                        lds_load(i >> ps.logC, i & (ps.C - 1)));
+        }
     }
 }
 
 // Inner blocks: forward transform of a (and b), pointwise product, inverse,
 // all in LDS. small = 1: the whole transform, read from the limbs and
 // scaled for the CRT at the end.
-__global__ __launch_bounds__(FF_THREADS) void k_inner(FFInner in, const FFXform *xa, const FFXform *xb,
-                                                      const FFConv *cv, const FFTables *T, FFPrimes PR) {
+__global__ __launch_bounds__(FF_THREADS) void k_inner(FFInner in, FFConvSet cs, const FFTables *T, FFPrimes PR) {
     extern __shared__ uint32_t lds[];
     const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
     const FFPrime P = PR.p[pi];
     const uint32_t p = P.p;
-    const FFConv C = cv[item];
-    const bool square = in.small ? (xb[item].src == nullptr) : (C.fb == nullptr);
+    const FFConv C = cs.c[item];
+    const bool square = in.small ? (C.bsrc == nullptr) : (C.fb == nullptr);
     const uint32_t n = in.small ? in.L : in.S;
     const int words = (int)(n + (n >> 4) + 16);
     uint32_t *A = lds, *B = lds + words;
-    FFTile ta{A, 1, 1, 0, in.small && in.r3 ? in.s : in.s, in.small && in.r3 ? 3 : 1, 1};
+    FFTile ta{A, 1, 1, 0, in.s, in.small && in.r3 ? 3 : 1, 1};
     FFTile tb = ta;
     tb.s = B;
     const size_t off = (size_t)pi * in.L + (in.small ? 0 : (size_t)blockIdx.x * in.S);
 
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        uint32_t va, vb = 0;
-        if (in.small) {
-            va = ff_load_coef(xa[item].src, xa[item].nw, i, in.bw, P);
-            if (!square) vb = ff_load_coef(xb[item].src, xb[item].nw, i, in.bw, P);
-        } else {
-            va = C.fa[off + i];
-            if (!square) vb = C.fb[off + i];
-        }
-        A[ff_pos_addr(ta, (int)i, 0)] = va;
-        if (!square) B[ff_pos_addr(tb, (int)i, 0)] = vb;
-    }
-    if (in.small && in.r3) {
-        ff_radix3_layer<false>(ta, T, pi, P, in.rstep3);
-        if (!square) ff_radix3_layer<false>(tb, T, pi, P, in.rstep3);
-    }
-    ff_tile_transform<false>(ta, T->rt[pi], T->rtp[pi], p);
-    if (!square) ff_tile_transform<false>(tb, T->rt[pi], T->rtp[pi], p);
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const int a = ff_pos_addr(ta, (int)i, 0);
-        const uint32_t x = A[a];
-        A[a] = ff_mont(x, square ? x : B[a], p, P.pinv);
-    }
-    __syncthreads();
-    ff_tile_transform<true>(ta, T->irt[pi], T->irtp[pi], p);
-    if (in.small && in.r3) ff_radix3_layer<true>(ta, T, pi, P, in.rstep3);
+    const uint32_t pinv = P.pinv;
+    const bool small = in.small;
+    const int bw = in.bw;
+    const uint32_t sc = in.sc[pi];
     uint32_t *dst = C.fa + off;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        uint32_t v = A[ff_pos_addr(ta, (int)i, 0)];
-        if (in.small) v = ff_mont(v, in.sc[pi], p, P.pinv);
-        dst[i] = v;
+    // element i of the block <-> limbs' coefficient i (one-block form) or
+    // the forward transform's word off + i
+    auto load_a = [&](int pos, int c) -> uint32_t {
+        (void)c;
+        return small ? ff_load_coef(C.asrc, C.anw, (uint64_t)pos, bw, P) : C.fa[off + pos];
+    };
+    auto load_b = [&](int pos, int c) -> uint32_t {
+        (void)c;
+        return small ? ff_load_coef(C.bsrc, C.bnw, (uint64_t)pos, bw, P) : C.fb[off + pos];
+    };
+    auto store_a = [&](int pos, int c, uint32_t v) {
+        (void)c;
+        dst[pos] = small ? ff_mont(v, sc, p, pinv) : v;
+    };
+    auto lds_a = [&](int pos, int c) -> uint32_t { return A[ff_pos_addr(ta, pos, c)]; };
+    auto lds_a_st = [&](int pos, int c, uint32_t v) { A[ff_pos_addr(ta, pos, c)] = v; };
+    auto lds_b_st = [&](int pos, int c, uint32_t v) { B[ff_pos_addr(tb, pos, c)] = v; };
+
+    if (!(small && in.r3)) {
+        if (!square) {
+            ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_b, lds_b_st);
+            __syncthreads();
+        }
+        ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, true, true,
+                   load_a, store_a);
+        return;
     }
+    // 3 * 2^k in one block: the radix-3 layers work on the LDS tiles
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        lds_a_st((int)i, 0, load_a((int)i, 0));
+        if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
+    }
+    ff_radix3_layer<false>(ta, T, pi, P, in.rstep3);
+    if (!square) {
+        ff_radix3_layer<false>(tb, T, pi, P, in.rstep3);
+        ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, false, false, lds_a, lds_b_st);
+        __syncthreads();
+    }
+    ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, false, false,
+               lds_a, lds_a_st);
+    ff_radix3_layer<true>(ta, T, pi, P, in.rstep3);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) store_a((int)i, 0, lds_a((int)i, 0));
 }
 
 // ------------------------------------------------------------------ plans
@@ -320,33 +347,50 @@ extern "C" int ff_plan_build(FFPlan *pl, uint32_t L, int bw, int lg, int r3, uin
 
 static size_t outer_lds(const FFPass &ps) { return (size_t)ps.R * ps.ld * sizeof(uint32_t); }
 
-extern "C" hipError_t ff_launch_forward(const FFPlan *pl, const FFXform *x_dev, int nx, hipStream_t st) {
+// Algorithmic bytes (per prime, per array): a top forward pass reads the
+// operand's limbs and writes 4L; every other outer pass reads and writes 4L.
+extern "C" hipError_t ff_launch_forward(const FFPlan *pl, const FFXformSet *xs, int nx, double operand_bytes,
+                                        hipStream_t st) {
     for (int i = 0; i < pl->npass; i++) {
         const FFPass &ps = pl->pass[i];
         dim3 grid(ps.nblocks * ps.tpb, 3 * nx);
-        hipLaunchKernelGGL(k_outer<false>, grid, dim3(FF_THREADS), outer_lds(ps), st, ps, x_dev, g_tables,
+        const int kind = i == 0 ? FK_FWD_TOP : FK_FWD;
+        ff_prof_begin(kind, st);
+        hipLaunchKernelGGL(k_outer<false>, grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs, g_tables,
                            g_primes_arg);
+        const double L4 = 4.0 * pl->L;
+        ff_prof_end(kind, i == 0 ? 3.0 * (operand_bytes + nx * L4) : 3.0 * nx * 2 * L4, st);
     }
     return hipGetLastError();
 }
 
-extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFXform *a_dev, const FFXform *b_dev,
-                                     const FFConv *c_dev, int nc, hipStream_t st) {
+// Algorithmic bytes per prime: the inner kernel reads a's (and b's)
+// transform and writes the product's, 4L each; the one-block kernel reads
+// the operands' limbs instead.
+extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int nc, int ngeneral,
+                                     double operand_bytes, hipStream_t st) {
     const FFInner &in = pl->inner;
     const uint32_t n = in.small ? in.L : in.S;
     const size_t words = n + (n >> 4) + 16;
     dim3 grid(in.small ? 1 : pl->L / in.S, 3 * nc);
-    hipLaunchKernelGGL(k_inner, grid, dim3(FF_THREADS), 2 * words * sizeof(uint32_t), st, in, a_dev, b_dev, c_dev,
-                       g_tables, g_primes_arg);
+    const int kind = in.small ? FK_SMALL : FK_INNER;
+    const double L4 = 4.0 * pl->L;
+    ff_prof_begin(kind, st);
+    hipLaunchKernelGGL(k_inner, grid, dim3(FF_THREADS), 2 * words * sizeof(uint32_t), st, in, *cs, g_tables,
+                       g_primes_arg);
+    ff_prof_end(kind, in.small ? 3.0 * (operand_bytes + nc * L4) : 3.0 * (nc + ngeneral + nc) * L4, st);
     return hipGetLastError();
 }
 
-extern "C" hipError_t ff_launch_inverse(const FFPlan *pl, const FFXform *x_dev, int nx, hipStream_t st) {
+extern "C" hipError_t ff_launch_inverse(const FFPlan *pl, const FFXformSet *xs, int nx, hipStream_t st) {
     for (int i = pl->npass - 1; i >= 0; i--) {
         const FFPass &ps = pl->pass[i];
         dim3 grid(ps.nblocks * ps.tpb, 3 * nx);
-        hipLaunchKernelGGL(k_outer<true>, grid, dim3(FF_THREADS), outer_lds(ps), st, ps, x_dev, g_tables,
+        const int kind = i == 0 ? FK_INV_TOP : FK_INV;
+        ff_prof_begin(kind, st);
+        hipLaunchKernelGGL(k_outer<true>, grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs, g_tables,
                            g_primes_arg);
+        ff_prof_end(kind, 3.0 * nx * 8.0 * pl->L, st);
     }
     return hipGetLastError();
 }

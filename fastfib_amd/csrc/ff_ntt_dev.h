@@ -101,6 +101,162 @@ __device__ void ff_round(const FFTile &t, int b_lo, const uint32_t *rt, const ui
     }
 }
 
+// A round whose groups may come from a loader instead of LDS and go to a
+// storer instead of LDS: ld(pos, c) -> value, st(pos, c, value). The first
+// round of a transform reads HBM straight into registers and the last
+// writes HBM straight from them, so a tile makes one fewer LDS round trip
+// at each end. MID fuses the last DIF round, the pointwise product with
+// the other operand's spectrum (LDS array B, or a square) and the first DIT
+// round, which cover the same row bits and therefore the same groups.
+enum { FF_R_DIF = 0, FF_R_DIT = 1, FF_R_MID = 2 };
+
+template <int NL, int KIND, bool FROM_G, bool TO_G, class Ld, class St>
+__device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uint32_t *rt, const uint32_t *rtp,
+                                            const uint32_t *irt, const uint32_t *irtp, const uint32_t *B,
+                                            uint32_t p, uint32_t pinv, Ld ld, St st) {
+    const int G = 1 << NL;
+    const int per_sub = (1 << (t.rho - NL)) << t.logC;
+    const int ngroups = per_sub * t.nsub;
+    for (int gi = threadIdx.x; gi < ngroups; gi += blockDim.x) {
+        const int sub = gi / per_sub;
+        const int g = gi - sub * per_sub;
+        const int c = g & (t.C - 1);
+        const int rest = g >> t.logC;
+        const int low = rest & ((1 << b_lo) - 1);
+        const int base = low | ((rest >> b_lo) << (b_lo + NL));
+        const int pos0 = sub << t.rho;
+        uint32_t v[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const int pos = pos0 + (base | (j << b_lo));
+            v[j] = FROM_G ? ld(pos, c) : t.s[ff_pos_addr(t, pos, c)];
+        }
+        if (KIND == FF_R_DIF) {
+            ff_dif_group<NL>(v, low, b_lo, rt, rtp, p);
+        } else if (KIND == FF_R_DIT) {
+            ff_dit_group<NL>(v, low, b_lo, irt, irtp, p);
+        } else {
+            ff_dif_group<NL>(v, low, b_lo, rt, rtp, p);
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                const int pos = pos0 + (base | (j << b_lo));
+                v[j] = ff_mont(v[j], B ? B[ff_pos_addr(t, pos, c)] : v[j], p, pinv);
+            }
+            ff_dit_group<NL>(v, low, b_lo, irt, irtp, p);
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const int pos = pos0 + (base | (j << b_lo));
+            if (TO_G) st(pos, c, v[j]);
+            else t.s[ff_pos_addr(t, pos, c)] = v[j];
+        }
+    }
+}
+
+// Dispatch of one round by its level count and ends.
+template <int KIND, class Ld, class St>
+__device__ void ff_round_any(int nl, bool from_g, bool to_g, const FFTile &t, int b_lo, const uint32_t *rt,
+                             const uint32_t *rtp, const uint32_t *irt, const uint32_t *irtp, const uint32_t *B,
+                             uint32_t p, uint32_t pinv, Ld ld, St st) {
+#define FF_RCASE(N)                                                                                              \
+    case N:                                                                                                      \
+        if (from_g && to_g) ff_round_io<N, KIND, true, true>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st);   \
+        else if (from_g) ff_round_io<N, KIND, true, false>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st);     \
+        else if (to_g) ff_round_io<N, KIND, false, true>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st);       \
+        else ff_round_io<N, KIND, false, false>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st);                \
+        break;
+    switch (nl) {
+        FF_RCASE(4)
+        FF_RCASE(3)
+        FF_RCASE(2)
+        FF_RCASE(1)
+    }
+#undef FF_RCASE
+}
+
+// Round sizes: DIF rounds cover the row bits from the top, 4 at a time, the
+// remainder (rho % 4, or 4) last; DIT rounds mirror them.
+__device__ __forceinline__ int ff_nrounds(int rho) { return (rho + 3) / 4; }
+__device__ __forceinline__ int ff_last_nl(int rho) { return rho % 4 == 0 ? 4 : rho % 4; }
+
+// Forward (DIF) over the tile; the first round reads through ld when
+// from_g, the last writes through st when to_g.
+// A tile of one-row sub-transforms (rho = 0, under a radix-3 layer) has no
+// levels: its elements only pass through.
+template <class Ld, class St>
+__device__ void ff_pass_through(const FFTile &t, bool from_g, bool to_g, Ld ld, St st) {
+    const int n = t.nsub << t.logC;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int pos = i >> t.logC, c = i & (t.C - 1);
+        const uint32_t v = from_g ? ld(pos, c) : t.s[ff_pos_addr(t, pos, c)];
+        if (to_g) st(pos, c, v);
+        else t.s[ff_pos_addr(t, pos, c)] = v;
+    }
+}
+
+template <class Ld, class St>
+__device__ void ff_forward_io(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, uint32_t p, uint32_t pinv,
+                              bool from_g, bool to_g, Ld ld, St st) {
+    if (t.rho == 0) {
+        ff_pass_through(t, from_g, to_g, ld, st);
+        return;
+    }
+    const int R = ff_nrounds(t.rho);
+    int b = t.rho;
+    for (int r = 0; r < R; r++) {
+        const int nl = r == R - 1 ? ff_last_nl(t.rho) : 4;
+        b -= nl;
+        if (r) __syncthreads();
+        ff_round_any<FF_R_DIF>(nl, from_g && r == 0, to_g && r == R - 1, t, b, rt, rtp, nullptr, nullptr, nullptr,
+                               p, pinv, ld, st);
+    }
+}
+
+// Inverse (DIT) over the tile, mirror of ff_forward_io.
+template <class Ld, class St>
+__device__ void ff_inverse_io(const FFTile &t, const uint32_t *irt, const uint32_t *irtp, uint32_t p, uint32_t pinv,
+                              bool from_g, bool to_g, Ld ld, St st) {
+    if (t.rho == 0) {
+        ff_pass_through(t, from_g, to_g, ld, st);
+        return;
+    }
+    const int R = ff_nrounds(t.rho);
+    int b = 0;
+    for (int r = 0; r < R; r++) {
+        const int nl = r == 0 ? ff_last_nl(t.rho) : 4;
+        if (r) __syncthreads();
+        ff_round_any<FF_R_DIT>(nl, from_g && r == 0, to_g && r == R - 1, t, b, nullptr, nullptr, irt, irtp, nullptr,
+                               p, pinv, ld, st);
+        b += nl;
+    }
+}
+
+// Whole convolution of one LDS-resident block: forward of A (first round
+// through ld when from_g), pointwise with B (LDS, already transformed) or a
+// square, inverse, the last round through st when to_g.
+template <class Ld, class St>
+__device__ void ff_conv_io(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, const uint32_t *irt,
+                           const uint32_t *irtp, const uint32_t *B, uint32_t p, uint32_t pinv, bool from_g, bool to_g,
+                           Ld ld, St st) {
+    const int R = ff_nrounds(t.rho);
+    int b = t.rho;
+    for (int r = 0; r < R - 1; r++) {
+        b -= 4;
+        if (r) __syncthreads();
+        ff_round_any<FF_R_DIF>(4, from_g && r == 0, false, t, b, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, st);
+    }
+    const int nl0 = ff_last_nl(t.rho);
+    if (R > 1) __syncthreads();
+    ff_round_any<FF_R_MID>(nl0, from_g && R == 1, to_g && R == 1, t, 0, rt, rtp, irt, irtp, B, p, pinv, ld, st);
+    b = nl0;
+    for (int r = 1; r < R; r++) {
+        __syncthreads();
+        ff_round_any<FF_R_DIT>(4, false, to_g && r == R - 1, t, b, nullptr, nullptr, irt, irtp, nullptr, p, pinv, ld,
+                               st);
+        b += 4;
+    }
+}
+
 // All DIF levels (forward) or DIT levels (inverse) of the tile's columns.
 template <bool INV>
 __device__ void ff_tile_transform(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, uint32_t p) {

@@ -1,0 +1,25 @@
+// Runtime services shared by the host side of fastfib_amd: the library
+// stream, the stream-ordered device memory pool, errors and options.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <mutex>
+
+extern hipStream_t g_st;
+extern std::recursive_mutex g_mu;
+extern int g_inner_log;
+extern int g_tile_words;
+extern uint64_t g_force_len;
+
+void ff_set_err(const char *fmt, ...);
+void ff_ensure_init(void);
+
+// Device memory: blocks are cached by size class and reused in stream
+// order (all device work runs on g_st), hipMalloc only on a miss.
+void *ff_dalloc(size_t bytes);
+void ff_dfree(void *p);
+
+// Pinned host buffer for reading small results back.
+uint64_t *ff_pinned_results(size_t words);

@@ -1,0 +1,254 @@
+// Runtime of fastfib_amd: device selection and stream, the device memory
+// pool, the live kernel timer, options and error reporting.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/fastfib_amd.h"
+#include "ff_internal.h"
+#include "ff_runtime.h"
+
+static int g_inited = 0;
+static int g_device = 0;
+hipStream_t g_st = nullptr;
+std::recursive_mutex g_mu;  // the drop-in calls may come from several threads
+static char g_err[512];
+
+int g_inner_log = 12;
+int g_tile_words = 16384;
+uint64_t g_force_len = 0;
+
+void ff_set_err(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *ff_last_error(void) { return g_err; }
+
+// Drop-in failure behaviour: a message and exit(1) (alloc.c:142-146).
+extern "C" void ff_fail(const char *file, int line, const char *what, hipError_t e) {
+    fprintf(stderr, "fastfib_amd: %s failed at %s:%d: %s\n", what, file, line, hipGetErrorString(e));
+    exit(1);
+}
+
+// ------------------------------------------------------------------ device memory
+
+static std::multimap<size_t, void *> g_free;
+static std::unordered_map<void *, size_t> g_sizes;
+
+static size_t size_class(size_t b) {
+    if (b < 4096) return 4096;
+    size_t p = 1;
+    while (p < b) p <<= 1;
+    const size_t step = p / 8;  // within 1/8 of the next power of two
+    return (b + step - 1) / step * step;
+}
+
+void *ff_dalloc(size_t bytes) {
+    const size_t sz = size_class(bytes);
+    auto it = g_free.lower_bound(sz);
+    if (it != g_free.end() && it->first <= sz + sz / 2) {
+        void *p = it->second;
+        g_free.erase(it);
+        return p;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+        // give the cached blocks back and retry once
+        (void)hipStreamSynchronize(g_st);
+        for (auto &kv : g_free) {
+            (void)hipFree(kv.second);
+            g_sizes.erase(kv.second);
+        }
+        g_free.clear();
+        (void)hipGetLastError();
+        e = hipMalloc(&p, sz);
+        if (e != hipSuccess) {
+            fprintf(stderr, "out of memory (%zu bytes)\n", bytes);
+            exit(1);
+        }
+    }
+    g_sizes[p] = sz;
+    return p;
+}
+
+void ff_dfree(void *p) {
+    if (!p) return;
+    auto it = g_sizes.find(p);
+    if (it == g_sizes.end()) return;
+    g_free.emplace(it->second, p);
+}
+
+static uint64_t *g_res_h = nullptr;
+static size_t g_res_words = 0;
+
+uint64_t *ff_pinned_results(size_t words) {
+    if (words > g_res_words) {
+        if (g_res_h) (void)hipHostFree(g_res_h);
+        g_res_words = words < 4096 ? 4096 : words;
+        FF_CHECK(hipHostMalloc((void **)&g_res_h, g_res_words * sizeof(uint64_t), hipHostMallocDefault));
+    }
+    return g_res_h;
+}
+
+// ------------------------------------------------------------------ live kernel timer
+// When enabled, every launch is bracketed by two HIP events on its stream;
+// ff_prof_read syncs and books the elapsed time and the launch's
+// algorithmic bytes per kernel kind.
+
+struct ProfRec {
+    int kind;
+    double bytes;
+    hipEvent_t a, b;
+};
+static int g_prof_on = 0;
+static std::vector<ProfRec> g_prof_pending;
+static std::vector<hipEvent_t> g_prof_spare;
+static double g_prof_ms[FK_COUNT], g_prof_bytes[FK_COUNT];
+static long g_prof_n[FK_COUNT];
+static hipEvent_t g_prof_open = nullptr;
+
+static hipEvent_t prof_event(void) {
+    if (!g_prof_spare.empty()) {
+        hipEvent_t e = g_prof_spare.back();
+        g_prof_spare.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    FF_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+extern "C" void ff_prof_begin(int kind, hipStream_t st) {
+    (void)kind;
+    if (!g_prof_on) return;
+    g_prof_open = prof_event();
+    FF_CHECK(hipEventRecord(g_prof_open, st));
+}
+
+extern "C" void ff_prof_end(int kind, double bytes, hipStream_t st) {
+    if (!g_prof_on || !g_prof_open) return;
+    hipEvent_t e = prof_event();
+    FF_CHECK(hipEventRecord(e, st));
+    g_prof_pending.push_back(ProfRec{kind, bytes, g_prof_open, e});
+    g_prof_open = nullptr;
+}
+
+static void prof_collect(void) {
+    if (g_prof_pending.empty()) return;
+    FF_CHECK(hipStreamSynchronize(g_st));
+    for (auto &r : g_prof_pending) {
+        float ms = 0;
+        FF_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+        g_prof_ms[r.kind] += ms;
+        g_prof_bytes[r.kind] += r.bytes;
+        g_prof_n[r.kind]++;
+        g_prof_spare.push_back(r.a);
+        g_prof_spare.push_back(r.b);
+    }
+    g_prof_pending.clear();
+}
+
+static const char *kKindNames[FK_COUNT] = {"ntt_fwd_top", "ntt_fwd", "ntt_inner", "ntt_small", "ntt_inv",
+                                           "ntt_inv_top", "crt_pack", "crt_scan", "crt_apply", "fib_step",
+                                           "carry", "normlen"};
+
+extern "C" int ff_prof_enable(int on) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    ff_ensure_init();
+    prof_collect();
+    g_prof_on = on;
+    return FF_OK;
+}
+
+extern "C" void ff_prof_reset(void) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    ff_ensure_init();
+    prof_collect();
+    memset(g_prof_ms, 0, sizeof g_prof_ms);
+    memset(g_prof_bytes, 0, sizeof g_prof_bytes);
+    memset(g_prof_n, 0, sizeof g_prof_n);
+}
+
+extern "C" int ff_prof_read(int k, const char **name, long *launches, double *ms, double *bytes) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (k < 0 || k >= FK_COUNT) return FF_ERR_ARG;
+    ff_ensure_init();
+    prof_collect();
+    *name = kKindNames[k];
+    *launches = g_prof_n[k];
+    *ms = g_prof_ms[k];
+    *bytes = g_prof_bytes[k];
+    return FF_OK;
+}
+
+// ------------------------------------------------------------------ init
+
+extern "C" int ff_init(int device) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (g_inited) return FF_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        ff_set_err("no HIP device");
+        return FF_ERR_DEVICE;
+    }
+    if (device >= 0 && hipSetDevice(device) != hipSuccess) {
+        ff_set_err("hipSetDevice(%d) failed", device);
+        return FF_ERR_DEVICE;
+    }
+    (void)hipGetDevice(&g_device);
+    if (hipStreamCreateWithFlags(&g_st, hipStreamNonBlocking) != hipSuccess) {
+        ff_set_err("stream creation failed");
+        return FF_ERR_DEVICE;
+    }
+    if (ff_tables_init() != 0) {
+        ff_set_err("table upload failed");
+        return FF_ERR_DEVICE;
+    }
+    g_inited = 1;
+    return FF_OK;
+}
+
+void ff_ensure_init(void) {
+    if (!g_inited && ff_init(-1) != FF_OK) {
+        fprintf(stderr, "fastfib_amd: %s\n", g_err);
+        exit(1);
+    }
+}
+
+extern "C" void *ff_stream(void) {
+    ff_ensure_init();
+    return (void *)g_st;
+}
+
+extern "C" int ff_sync(void) {
+    ff_ensure_init();
+    return hipStreamSynchronize(g_st) == hipSuccess ? FF_OK : FF_ERR_DEVICE;
+}
+
+extern "C" int ff_set_option(const char *name, long long v) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!name) return FF_ERR_ARG;
+    if (!strcmp(name, "inner_log")) {
+        if (v < 4 || v > 12) return FF_ERR_ARG;
+        g_inner_log = (int)v;
+    } else if (!strcmp(name, "tile_words")) {
+        if (v < 1024 || v > 16384) return FF_ERR_ARG;
+        g_tile_words = (int)v;
+    } else if (!strcmp(name, "force_len")) {
+        if (v < 0) return FF_ERR_ARG;
+        g_force_len = (uint64_t)v;
+    } else {
+        return FF_ERR_ARG;
+    }
+    return FF_OK;
+}

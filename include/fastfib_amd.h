@@ -64,6 +64,16 @@ FF_API Big ff_fibonacci(uint64_t n);
 /* Replaces big_free (src/bigint.c:26): frees x's limbs, sets d = NULL, n = 0. */
 FF_API void ff_big_free(Big *x);
 
+/* Replaces fib_step_linear (src/fib.c:81): from s1 = F(k)^2 and
+   s2 = F(k-1)^2 builds (lo, hi) = bit ? (F(2k), F(2k+1)) : (F(2k-1), F(2k)),
+   i.e. A = s1 + s2, B = 4 s1 - s2 + 2(-1)^k (k_odd gives the sign),
+   C = B - A, keeping (C, B) or (A, C). Requires s2 <= s1. */
+FF_API void ff_fib_step_linear(const Big *s1, const Big *s2, int k_odd, int bit, Big *lo, Big *hi);
+
+/* Replaces last_factors (src/fib.c:147): x = 2u + v and, if y is not NULL,
+   y = 2u - v (requires 2u >= v). */
+FF_API void ff_last_factors(const Big *u, const Big *v, Big *x, Big *y);
+
 /* ------------------------------------------------------------------------
  * Device-resident API (no counterpart in the reference). Returns FF_OK or a
  * negative status; ff_last_error() describes the last failure.
@@ -117,6 +127,15 @@ FF_API int ff_dfibonacci(uint64_t n, ff_dbig *out);
      "force_len"   transform length to use instead of the shortest (0: off)
    Returns FF_ERR_ARG for an unknown name or value. */
 FF_API int ff_set_option(const char *name, long long value);
+
+/* Live kernel timer (no counterpart in the reference; its profiling was a
+   sampling profiler, tests/sprof.c). While enabled, each kernel launch is
+   bracketed by HIP events on the library stream. ff_prof_read returns, for
+   kernel kind k (0 .. until it returns FF_ERR_ARG), the kind's name, its
+   launches, total milliseconds and total algorithmic bytes moved. */
+FF_API int ff_prof_enable(int on);
+FF_API void ff_prof_reset(void);
+FF_API int ff_prof_read(int k, const char **name, long *launches, double *ms, double *bytes);
 
 #ifdef __cplusplus
 }

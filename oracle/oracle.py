@@ -307,8 +307,8 @@ FINGERPRINT_PRIMES = (18446744073709551557, 18446744073709551533, 92233720368547
 def fib_bits(n: int) -> int:
     """Bit length of F(n) from Binet's formula at high precision
     (tests/fingerprint.py:16-19)."""
-    if n == 0:
-        return 0
+    if n < 100:  # Binet's rounding term matters only for tiny n
+        return fib_int(n).bit_length()
     from mpmath import floor, log, mp, sqrt
     mp.dps = 60
     phi = (1 + sqrt(5)) / 2

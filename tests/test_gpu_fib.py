@@ -1,0 +1,96 @@
+"""F(n) on the GPU through the C-ABI, bit-exact.
+
+The n lists follow the reference's own checks: every n < 200, 40 seeded
+random n up to 3e6 and the values around 5e6 and 2^22 (tests/check.py:17-18),
+the large list of tests/bigcheck.ps1:8 up to 1e8 (against the reference
+build), and the fingerprint of tests/fingerprint.py / memrun.c (bit length,
+low 64 bits, F(n) mod four primes) at full size. The linear step and the
+last-step factors are also checked on their own with carry-adversarial
+operands (all-ones and all-zero limb runs), which force the exact carry scan.
+"""
+import random
+
+import numpy as np
+import pytest
+
+from oracle.oracle import fib_fingerprint, fib_int, from_limbs, to_limbs
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fib_small_n(ff):
+    for n in range(200):
+        assert from_limbs(ff.fibonacci(n)) == fib_int(n), n
+
+
+def test_fib_check_py_list(ff):
+    rng = random.Random(1)
+    ns = [rng.randint(200, 3_000_000) for _ in range(40)]
+    ns += [4_999_999, 5_000_000, 2 ** 22 - 1, 2 ** 22, 2 ** 22 + 1]
+    for n in ns:
+        assert from_limbs(ff.fibonacci(n)) == fib_int(n), n
+
+
+@pytest.mark.parametrize("n", [20_000_000, 33_554_431, 50_000_001, 77_777_777, 100_000_000])
+def test_fib_large_vs_reference(ff, ref, n):
+    """tests/bigcheck.ps1:8's n (up to 1e8) against the reference build."""
+    assert (ff.fibonacci(n) == ref.fibonacci(n)).all(), n
+
+
+def test_fib_1e8_fingerprint(ff):
+    fp = fib_fingerprint(10 ** 8)
+    d = ff.dfibonacci(10 ** 8)
+    x = d.value()
+    assert d.bits == fp["bits"] == x.bit_length() == 69424191
+    assert x % (1 << 64) == fp["low64"]
+    from oracle.oracle import FINGERPRINT_PRIMES
+    assert [x % p for p in FINGERPRINT_PRIMES] == fp["mods"]
+
+
+def test_fib_repeatable(ff):
+    """The sync-free path (lengths predicted on the host) and repeated runs
+    agree with the reference values."""
+    want = fib_int(1_234_567)
+    for _ in range(3):
+        assert from_limbs(ff.fibonacci(1_234_567)) == want
+
+
+def _step_ref(s1, s2, k_odd, bit):
+    A = s1 + s2
+    B = 4 * s1 - s2 + (-2 if k_odd else 2)
+    C = B - A
+    return (C, B) if bit else (A, C)
+
+
+@pytest.mark.parametrize("nl", [1, 2, 15, 16, 17, 33, 100, 5000, 70000])
+def test_fib_step_linear_adversarial(ff, nl):
+    """fib.c:36-62 on operands whose chunks are all ones or all zero, so that
+    carries and borrows run across chunk boundaries (the exact carry path)."""
+    rng = random.Random(nl)
+    ones = (1 << (64 * nl)) - 1
+    cases = [
+        (ones, 1),
+        (ones, ones),
+        (1 << (64 * nl), (1 << (64 * nl)) - 1),
+        (1 << (64 * nl - 1), 3),
+        (rng.getrandbits(64 * nl) | (1 << (64 * nl - 1)), rng.getrandbits(64 * nl - 5)),
+        ((ones << 64) | 2, 3),
+    ]
+    for s1, s2 in cases:
+        assert s2 <= s1
+        for k_odd in (0, 1):
+            for bit in (0, 1):
+                lo, hi = ff.fib_step_linear(s1, s2, k_odd, bit)
+                want = _step_ref(s1, s2, k_odd, bit)
+                assert (from_limbs(lo), from_limbs(hi)) == want, (nl, k_odd, bit)
+
+
+@pytest.mark.parametrize("nl", [1, 16, 17, 40, 3000])
+def test_last_factors_adversarial(ff, nl):
+    ones = (1 << (64 * nl)) - 1
+    for u, v in [(ones, ones), (ones, 1), (1 << (64 * nl), 1), (12345 << (64 * nl - 20), 777)]:
+        x, y = ff.last_factors(u, v)
+        assert from_limbs(x) == 2 * u + v
+        assert from_limbs(y) == 2 * u - v
+        x, none = ff.last_factors(v, u, with_y=False)
+        assert from_limbs(x) == 2 * v + u and none is None

@@ -120,7 +120,7 @@ extern "C" int ff_tables_init(void) {
 #define FF_THREADS 256
 
 // Outer pass over a tile of R rows x C columns of one block.
-template <bool INV>
+template <bool INV, bool TOP, bool R3>
 __global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, const FFTables *T,
                                                       FFPrimes PR) {
     extern __shared__ uint32_t lds[];
@@ -131,15 +131,15 @@ __global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, 
     const uint32_t blk = blockIdx.x / ps.tpb, ct = blockIdx.x - blk * ps.tpb;
     const uint32_t c0 = ct << ps.logC;
     uint32_t *data = X.data + (size_t)pi * ps.L + (size_t)blk * ps.Lb;
-    FFTile t{lds, ps.ld, ps.C, ps.logC, ps.rho, ps.r3 ? 3 : 1, 0};
+    FFTile t{lds, ps.ld, ps.C, ps.logC, ps.rho, R3 ? 3 : 1, 0};
     const int n = (int)ps.R << ps.logC;
 
     const uint32_t pinv = P.pinv;
-    const uint32_t scm = (INV && ps.top) ? ps.sc[pi] : 0;
-    const bool top = ps.top;
+    const uint32_t scm = (INV && TOP) ? ps.sc[pi] : 0;
+    const bool top = TOP;
     const int bw = ps.bw;
     const uint32_t Sb = ps.Sb, rootstep = ps.rootstep;
-    const int rho = ps.rho, r3 = ps.r3;
+    const int rho = ps.rho, r3 = R3;
     const uint32_t *src = X.src;
     const uint32_t nw = X.nw;
     // element (row position pos, tile column c) <-> data[pos * Sb + c0 + c]
@@ -165,17 +165,20 @@ __global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, 
     auto lds_store = [&](int pos, int c, uint32_t v) { lds[ff_pos_addr(t, pos, c)] = v; };
 
     if (!INV) {
-        if (!r3) {
+        if (!R3 && !TOP) {
             ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, true, true, load_fwd, store_fwd);
         } else {
-            // the radix-3 layer works on the LDS tile
+            // limbs are unpacked element by element into the LDS tile (16
+            // unpacks in flight per thread would not fit in registers), and
+            // the radix-3 layer works on the tile
             for (int i = threadIdx.x; i < n; i += blockDim.x) lds_store(i >> ps.logC, i & (ps.C - 1),
                                                                         load_fwd(i >> ps.logC, i & (ps.C - 1)));
-            ff_radix3_layer<false>(t, T, pi, P, ps.rstep3);
+            if (R3) ff_radix3_layer<false>(t, T, pi, P, ps.rstep3);
+            else __syncthreads();
             ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, false, true, lds_load, store_fwd);
         }
     } else {
-        if (!r3) {
+        if (!R3) {
             ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, true, load_inv, store_inv);
         } else {
             ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, false, load_inv, lds_store);
@@ -189,23 +192,24 @@ __global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, 
 // Inner blocks: forward transform of a (and b), pointwise product, inverse,
 // all in LDS. small = 1: the whole transform, read from the limbs and
 // scaled for the CRT at the end.
-__global__ __launch_bounds__(FF_THREADS) void k_inner(FFInner in, FFConvSet cs, const FFTables *T, FFPrimes PR) {
+template <bool SMALL, bool R3>
+__global__ __launch_bounds__(FF_THREADS, 4) void k_inner(FFInner in, FFConvSet cs, const FFTables *T, FFPrimes PR) {
     extern __shared__ uint32_t lds[];
     const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
     const FFPrime P = PR.p[pi];
     const uint32_t p = P.p;
     const FFConv C = cs.c[item];
-    const bool square = in.small ? (C.bsrc == nullptr) : (C.fb == nullptr);
-    const uint32_t n = in.small ? in.L : in.S;
+    const bool square = SMALL ? (C.bsrc == nullptr) : (C.fb == nullptr);
+    const uint32_t n = SMALL ? in.L : in.S;
     const int words = (int)(n + (n >> 4) + 16);
     uint32_t *A = lds, *B = lds + words;
-    FFTile ta{A, 1, 1, 0, in.s, in.small && in.r3 ? 3 : 1, 1};
+    FFTile ta{A, 1, 1, 0, in.s, R3 ? 3 : 1, 1};
     FFTile tb = ta;
     tb.s = B;
-    const size_t off = (size_t)pi * in.L + (in.small ? 0 : (size_t)blockIdx.x * in.S);
+    const size_t off = (size_t)pi * in.L + (SMALL ? 0 : (size_t)blockIdx.x * in.S);
 
     const uint32_t pinv = P.pinv;
-    const bool small = in.small;
+    const bool small = SMALL;
     const int bw = in.bw;
     const uint32_t sc = in.sc[pi];
     uint32_t *dst = C.fa + off;
@@ -227,13 +231,28 @@ __global__ __launch_bounds__(FF_THREADS) void k_inner(FFInner in, FFConvSet cs, 
     auto lds_a_st = [&](int pos, int c, uint32_t v) { A[ff_pos_addr(ta, pos, c)] = v; };
     auto lds_b_st = [&](int pos, int c, uint32_t v) { B[ff_pos_addr(tb, pos, c)] = v; };
 
-    if (!(small && in.r3)) {
+    if (!SMALL) {
         if (!square) {
             ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_b, lds_b_st);
             __syncthreads();
         }
         ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, true, true,
                    load_a, store_a);
+        return;
+    }
+    if (!R3) {
+        // one block read from limbs: unpack element by element into LDS
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            lds_a_st((int)i, 0, load_a((int)i, 0));
+            if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
+        }
+        __syncthreads();
+        if (!square) {
+            ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, false, false, lds_a, lds_b_st);
+            __syncthreads();
+        }
+        ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, false, true,
+                   lds_a, store_a);
         return;
     }
     // 3 * 2^k in one block: the radix-3 layers work on the LDS tiles
@@ -356,8 +375,15 @@ extern "C" hipError_t ff_launch_forward(const FFPlan *pl, const FFXformSet *xs, 
         dim3 grid(ps.nblocks * ps.tpb, 3 * nx);
         const int kind = i == 0 ? FK_FWD_TOP : FK_FWD;
         ff_prof_begin(kind, st);
-        hipLaunchKernelGGL(k_outer<false>, grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs, g_tables,
-                           g_primes_arg);
+        if (i == 0 && ps.r3)
+            hipLaunchKernelGGL((k_outer<false, true, true>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
+                               g_tables, g_primes_arg);
+        else if (i == 0)
+            hipLaunchKernelGGL((k_outer<false, true, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
+                               g_tables, g_primes_arg);
+        else
+            hipLaunchKernelGGL((k_outer<false, false, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
+                               g_tables, g_primes_arg);
         const double L4 = 4.0 * pl->L;
         ff_prof_end(kind, i == 0 ? 3.0 * (operand_bytes + nx * L4) : 3.0 * nx * 2 * L4, st);
     }
@@ -376,8 +402,13 @@ extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int 
     const int kind = in.small ? FK_SMALL : FK_INNER;
     const double L4 = 4.0 * pl->L;
     ff_prof_begin(kind, st);
-    hipLaunchKernelGGL(k_inner, grid, dim3(FF_THREADS), 2 * words * sizeof(uint32_t), st, in, *cs, g_tables,
-                       g_primes_arg);
+    const size_t lds = 2 * words * sizeof(uint32_t);
+    if (!in.small)
+        hipLaunchKernelGGL((k_inner<false, false>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
+    else if (in.r3)
+        hipLaunchKernelGGL((k_inner<true, true>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
+    else
+        hipLaunchKernelGGL((k_inner<true, false>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
     ff_prof_end(kind, in.small ? 3.0 * (operand_bytes + nc * L4) : 3.0 * (nc + ngeneral + nc) * L4, st);
     return hipGetLastError();
 }
@@ -388,8 +419,15 @@ extern "C" hipError_t ff_launch_inverse(const FFPlan *pl, const FFXformSet *xs, 
         dim3 grid(ps.nblocks * ps.tpb, 3 * nx);
         const int kind = i == 0 ? FK_INV_TOP : FK_INV;
         ff_prof_begin(kind, st);
-        hipLaunchKernelGGL(k_outer<true>, grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs, g_tables,
-                           g_primes_arg);
+        if (i == 0 && ps.r3)
+            hipLaunchKernelGGL((k_outer<true, true, true>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
+                               g_tables, g_primes_arg);
+        else if (i == 0)
+            hipLaunchKernelGGL((k_outer<true, true, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
+                               g_tables, g_primes_arg);
+        else
+            hipLaunchKernelGGL((k_outer<true, false, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
+                               g_tables, g_primes_arg);
         ff_prof_end(kind, 3.0 * nx * 8.0 * pl->L, st);
     }
     return hipGetLastError();

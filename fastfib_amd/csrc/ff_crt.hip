@@ -2,23 +2,28 @@
 //
 // The three residue arrays y_p[i] (exact convolution coefficient i mod p,
 // natural order, as the inverse top pass leaves them) are combined by
-// Garner's method into X_i < p1 p2 p3 < 2^93 and added into the output at
-// bit i * bw (crt.c:180-229, reference). Parallel form:
-//   pack   one thread per chunk of 64 coefficients, which fill exactly bw
-//          limbs; it writes them with a zero carry-in and records the
-//          chunk's carry-out H (bits above its bw limbs) and whether limbs
-//          2 .. bw-1 are all ones (then a carry could run through it);
-//   scan   the carry into chunk k+1 is H_k + e_{k+1}, e_{k+1} in {0, 1} the
-//          overflow of chunk k's low limbs plus its own carry-in: a
-//          generate/propagate scan over chunks. Only chunks with that
-//          all-ones flag can overflow, so when no chunk has it (the flag the
-//          pack kernel raises stays clear) e = 0 throughout, the scan exits
-//          at once and the carries are just the H's;
-//   apply  adds the carry into each chunk (ripple confined to the chunk),
-//          writes the final carry above the last chunk, and finds the
-//          result's top nonzero limb.
+// Garner's method into X_j < p1 p2 p3 < 2^93 (crt.c:180-206, reference),
+// and the product is N = sum_j X_j 2^(j bw) (crt.c:208-296). The reference
+// packs coefficients into limbs with one running carry per chunk; here every
+// output limb has its own thread:
+//   N = sum_i T_i 2^(64 i),  T_i = sum_j digit_i(X_j 2^(j bw)),
+// the base-2^64 digit i of each shifted coefficient; at most five
+// coefficients touch a limb, so T_i < 5 * 2^64. With u_i = lo(T_i) +
+// hi(T_{i-1}) <= 2^64 + 3 the remaining carries are 0/1: limb i =
+// u_i + c_i mod 2^64 with c_{i+1} = g_i | (p_i & c_i), g_i = u_i >= 2^64,
+// p_i = (u_i == 2^64 - 1). Inside a wavefront the carries come from one
+// 64-bit addition of the ballots (A = g, B = g | p: C = (A + B) ^ A ^ B),
+// across the block's waves from their carry-outs, across blocks from each
+// block's generate/propagate pair:
+//   pack   Garner into LDS, T_i, u_i, in-block carries; limbs written for a
+//          zero carry into the block; block (G, P) recorded;
+//   scan   only if some block propagates (P = 1: all its 256 limbs are all
+//          ones, which needs adversarial operands), the exact carry into
+//          every block by a lookahead scan; otherwise carry_in(w) = G(w-1);
+//   apply  adds the block's carry (ripple bounded by the block) and books
+//          the top nonzero limb.
 // This replaces the reference's 8 sequential chunks plus carry fix
-// (crt.c:326-336) with a segmented parallel carry scan.
+// (crt.c:326-336) with a parallel carry-lookahead over all limbs.
 #include <stdint.h>
 
 #include "ff_internal.h"
@@ -76,125 +81,123 @@ __device__ __forceinline__ u128 ff_garner(uint32_t r1, uint32_t r2, uint32_t r3,
     return (u128)x3 * G.p12 + lo;
 }
 
-#define PACK_THREADS 64
+#define CRT_LIMBS 256  // limbs per block, one per thread
+// Coefficients a block can touch: limbs [i0 - 1, i0 + 256) span 257 * 64 bits
+// and a coefficient 93 more, so at most (257 * 64 + 93) / bw + 2 of them;
+// 2070 at the narrowest width, 8 (mul_split.c:151-152).
+#define CRT_MAXC 2112
 
-// One thread per chunk of 64 coefficients, read 16 at a time as 128-bit
-// loads (all issued before use).
-__global__ __launch_bounds__(PACK_THREADS) void k_crt_pack(FFCrtSet js, FFGarner G) {
+// First coefficient touching limb i: coefficient j spans bits [j bw, j bw + 93).
+__device__ __forceinline__ uint64_t first_coef(uint64_t i, int bw) {
+    const uint64_t lo = 64 * i;
+    return lo >= 93 ? (lo - 93) / bw + 1 : 0;
+}
+
+// T_i = sum over the coefficients touching limb i of digit i of X_j 2^(j bw).
+__device__ __forceinline__ u128 limb_sum(uint64_t i, int bw, uint64_t j0, const uint64_t *xlo, const uint32_t *xhi,
+                                         uint64_t jbase, uint64_t jend) {
+    u128 T = 0;
+    const uint64_t bit = 64 * i;
+    for (uint64_t j = first_coef(i, bw) > j0 ? first_coef(i, bw) : j0; j < jend; j++) {
+        const uint64_t jb = j * (uint64_t)bw;
+        if (jb >= bit + 64) break;
+        const u128 X = ((u128)xhi[j - jbase] << 64) | xlo[j - jbase];
+        uint64_t d;
+        if (jb >= bit) d = (uint64_t)(X << (unsigned)(jb - bit));
+        else d = (uint64_t)(X >> (unsigned)(bit - jb));
+        T += d;
+    }
+    return T;
+}
+
+// One block per CRT_LIMBS output limbs.
+__global__ __launch_bounds__(CRT_LIMBS) void k_crt_pack(FFCrtSet js, FFGarner G) {
     const FFCrtJob &J = js.j[blockIdx.y];
-    const uint64_t nchunks = (J.conv + 63) / 64;
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x * (uint64_t)blockDim.x >= nchunks) return;
+    const uint64_t nblk = (J.nout + CRT_LIMBS - 1) / CRT_LIMBS;
+    if (blockIdx.x >= nblk) return;
     const int bw = J.bw;
-    const u128 mask = ((u128)1 << bw) - 1;
+    const uint64_t i0 = (uint64_t)blockIdx.x * CRT_LIMBS;
+    const uint64_t i1 = i0 + CRT_LIMBS < J.nout ? i0 + CRT_LIMBS : J.nout;
+    // coefficients touching limbs [i0 - 1, i1): limb i0 - 1 gives the block's first hi
+    const uint64_t lim = J.conv < J.L ? J.conv : J.L;
+    const uint64_t jbase = first_coef(i0 ? i0 - 1 : 0, bw);
+    uint64_t jend = (64 * i1 + bw - 1) / bw;
+    if (jend > lim) jend = lim;
+    __shared__ uint64_t xlo[CRT_MAXC];
+    __shared__ uint32_t xhi[CRT_MAXC];
+    __shared__ uint64_t hi_s[CRT_LIMBS];
+    __shared__ uint64_t wave_c[CRT_LIMBS / 64];
     const uint32_t *__restrict__ y1 = J.y;
     const uint32_t *__restrict__ y2 = J.y + J.L;
     const uint32_t *__restrict__ y3 = J.y + 2 * (size_t)J.L;
-    uint64_t *__restrict__ w = J.out + k * (uint64_t)bw;
-    u128 buf = 0, c = 0;
-    int nbits = 0, q = 0;
-    uint32_t ones = 1;
-    const uint64_t i0 = 64 * k;
-    // the chunk's coefficients past conv (and past L) are zero
-    const uint64_t lim = J.conv < J.L ? J.conv : J.L;
-    const bool live = k < nchunks;
-    for (int part = 0; part < 4 && live; part++) {
-        const uint64_t b0 = i0 + 16 * part;
-        uint32_t r1[16], r2[16], r3[16];
-        if (b0 + 16 <= lim) {
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const uint4 a1 = *(const uint4 *)(y1 + b0 + 4 * v);
-                const uint4 a2 = *(const uint4 *)(y2 + b0 + 4 * v);
-                const uint4 a3 = *(const uint4 *)(y3 + b0 + 4 * v);
-                r1[4 * v] = a1.x, r1[4 * v + 1] = a1.y, r1[4 * v + 2] = a1.z, r1[4 * v + 3] = a1.w;
-                r2[4 * v] = a2.x, r2[4 * v + 1] = a2.y, r2[4 * v + 2] = a2.z, r2[4 * v + 3] = a2.w;
-                r3[4 * v] = a3.x, r3[4 * v + 1] = a3.y, r3[4 * v + 2] = a3.z, r3[4 * v + 3] = a3.w;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const bool in = b0 + j < lim;
-                r1[j] = in ? y1[b0 + j] : 0;
-                r2[j] = in ? y2[b0 + j] : 0;
-                r3[j] = in ? y3[b0 + j] : 0;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const u128 X = ff_garner(r1[j], r2[j], r3[j], G);
-            const u128 acc = X + c;
-            buf |= (acc & mask) << nbits;
-            c = acc >> bw;
-            nbits += bw;
-            if (nbits >= 64) {
-                const uint64_t limb = (uint64_t)buf;
-                w[q] = limb;
-                if (q >= 2 && limb != ~0ull) ones = 0;
-                q++;
-                buf >>= 64;
-                nbits -= 64;
-            }
-        }
+    for (uint64_t j = jbase + threadIdx.x; j < jend; j += CRT_LIMBS) {
+        const u128 X = ff_garner(y1[j], y2[j], y3[j], G);
+        xlo[j - jbase] = (uint64_t)X;
+        xhi[j - jbase] = (uint32_t)(X >> 64);
     }
-    if (live) {
-        FFChunk r;
-        r.h_lo = (uint64_t)c;
-        r.h_hi = (uint64_t)(c >> 64);
-        r.ones = ones;
-        r.pad_ = 0;
-        J.chunks[k] = r;
+    __syncthreads();
+    const uint64_t i = i0 + threadIdx.x;
+    const bool live = i < i1;
+    u128 T = live ? limb_sum(i, bw, jbase, xlo, xhi, jbase, jend) : 0;
+    hi_s[threadIdx.x] = (uint64_t)(T >> 64);
+    uint64_t hprev = 0;
+    if (threadIdx.x == 0 && i0 > 0) hprev = (uint64_t)(limb_sum(i0 - 1, bw, jbase, xlo, xhi, jbase, jend) >> 64);
+    __syncthreads();
+    if (threadIdx.x > 0) hprev = hi_s[threadIdx.x - 1];
+    const u128 u = (u128)(uint64_t)T + hprev;
+    const uint64_t ulo = (uint64_t)u;
+    const bool g = live && (u >> 64) != 0;
+    const bool pp = live && !g && ulo == ~0ull;
+    // carries inside each wavefront: lane k's (generate, propagate) are bit k
+    // of A = g and B = g | p, so A + B + c_in carries exactly where the limbs
+    // do, and C = (A + B + c_in) ^ A ^ B has bit k = carry into lane k
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t A = __ballot(g), B = A | __ballot(pp);
+    const uint64_t wg = (uint64_t)(((u128)A + B) >> 64);  // carry out of the wave, zero carry in
+    const uint64_t wp = (~B) == 0;                          // every lane propagates (then wg = 0)
+    if (lane == 0) wave_c[wave] = wg | (wp << 1);
+    __syncthreads();
+    uint64_t cin = 0;  // carry into this wave from the earlier waves of the block
+    for (int w = 0; w < wave; w++) {
+        const uint64_t e = wave_c[w];
+        cin = (e & 1) | ((e >> 1) & cin);
     }
-    // one plain store per workgroup: did any of its chunks raise the flag
-    const int any = __syncthreads_or(live && ones);
-    if (threadIdx.x == 0) J.wgflag[blockIdx.x] = (uint32_t)any;
+    const uint64_t C = (A + B + cin) ^ A ^ B;
+    const uint64_t cbit = (C >> lane) & 1;
+    if (live) J.out[i] = ulo + cbit;
+    // the block's generate/propagate pair
+    if (threadIdx.x == 0) {
+        uint64_t Gb = 0, Pb = 1;
+        for (int w = 0; w < CRT_LIMBS / 64; w++) {
+            const uint64_t e = wave_c[w];
+            Gb = (e & 1) | ((e >> 1) & Gb);
+            Pb = Pb & (e >> 1);
+        }
+        J.bgp[blockIdx.x] = (uint32_t)(Gb | (Pb << 1));
+    }
 }
 
-// Exact carry-lookahead over chunks, one workgroup (1024 threads) per
-// product, only when some chunk could propagate a carry.
-// g_k: chunk k overflows with e_k = 0; p_k: it overflows with e_k = 1.
+// Exact carry into every block, only when some block propagates.
 #define SCAN_THREADS 1024
-
-__device__ __forceinline__ u128 chunk_h(const FFCrtJob &J, uint64_t k) {
-    const FFChunk pc = J.chunks[k];
-    return ((u128)pc.h_hi << 64) | pc.h_lo;
-}
-
-__device__ __forceinline__ void chunk_gp(const FFCrtJob &J, uint64_t k, uint32_t &g, uint32_t &p) {
-    const FFChunk ch = J.chunks[k];
-    if (!ch.ones) {
-        g = p = 0;
-        return;
-    }
-    const u128 hprev = k > 0 ? chunk_h(J, k - 1) : 0;
-    const uint64_t *w = J.out + k * (uint64_t)J.bw;
-    const u128 low = ((u128)w[1] << 64) | w[0];
-    const u128 s = low + hprev;
-    const uint32_t ov0 = s < low;  // carry out of 128 bits
-    g = ov0;
-    p = ov0 | (s == ~(u128)0);
-}
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_crt_scan(FFCrtSet js) {
     const FFCrtJob &J = js.j[blockIdx.x];
-    const uint64_t nchunks = (J.conv + 63) / 64;
-    const uint64_t nwg = (nchunks + PACK_THREADS - 1) / PACK_THREADS;
+    const uint64_t nblk = (J.nout + CRT_LIMBS - 1) / CRT_LIMBS;
     int mine = 0;
-    for (uint64_t i = threadIdx.x; i < nwg; i += SCAN_THREADS) mine |= J.wgflag[i] != 0;
+    for (uint64_t b = threadIdx.x; b < nblk; b += SCAN_THREADS) mine |= (J.bgp[b] >> 1) & 1;
     const int any = __syncthreads_or(mine);
     if (threadIdx.x == 0) {
         *J.mode = (uint32_t)any;
         *J.lenkey = 0;  // k_crt_apply books the top limb by atomicMax
     }
-    if (!any) return;  // no chunk can propagate: the H's are the carries
-    const uint64_t per = (nchunks + SCAN_THREADS - 1) / SCAN_THREADS;
-    const uint64_t k0 = threadIdx.x * per, k1 = k0 + per < nchunks ? k0 + per : nchunks;
+    if (!any) return;  // carry into block w is G(w-1)
+    const uint64_t per = (nblk + SCAN_THREADS - 1) / SCAN_THREADS;
+    const uint64_t b0 = threadIdx.x * per, b1 = b0 + per < nblk ? b0 + per : nblk;
     uint32_t Gs = 0, Ps = 1;
-    for (uint64_t k = k0; k < k1; k++) {
-        uint32_t g, p;
-        chunk_gp(J, k, g, p);
-        Gs = g | (p & Gs);
-        Ps = p & Ps;
+    for (uint64_t b = b0; b < b1; b++) {
+        const uint32_t e = J.bgp[b];
+        Gs = (e & 1) | ((e >> 1) & Gs);
+        Ps = Ps & (e >> 1);
     }
     __shared__ uint32_t sg[SCAN_THREADS], sp[SCAN_THREADS];
     sg[threadIdx.x] = Gs;
@@ -209,20 +212,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_crt_scan(FFCrtSet js) {
         sp[threadIdx.x] = p2 & p1;
         __syncthreads();
     }
-    uint32_t e = threadIdx.x ? sg[threadIdx.x - 1] : 0;
-    for (uint64_t k = k0; k < k1; k++) {
-        J.cin[k] = (k > 0 ? chunk_h(J, k - 1) : 0) + e;
-        uint32_t g, p;
-        chunk_gp(J, k, g, p);
-        e = g | (p & e);
+    uint32_t c = threadIdx.x ? sg[threadIdx.x - 1] : 0;
+    for (uint64_t b = b0; b < b1; b++) {
+        J.bcin[b] = (uint8_t)c;
+        const uint32_t e = J.bgp[b];
+        c = (e & 1) | ((e >> 1) & c);
     }
-    if (k1 == nchunks && k0 < k1) J.cin[nchunks] = chunk_h(J, nchunks - 1) + e;
 }
 
 // Top nonzero limb of d[i0..i1) booked into the product's length key.
-__device__ __forceinline__ void book_top(const FFCrtJob &J, const uint64_t *d, uint64_t i0, uint64_t i1) {
+__device__ __forceinline__ void book_top(const FFCrtJob &J, uint64_t i0, uint64_t i1) {
     for (uint64_t i = i1; i-- > i0;) {
-        const uint64_t v = d[i];
+        const uint64_t v = J.out[i];
         if (v) {
             const unsigned long long key = ((unsigned long long)(i + 1) << 8) | (unsigned)(64 - __clzll((long long)v));
             atomicMax(J.lenkey, key);
@@ -231,67 +232,41 @@ __device__ __forceinline__ void book_top(const FFCrtJob &J, const uint64_t *d, u
     }
 }
 
-// Adds the incoming carry into each chunk; the final carry goes above. The
-// chunks that hold limbs from lo_top up (where the top nonzero limb must
-// be) book it.
-__global__ __launch_bounds__(256) void k_crt_apply(FFCrtSet js, uint64_t lo_top0, uint64_t lo_top1,
-                                                   uint64_t lo_top2, uint64_t lo_top3) {
+// One thread per block of limbs: adds the block's carry-in (ripple bounded by
+// the block) and, for blocks holding limbs from nout - 3 up (where the top
+// nonzero limb of an na x nb product lies), books the top limb.
+__global__ __launch_bounds__(256) void k_crt_apply(FFCrtSet js) {
     const FFCrtJob &J = js.j[blockIdx.y];
-    const uint64_t lo_top = blockIdx.y == 0 ? lo_top0 : blockIdx.y == 1 ? lo_top1 : blockIdx.y == 2 ? lo_top2 : lo_top3;
-    const uint64_t nchunks = (J.conv + 63) / 64;
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > nchunks) return;
-    const bool exact = *J.mode != 0;
-    const uint64_t base = k * (uint64_t)J.bw;
-    uint64_t *w = J.out + base;
-    const u128 c = exact ? J.cin[k] : (k > 0 ? chunk_h(J, k - 1) : (u128)0);
-    if (k == nchunks) {
-        // limbs above the last chunk: the final carry, then zeros up to nout
-        w[0] = (uint64_t)c;
-        w[1] = (uint64_t)(c >> 64);
-        for (uint64_t i = base + 2; i < J.nout; i++) J.out[i] = 0;
-        const uint64_t top = J.nout > base + 2 ? J.nout : base + 2;
-        if (top > lo_top) book_top(J, J.out, base > lo_top ? base : lo_top, top);
-        return;
-    }
-    if (c != 0) {
-        const u128 low = ((u128)w[1] << 64) | w[0];
-        const u128 s = low + c;
-        w[0] = (uint64_t)s;
-        w[1] = (uint64_t)(s >> 64);
-        if (s < low) {
-            for (int q = 2; q < J.bw; q++) {
-                if (++w[q] != 0) break;
-            }
+    const uint64_t nblk = (J.nout + CRT_LIMBS - 1) / CRT_LIMBS;
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    const uint32_t c = *J.mode ? J.bcin[b] : (b ? J.bgp[b - 1] & 1 : 0);
+    const uint64_t i0 = b * CRT_LIMBS, i1 = i0 + CRT_LIMBS < J.nout ? i0 + CRT_LIMBS : J.nout;
+    if (c) {
+        for (uint64_t i = i0; i < i1; i++) {
+            if (++J.out[i] != 0) break;
         }
     }
-    const uint64_t end = base + J.bw;
-    if (end > lo_top) book_top(J, J.out, base > lo_top ? base : lo_top, end);
+    const uint64_t lo_top = J.nout >= 3 ? J.nout - 3 : 0;
+    if (i1 > lo_top) book_top(J, i0 > lo_top ? i0 : lo_top, i1);
 }
 
-// Algorithmic bytes: pack reads 3 residues per coefficient and writes the
-// limbs and one 24-byte record per chunk; scan (exact path only) reads the
-// records and two limbs per chunk; apply reads one record per chunk.
-extern "C" hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_chunks, double conv_total,
+// Algorithmic bytes: pack reads 3 residues per coefficient and writes each
+// output limb once; scan and apply touch one word per block of 256 limbs.
+extern "C" hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_nout, double conv_total,
                                     double out_limbs_total, hipStream_t st) {
     garner_init();
-    const double chunks = conv_total / 64.0;
-    dim3 g1((unsigned)((max_chunks + PACK_THREADS - 1) / PACK_THREADS), njobs);
+    const uint64_t max_blk = (max_nout + CRT_LIMBS - 1) / CRT_LIMBS;
+    dim3 g1((unsigned)max_blk, njobs);
     ff_prof_begin(FK_CRT_PACK, st);
-    hipLaunchKernelGGL(k_crt_pack, g1, dim3(PACK_THREADS), 0, st, *js, g_garner);
-    ff_prof_end(FK_CRT_PACK, 12.0 * conv_total + 8.0 * out_limbs_total + 24.0 * chunks, st);
+    hipLaunchKernelGGL(k_crt_pack, g1, dim3(CRT_LIMBS), 0, st, *js, g_garner);
+    ff_prof_end(FK_CRT_PACK, 12.0 * conv_total + 8.0 * out_limbs_total, st);
     ff_prof_begin(FK_CRT_SCAN, st);
     hipLaunchKernelGGL(k_crt_scan, dim3(njobs), dim3(SCAN_THREADS), 0, st, *js);
     ff_prof_end(FK_CRT_SCAN, 0.0, st);
-    uint64_t lo[4] = {0, 0, 0, 0};
-    for (int i = 0; i < njobs && i < 4; i++) {
-        // the product of an na-limb and an nb-limb number has its top limb
-        // at index >= na + nb - 2 = nout - 3 (nout = na + nb + 1)
-        lo[i] = js->j[i].nout >= 3 ? js->j[i].nout - 3 : 0;
-    }
-    dim3 g3((unsigned)((max_chunks + 1 + 255) / 256), njobs);
+    dim3 g3((unsigned)((max_blk + 255) / 256), njobs);
     ff_prof_begin(FK_CRT_APPLY, st);
-    hipLaunchKernelGGL(k_crt_apply, g3, dim3(256), 0, st, *js, lo[0], lo[1], lo[2], lo[3]);
-    ff_prof_end(FK_CRT_APPLY, 24.0 * chunks, st);
+    hipLaunchKernelGGL(k_crt_apply, g3, dim3(256), 0, st, *js);
+    ff_prof_end(FK_CRT_APPLY, 4.0 * out_limbs_total / CRT_LIMBS, st);
     return hipGetLastError();
 }

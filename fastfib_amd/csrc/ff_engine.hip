@@ -71,13 +71,14 @@ static ProdShape product_shape(uint64_t bits_a, uint64_t bits_b) {
     return s;
 }
 
-// Limbs the CRT writes for a product: whole chunks of 64 coefficients
-// (bw limbs each) plus two limbs of final carry, at least na + nb + 1.
+// Limbs the CRT writes for a product: na + nb + 1, as ntt_result
+// (mul_ntt.c:61-73) allocates (conv * bw bits can overshoot by bw zeros).
 static uint64_t out_room(const ProdShape &s, uint64_t na, uint64_t nb) {
-    const uint64_t nch = (s.conv + 63) / 64;
-    const uint64_t a = nch * s.bw + 2, b = na + nb + 1;
-    return a > b ? a : b;
+    (void)s;
+    return na + nb + 1;
 }
+
+#define FF_CRT_LIMBS 256  // limbs per CRT block (ff_crt.hip)
 
 // ------------------------------------------------------------------ length slots
 // Device words where kernels book a result's normalized length and top-limb
@@ -177,17 +178,16 @@ static void mul_group(const MulItem *it, int cnt, const ProdShape &s) {
     FFCrtSet crt = {};
     int nfwd = 0;
     std::vector<void *> tmp;
-    uint64_t max_chunks = 0;
+    uint64_t max_nout = 0;
     double opbytes = 0, conv_total = 0, out_limbs = 0;
     int ngeneral = 0;
     for (int i = 0; i < cnt; i++) {
-        const uint64_t nch = (it[i].conv + 63) / 64;
-        const uint64_t nwg = (nch + 63) / 64;
+        const uint64_t nout = it[i].na + (it[i].b ? it[i].nb : it[i].na) + 1;
+        const uint64_t nblk = (nout + FF_CRT_LIMBS - 1) / FF_CRT_LIMBS;
         const size_t o_fb = align256(words * 4);
-        const size_t o_ch = o_fb + (it[i].b ? align256(words * 4) : 0);
-        const size_t o_cin = o_ch + align256(nch * sizeof(FFChunk));
-        const size_t o_wg = o_cin + align256((nch + 1) * sizeof(u128));
-        const size_t o_mode = o_wg + align256(nwg * 4);
+        const size_t o_bgp = o_fb + (it[i].b ? align256(words * 4) : 0);
+        const size_t o_bcin = o_bgp + align256(nblk * 4);
+        const size_t o_mode = o_bcin + align256(nblk);
         char *base = (char *)ff_dalloc(o_mode + 256);
         tmp.push_back(base);
         uint32_t *fa = (uint32_t *)base;
@@ -201,13 +201,11 @@ static void mul_group(const MulItem *it, int cnt, const ProdShape &s) {
         }
         cv.c[i] = FFConv{asrc, bsrc, (uint32_t)(2 * it[i].na), (uint32_t)(2 * it[i].nb), fa, fb};
         inv.x[i] = FFXform{nullptr, 0, fa};
-        max_chunks = nch > max_chunks ? nch : max_chunks;
-        const uint64_t nout = it[i].na + (it[i].b ? it[i].nb : it[i].na) + 1;
+        max_nout = nout > max_nout ? nout : max_nout;
         crt.j[i] = FFCrtJob{fa,
                             it[i].out,
-                            (FFChunk *)(base + o_ch),
-                            (u128 *)(base + o_cin),
-                            (uint32_t *)(base + o_wg),
+                            (uint32_t *)(base + o_bgp),
+                            (uint8_t *)(base + o_bcin),
                             (uint32_t *)(base + o_mode),
                             g_slots + it[i].slot,
                             nout,
@@ -217,7 +215,7 @@ static void mul_group(const MulItem *it, int cnt, const ProdShape &s) {
         opbytes += 8.0 * (it[i].na + (it[i].b ? it[i].nb : 0));
         ngeneral += it[i].b ? 1 : 0;
         conv_total += (double)it[i].conv;
-        out_limbs += (double)(nch * s.bw);
+        out_limbs += (double)nout;
     }
     for (int f = 0; f * FF_MAXB < nfwd; f++) {
         const int nx = nfwd - f * FF_MAXB < FF_MAXB ? nfwd - f * FF_MAXB : FF_MAXB;
@@ -225,7 +223,7 @@ static void mul_group(const MulItem *it, int cnt, const ProdShape &s) {
     }
     FF_CHECK(ff_launch_conv(&pl, &cv, cnt, ngeneral, opbytes, g_st));
     FF_CHECK(ff_launch_inverse(&pl, &inv, cnt, g_st));
-    FF_CHECK(ff_launch_crt(&crt, cnt, max_chunks, conv_total, out_limbs, g_st));
+    FF_CHECK(ff_launch_crt(&crt, cnt, max_nout, conv_total, out_limbs, g_st));
     for (void *p : tmp) ff_dfree(p);
 }
 

@@ -79,21 +79,14 @@ struct FFPlan {
     FFInner inner;
 };
 
-// Per-chunk carry record written by the CRT pack kernel.
-struct FFChunk {
-    uint64_t h_lo, h_hi;  // carry out of the chunk's bw limbs (bits above)
-    uint32_t ones;        // limbs 2 .. bw-1 of the chunk are all ones
-    uint32_t pad_;
-};
-
-// One product's CRT: residues [3][L] -> limbs.
+// One product's CRT: residues [3][L] -> nout limbs, one thread per limb in
+// blocks of 256 limbs (ff_crt.hip).
 struct FFCrtJob {
     const uint32_t *y;       // residues [3][L]
-    uint64_t *out;           // limbs: room for max(nout, nchunks * bw + 2)
-    FFChunk *chunks;         // nchunks records
-    unsigned __int128 *cin;  // nchunks + 1 incoming carries (exact path)
-    uint32_t *wgflag;        // per pack workgroup: one of its chunks could propagate a carry
-    uint32_t *mode;          // written by the scan: 1 = exact carries in cin, 0 = the H's
+    uint64_t *out;           // nout limbs
+    uint32_t *bgp;           // per block of limbs: generate | propagate << 1
+    uint8_t *bcin;           // per block: exact carry in (written only when some block propagates)
+    uint32_t *mode;          // written by the scan: 1 = exact carries in bcin, 0 = generate of the block below
     unsigned long long *lenkey;  // (top nonzero limb + 1) << 8 | its bit length, by atomicMax
     uint64_t nout;
     uint64_t conv;
@@ -127,7 +120,7 @@ hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int nc, int nge
 hipError_t ff_launch_inverse(const FFPlan *pl, const FFXformSet *xs, int nx, hipStream_t st);
 
 // ff_crt.hip
-hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_chunks, double conv_total,
+hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_nout, double conv_total,
                          double out_limbs_total, hipStream_t st);
 }
 

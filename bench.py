@@ -109,6 +109,86 @@ def cpu_baseline(n: int, seconds: float) -> dict | None:
                       f"median {sorted(times)[len(times) // 2] * 1e3:.2f} ms; pool threads = cores"}
 
 
+BATCH_PRIMES = (2305843009213693951, 1152921504606846883)
+
+
+def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float) -> dict:
+    """The uneven batch: count products, operand bits log-uniform in
+    [2^12, 2^24], 30% squares, uniform limbs (fastfib_amd.workloads), one
+    step = the whole batch. Run forms: one product at a time (plain), grouped
+    work lists, the grouped launches replayed from a HIP graph. Every form's
+    outputs are checked: the first form's per product modulo two primes, the
+    others byte-equal to it."""
+    import numpy as np
+    from fastfib_amd.workloads import operand_bits, operand_limbs, uneven_mix
+
+    mix = uneven_mix(seed, count)
+    gbit = operand_bits(mix) / 1e9
+    rng = np.random.default_rng(seed)
+    pairs, res, host = [], [], []
+    for ba, bb in mix:
+        A = operand_limbs(rng, ba)
+        B = None if bb is None else operand_limbs(rng, bb)
+        a = int.from_bytes(A.tobytes(), "little")
+        b = a if B is None else int.from_bytes(B.tobytes(), "little")
+        res.append(tuple(a % p * (b % p) % p for p in BATCH_PRIMES))
+        pairs.append((ff.Device(A), None if B is None else ff.Device(B)))
+        host.append((A, B))
+    ff.sync()
+    forms = [("plain", ff.BATCH_PLAIN), ("grouped", ff.BATCH_GROUPED), ("graph", ff.BATCH_GRAPH)]
+    out, ref_limbs, ok = {}, None, True
+    for name, flags in forms:
+        bt = ff.Batch(pairs, flags)
+        bt.run()
+        bt.finish()
+        n_steps = steps if name != "plain" else max(1, min(steps, 3))
+        ff.sync()
+        t0 = time.perf_counter()
+        for _ in range(n_steps):
+            bt.run()
+        bt.finish()
+        dt = (time.perf_counter() - t0) / n_steps
+        limbs = [o.limbs() for o in bt.outs]
+        if ref_limbs is None:
+            good = all(tuple(int.from_bytes(x.tobytes(), "little") % p for p in BATCH_PRIMES) == r
+                       for x, r in zip(limbs, res))
+            ref_limbs = limbs
+        else:
+            good = all(np.array_equal(x, y) for x, y in zip(limbs, ref_limbs))
+        ok &= good
+        out[name] = {"ms": round(dt * 1e3, 3), "gbit_products_per_s": round(gbit / dt, 2), "bit_exact": good,
+                     "steps": n_steps}
+        bt.close()
+    best = max(("grouped", "graph"), key=lambda k: out[k]["gbit_products_per_s"])
+    leg = {"workload": f"{count} uneven products, operands 2^12..2^24 bits log-uniform, 30% squares, seed {seed}",
+           "operand_gbit": round(gbit, 3), "forms": out, "bit_exact": ok,
+           "speedup_vs_plain": round(out[best]["gbit_products_per_s"] / out["plain"]["gbit_products_per_s"], 2)}
+    leg["cpu_baseline"] = batch_cpu_baseline(host, gbit, cpu_seconds)
+    return leg
+
+
+def batch_cpu_baseline(host, gbit: float, seconds: float) -> dict | None:
+    """The reference's big_mul / big_sqr over the same products, in the
+    batch's order, as far as the time budget goes (whole passes)."""
+    try:
+        from oracle.oracle import Reference
+        ref = Reference()
+    except Exception as e:
+        print(f"cpu baseline unavailable: {e}", file=sys.stderr)
+        return None
+    items = [(A, B, B is None) for A, B in host]
+    times = []
+    t_end = time.perf_counter() + seconds
+    while not times or time.perf_counter() < t_end:
+        times.append(ref.time_mul_list(items))
+        if len(times) >= 5:
+            break
+    best = min(times)
+    return {"value": round(gbit / best, 3), "unit": "Gbit-products/s", "cores": ref.pool_cpus(),
+            "kind": "reference", "sample": f"the whole batch through big_mul/big_sqr x {len(times)}, best "
+                                           f"{best * 1e3:.1f} ms"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +196,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=10 ** 8)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--batch-count", type=int, default=4096, help="uneven batch size (0: skip the batch leg)")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-seed", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (products shard by prime or "
@@ -159,6 +242,10 @@ def main() -> int:
                 "avg_launch_us": round(per_launch_ms * 1e3, 2), "launches": d["launches"]}
 
     cpu = cpu_baseline(n, args.cpu_seconds)
+    batch = None
+    if args.batch_count > 0:
+        batch = batch_leg(ff, args.batch_count, args.batch_seed, args.batch_steps, args.cpu_seconds)
+        ok &= batch["bit_exact"]
     line = {
         "metric": METRIC, "value": round(ms, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False, "scaling": "weak",
@@ -168,6 +255,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(cpu["value"] / ms, 2) if cpu else None,
+        "batch": batch,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 4),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
                     for k, v in stats.items()},

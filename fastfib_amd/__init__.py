@@ -187,6 +187,11 @@ class Device:
             arr = to_limbs(value) if isinstance(value, int) else np.ascontiguousarray(value, dtype=np.uint64)
             _check(lib().ff_dbig_upload(ctypes.byref(self.h), arr.ctypes.data_as(_u64p), arr.size))
 
+    def assign(self, value) -> None:
+        """New value, written into this number's own memory when it fits."""
+        arr = to_limbs(value) if isinstance(value, int) else np.ascontiguousarray(value, dtype=np.uint64)
+        _check(lib().ff_dbig_upload(ctypes.byref(self.h), arr.ctypes.data_as(_u64p), arr.size))
+
     @property
     def n(self) -> int:
         return int(self.h.n)
@@ -224,3 +229,73 @@ def dfibonacci(n: int) -> Device:
     out = Device()
     _check(lib().ff_dfibonacci(n, ctypes.byref(out.h)))
     return out
+
+
+class Prod(ctypes.Structure):
+    """ff_prod: one product of a batch (device pointers)."""
+    _fields_ = [("a", ctypes.c_void_p), ("na", ctypes.c_size_t), ("b", ctypes.c_void_p), ("nb", ctypes.c_size_t),
+                ("out", ctypes.c_void_p), ("nout", ctypes.c_size_t), ("bits_a", ctypes.c_uint64),
+                ("bits_b", ctypes.c_uint64)]
+
+
+BATCH_PLAIN, BATCH_GROUPED, BATCH_GRAPH, BATCH_TUNED, BATCH_LEARNED = 0, 1, 2, 4, 8
+
+
+def _batch_sigs(L) -> None:
+    if getattr(L, "_batch_sigs", False):
+        return
+    L.ff_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(Prod), ctypes.c_size_t,
+                                  ctypes.c_int]
+    L.ff_batch_run.argtypes = [ctypes.c_void_p]
+    L.ff_batch_finish.argtypes = [ctypes.c_void_p, ctypes.POINTER(Prod)]
+    L.ff_batch_destroy.argtypes = [ctypes.c_void_p]
+    L._batch_sigs = True
+
+
+class Batch:
+    """A fixed list of device products run together (ff_batch_*).
+
+    pairs: (a, b) of Device numbers; b None or b is a squares. Outputs are
+    allocated here with room for na + nb + 1 limbs."""
+
+    def __init__(self, pairs, flags: int = BATCH_GROUPED):
+        L = lib()
+        _batch_sigs(L)
+        self.pairs = list(pairs)
+        self.outs = []
+        self.items = (Prod * max(1, len(self.pairs)))()
+        for i, (a, b) in enumerate(self.pairs):
+            sq = b is None or b is a
+            o = Device()
+            _check(L.ff_dbig_alloc(ctypes.byref(o.h), a.n + (a.n if sq else b.n) + 1))
+            self.outs.append(o)
+            it = self.items[i]
+            it.a, it.na, it.bits_a = a.h.d, a.n, a.bits
+            if not sq:
+                it.b, it.nb, it.bits_b = b.h.d, b.n, b.bits
+            it.out = o.h.d
+        self.h = ctypes.c_void_p()
+        _check(L.ff_batch_create(ctypes.byref(self.h), self.items, len(self.pairs), flags))
+
+    def run(self) -> None:
+        _check(lib().ff_batch_run(self.h))
+
+    def finish(self) -> None:
+        _check(lib().ff_batch_finish(self.h, self.items))
+        for i, o in enumerate(self.outs):
+            o.h.n = self.items[i].nout
+
+    def results(self) -> list:
+        return [o.value() for o in self.outs]
+
+    def close(self) -> None:
+        if self.h:
+            lib().ff_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.h and _lib is not None:
+                self.close()
+        except Exception:
+            pass

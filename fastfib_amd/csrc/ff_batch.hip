@@ -1,0 +1,415 @@
+// Batched products: many independent products of mixed sizes, as the
+// reference's callers would issue them one big_mul at a time.
+//
+// FF_BATCH_PLAIN runs them one after another, each as its own launch
+// sequence (forward passes, inner convolution, inverse passes, CRT). The
+// grouped forms build, once, a work list per pass kind: every transform
+// array (or product) of the batch with its own schedule, the flat grid cut
+// among them by a start array, largest first; one launch then covers the
+// whole batch for that pass kind (about a dozen launches per batch however
+// many products). FF_BATCH_GRAPH captures those launches in a HIP graph and
+// replays it. Schedules come from the reference rule, the autotuner table
+// or the learned dispatcher (ff_tune.hip).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/fastfib_amd.h"
+#include "ff_engine.h"
+#include "ff_internal.h"
+#include "ff_limbs.h"
+#include "ff_runtime.h"
+
+namespace {
+
+struct BLaunch {
+    int kind;  // 0 outer list, 1 inner list, 2 crt list, 3 zero fill
+    bool inv, top, r3, small;
+    const void *items;
+    const uint32_t *start, *astart;
+    int n;
+    uint32_t blocks, ablocks;
+    size_t lds;
+    double bytes, conv_total, out_total;
+    uint64_t *fill;
+};
+
+size_t al(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+struct ff_batch {
+    int flags = 0;
+    std::vector<ff_prod> items;
+    std::vector<ProdShape> shape;
+    std::vector<FFSched> sched;
+    std::vector<char> zero;
+    std::vector<void *> work;
+    unsigned long long *slots = nullptr;
+    char *desc = nullptr;
+    std::vector<BLaunch> launches;
+    std::vector<MulItem> plain;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// Bit lengths of many device numbers: thread k scans number k down from its
+// top limb.
+__global__ void k_bits(const uint64_t *const *ptr, const uint64_t *len, int n, uint64_t *bits) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint64_t *d = ptr[k];
+    uint64_t i = len[k];
+    while (i > 0 && d[i - 1] == 0) i--;
+    bits[k] = i ? 64 * (i - 1) + (64 - __clzll((long long)d[i - 1])) : 0;
+}
+
+static void read_bits(std::vector<ff_prod> &items) {
+    std::vector<const uint64_t *> ptr;
+    std::vector<uint64_t> len;
+    std::vector<std::pair<size_t, int>> who;
+    for (size_t i = 0; i < items.size(); i++) {
+        if (!items[i].bits_a) ptr.push_back(items[i].a), len.push_back(items[i].na), who.push_back({i, 0});
+        const bool sq = !items[i].b || items[i].b == items[i].a;
+        if (!sq && !items[i].bits_b) ptr.push_back(items[i].b), len.push_back(items[i].nb), who.push_back({i, 1});
+    }
+    if (ptr.empty()) return;
+    const int n = (int)ptr.size();
+    char *buf = (char *)ff_dalloc(al(n * 8) * 3);
+    const uint64_t **dptr = (const uint64_t **)buf;
+    uint64_t *dlen = (uint64_t *)(buf + al(n * 8)), *dbits = (uint64_t *)(buf + 2 * al(n * 8));
+    FF_CHECK(hipMemcpyAsync(dptr, ptr.data(), n * 8, hipMemcpyHostToDevice, g_st));
+    FF_CHECK(hipMemcpyAsync(dlen, len.data(), n * 8, hipMemcpyHostToDevice, g_st));
+    hipLaunchKernelGGL(k_bits, dim3((n + 255) / 256), dim3(256), 0, g_st, dptr, dlen, n, dbits);
+    FF_CHECK(hipGetLastError());
+    std::vector<uint64_t> bits(n);
+    FF_CHECK(hipMemcpyAsync(bits.data(), dbits, n * 8, hipMemcpyDeviceToHost, g_st));
+    FF_CHECK(hipStreamSynchronize(g_st));
+    ff_dfree(buf);
+    for (int k = 0; k < n; k++) {
+        if (who[k].second == 0) items[who[k].first].bits_a = bits[k];
+        else items[who[k].first].bits_b = bits[k];
+    }
+}
+
+// Lays out the work lists of a grouped batch and uploads them in one copy.
+static void build_lists(ff_batch *B) {
+    const size_t n = B->items.size();
+    std::vector<size_t> order;
+    for (size_t i = 0; i < n; i++)
+        if (!B->zero[i]) order.push_back(i);
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return B->shape[x].L > B->shape[y].L; });
+
+    std::map<std::tuple<uint64_t, unsigned, int, int>, FFPlan> plans;
+    std::vector<const FFPlan *> plan(n, nullptr);
+    B->work.assign(n, nullptr);
+    std::vector<uint32_t *> fa(n), fb(n);
+    std::vector<FFCrtJob> jobs(n);
+    for (size_t i : order) {
+        const ProdShape &s = B->shape[i];
+        const int il = ff_sched_inner_log(B->sched[i]), tw = ff_sched_tile_words(B->sched[i]);
+        auto key = std::make_tuple(s.L, s.bw, il, tw);
+        auto it = plans.find(key);
+        if (it == plans.end()) {
+            FFPlan pl;
+            if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, il, tw) != 0) {
+                fprintf(stderr, "fastfib_amd: no schedule for L=%llu\n", (unsigned long long)s.L);
+                exit(1);
+            }
+            it = plans.emplace(key, pl).first;
+        }
+        plan[i] = &it->second;
+        const ff_prod &p = B->items[i];
+        const bool sq = !p.b || p.b == p.a;
+        const size_t words = 3 * (size_t)s.L;
+        const uint64_t nout = p.na + (sq ? p.na : p.nb) + 1;
+        const uint64_t nblk = (nout + 255) / 256;
+        const size_t o_fb = al(words * 4), o_bgp = o_fb + (sq ? 0 : al(words * 4));
+        const size_t o_bcin = o_bgp + al(nblk * 4), o_mode = o_bcin + al(nblk);
+        char *w = (char *)ff_dalloc(o_mode + 256);
+        B->work[i] = w;
+        fa[i] = (uint32_t *)w;
+        fb[i] = sq ? nullptr : (uint32_t *)(w + o_fb);
+        jobs[i] = FFCrtJob{fa[i], p.out, (uint32_t *)(w + o_bgp), (uint8_t *)(w + o_bcin), (uint32_t *)(w + o_mode),
+                           B->slots + i, nout, s.conv, (uint32_t)s.L, (int)s.bw};
+    }
+
+    // host images of every list, then one device buffer
+    std::vector<char> img;
+    auto put = [&](const void *src, size_t bytes) {
+        const size_t off = img.size();
+        img.resize(off + al(bytes));
+        memcpy(img.data() + off, src, bytes);
+        return off;
+    };
+    struct Pending {
+        BLaunch L;
+        size_t items_off, start_off, astart_off;
+    };
+    std::vector<Pending> pend;
+
+    auto outer_list = [&](int pass, bool inv, int r3sel) {
+        std::vector<FFOuterItem> its;
+        std::vector<uint32_t> start;
+        uint32_t blocks = 0;
+        size_t lds = 0;
+        double bytes = 0;
+        for (size_t i : order) {
+            const FFPlan &pl = *plan[i];
+            if (pl.npass <= pass) continue;
+            const FFPass &ps = pl.pass[pass];
+            if (pass == 0 && (int)ps.r3 != r3sel) continue;
+            const ff_prod &p = B->items[i];
+            const bool sq = !p.b || p.b == p.a;
+            for (int op = 0; op < (sq ? 1 : 2); op++) {
+                FFOuterItem oi;
+                oi.ps = ps;
+                oi.x = FFXform{nullptr, 0, op ? fb[i] : fa[i]};
+                if (!inv && pass == 0) {
+                    oi.x.src = (const uint32_t *)(op ? p.b : p.a);
+                    oi.x.nw = (uint32_t)(2 * (op ? p.nb : p.na));
+                    bytes += 3.0 * 8.0 * (op ? p.nb : p.na);
+                }
+                bytes += (!inv && pass == 0) ? 3.0 * 4.0 * pl.L : 3.0 * 8.0 * pl.L;
+                its.push_back(oi);
+                start.push_back(blocks);
+                blocks += 3 * ps.nblocks * ps.tpb;
+                const size_t l = ff_outer_lds_bytes(&ps);
+                lds = l > lds ? l : lds;
+                if (inv) break;  // the inverse runs on the product's array only
+            }
+        }
+        if (its.empty()) return;
+        Pending pd{};
+        pd.L.kind = 0, pd.L.inv = inv, pd.L.top = pass == 0, pd.L.r3 = r3sel, pd.L.n = (int)its.size();
+        pd.L.blocks = blocks, pd.L.lds = lds, pd.L.bytes = bytes;
+        pd.items_off = put(its.data(), its.size() * sizeof(FFOuterItem));
+        pd.start_off = put(start.data(), start.size() * 4);
+        pend.push_back(pd);
+    };
+    auto inner_list = [&](int small, int r3sel) {
+        std::vector<FFInnerItem> its;
+        std::vector<uint32_t> start;
+        uint32_t blocks = 0;
+        size_t lds = 0;
+        double bytes = 0;
+        for (size_t i : order) {
+            const FFPlan &pl = *plan[i];
+            if (pl.inner.small != small || (small && pl.inner.r3 != r3sel)) continue;
+            const ff_prod &p = B->items[i];
+            const bool sq = !p.b || p.b == p.a;
+            FFInnerItem ii;
+            ii.in = pl.inner;
+            ii.c = FFConv{(const uint32_t *)p.a, sq ? nullptr : (const uint32_t *)p.b, (uint32_t)(2 * p.na),
+                          (uint32_t)(2 * (sq ? p.na : p.nb)), fa[i], fb[i]};
+            its.push_back(ii);
+            start.push_back(blocks);
+            blocks += 3 * (small ? 1 : pl.L / pl.inner.S);
+            const size_t l = ff_inner_lds_bytes(&pl.inner);
+            lds = l > lds ? l : lds;
+            bytes += small ? 3.0 * (8.0 * (p.na + (sq ? 0 : p.nb)) + 4.0 * pl.L)
+                           : 3.0 * 4.0 * pl.L * (sq ? 2 : 3);
+        }
+        if (its.empty()) return;
+        Pending pd{};
+        pd.L.kind = 1, pd.L.small = small, pd.L.r3 = r3sel, pd.L.n = (int)its.size();
+        pd.L.blocks = blocks, pd.L.lds = lds, pd.L.bytes = bytes;
+        pd.items_off = put(its.data(), its.size() * sizeof(FFInnerItem));
+        pd.start_off = put(start.data(), start.size() * 4);
+        pend.push_back(pd);
+    };
+
+    for (int pass = 0; pass < 3; pass++) {
+        if (pass == 0) {
+            outer_list(0, false, 1);
+            outer_list(0, false, 0);
+        } else {
+            outer_list(pass, false, 0);
+        }
+    }
+    inner_list(0, 0);
+    inner_list(1, 1);
+    inner_list(1, 0);
+    for (int pass = 2; pass >= 0; pass--) {
+        if (pass == 0) {
+            outer_list(0, true, 1);
+            outer_list(0, true, 0);
+        } else {
+            outer_list(pass, true, 0);
+        }
+    }
+    {
+        std::vector<FFCrtJob> js;
+        std::vector<uint32_t> start, astart;
+        uint32_t blocks = 0, ablocks = 0;
+        double conv = 0, out = 0;
+        for (size_t i : order) {
+            js.push_back(jobs[i]);
+            start.push_back(blocks);
+            astart.push_back(ablocks);
+            const uint64_t nblk = (jobs[i].nout + 255) / 256;
+            blocks += (uint32_t)nblk;
+            ablocks += (uint32_t)((nblk + 255) / 256);
+            conv += (double)jobs[i].conv;
+            out += (double)jobs[i].nout;
+        }
+        if (!js.empty()) {
+            Pending pd{};
+            pd.L.kind = 2, pd.L.n = (int)js.size(), pd.L.blocks = blocks, pd.L.ablocks = ablocks;
+            pd.L.conv_total = conv, pd.L.out_total = out;
+            pd.items_off = put(js.data(), js.size() * sizeof(FFCrtJob));
+            pd.start_off = put(start.data(), start.size() * 4);
+            pd.astart_off = put(astart.data(), astart.size() * 4);
+            pend.push_back(pd);
+        }
+    }
+    B->desc = (char *)ff_dalloc(img.size() + 256);
+    FF_CHECK(hipMemcpy(B->desc, img.data(), img.size(), hipMemcpyHostToDevice));
+    for (auto &pd : pend) {
+        BLaunch L = pd.L;
+        L.items = B->desc + pd.items_off;
+        L.start = (const uint32_t *)(B->desc + pd.start_off);
+        L.astart = L.kind == 2 ? (const uint32_t *)(B->desc + pd.astart_off) : nullptr;
+        B->launches.push_back(L);
+    }
+    for (size_t i = 0; i < n; i++) {
+        if (!B->zero[i]) continue;
+        BLaunch L{};
+        L.kind = 3;
+        L.fill = B->items[i].out;
+        B->launches.push_back(L);
+    }
+}
+
+static void enqueue_grouped(ff_batch *B) {
+    for (const BLaunch &L : B->launches) {
+        switch (L.kind) {
+        case 0:
+            FF_CHECK(ff_launch_outer_list(L.inv, L.top, L.r3, (const FFOuterItem *)L.items, L.start, L.n, L.blocks,
+                                          L.lds, L.bytes, g_st));
+            break;
+        case 1:
+            FF_CHECK(ff_launch_inner_list(L.small, L.r3, (const FFInnerItem *)L.items, L.start, L.n, L.blocks, L.lds,
+                                          L.bytes, g_st));
+            break;
+        case 2:
+            FF_CHECK(ff_launch_crt_list((const FFCrtJob *)L.items, L.start, L.astart, L.n, L.blocks, L.ablocks,
+                                        L.conv_total, L.out_total, g_st));
+            break;
+        default:
+            FF_CHECK(ff_launch_fill(L.fill, 1, 0, g_st));
+            break;
+        }
+    }
+}
+
+extern "C" int ff_batch_create(ff_batch **out, const ff_prod *items, size_t count, int flags) {
+    if (!out || (!items && count)) return FF_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    ff_ensure_init();
+    ff_batch *B = new ff_batch;
+    B->flags = flags;
+    B->items.assign(items, items + count);
+    for (const ff_prod &p : B->items) {
+        if (!p.a || !p.out || p.na == 0 || (p.b && p.b != p.a && p.nb == 0)) {
+            delete B;
+            ff_set_err("batch item without operands or output");
+            return FF_ERR_ARG;
+        }
+    }
+    read_bits(B->items);
+    B->shape.resize(count);
+    B->sched.resize(count);
+    B->zero.assign(count, 0);
+    B->slots = (unsigned long long *)ff_dalloc((count ? count : 1) * 8);
+    for (size_t i = 0; i < count; i++) {
+        ff_prod &p = B->items[i];
+        const bool sq = !p.b || p.b == p.a;
+        const uint64_t bb = sq ? p.bits_a : p.bits_b;
+        if (p.bits_a == 0 || bb == 0) {
+            B->zero[i] = 1;
+            continue;
+        }
+        const ProdShape s0 = ff_product_shape(p.bits_a, bb);
+        if (s0.split) {
+            delete B;
+            ff_set_err("batch item %zu needs the split multiply", i);
+            return FF_ERR_UNSUPPORTED;
+        }
+        B->sched[i] = ff_pick_sched(s0, p.bits_a, bb, sq, count, flags);
+        B->shape[i] = ff_shape_with(s0, B->sched[i]);
+    }
+    if (!(flags & (FF_BATCH_GROUPED | FF_BATCH_GRAPH))) {
+        for (size_t i = 0; i < count; i++) {
+            if (B->zero[i]) continue;
+            const ff_prod &p = B->items[i];
+            const bool sq = !p.b || p.b == p.a;
+            B->plain.push_back(MulItem{p.a, p.na, p.bits_a, sq ? nullptr : p.b, sq ? p.na : p.nb,
+                                       sq ? p.bits_a : p.bits_b, p.out, B->shape[i].conv, B->slots + i});
+        }
+    } else {
+        build_lists(B);
+        if (flags & FF_BATCH_GRAPH) {
+            FF_CHECK(hipStreamSynchronize(g_st));
+            ff_prof_pause(1);
+            FF_CHECK(hipStreamBeginCapture(g_st, hipStreamCaptureModeRelaxed));
+            enqueue_grouped(B);
+            FF_CHECK(hipStreamEndCapture(g_st, &B->graph));
+            ff_prof_pause(0);
+            FF_CHECK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        }
+    }
+    *out = B;
+    return FF_OK;
+}
+
+extern "C" int ff_batch_run(ff_batch *B) {
+    if (!B) return FF_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (B->exec) {
+        FF_CHECK(hipGraphLaunch(B->exec, g_st));
+    } else if (!B->launches.empty() || (B->flags & FF_BATCH_GROUPED)) {
+        enqueue_grouped(B);
+    } else {
+        size_t k = 0;
+        for (size_t i = 0; i < B->items.size(); i++) {
+            if (B->zero[i]) {
+                FF_CHECK(ff_launch_fill(B->items[i].out, 1, 0, g_st));
+                continue;
+            }
+            ff_mul_group(&B->plain[k++], 1, B->shape[i], B->sched[i]);
+        }
+    }
+    return FF_OK;
+}
+
+extern "C" int ff_batch_finish(ff_batch *B, ff_prod *items) {
+    if (!B) return FF_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    const size_t n = B->items.size();
+    std::vector<uint64_t> keys(n ? n : 1);
+    if (n) FF_CHECK(hipMemcpyAsync(keys.data(), B->slots, n * 8, hipMemcpyDeviceToHost, g_st));
+    FF_CHECK(hipStreamSynchronize(g_st));
+    if (items) {
+        for (size_t i = 0; i < n; i++) items[i].nout = B->zero[i] || keys[i] == 0 ? 1 : (size_t)(keys[i] >> 8);
+    }
+    return FF_OK;
+}
+
+extern "C" void ff_batch_destroy(ff_batch *B) {
+    if (!B) return;
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    (void)hipStreamSynchronize(g_st);
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    for (void *w : B->work) ff_dfree(w);
+    ff_dfree(B->desc);
+    ff_dfree(B->slots);
+    delete B;
+}

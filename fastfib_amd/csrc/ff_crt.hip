@@ -110,13 +110,12 @@ __device__ __forceinline__ u128 limb_sum(uint64_t i, int bw, uint64_t j0, const 
     return T;
 }
 
-// One block per CRT_LIMBS output limbs.
-__global__ __launch_bounds__(CRT_LIMBS) void k_crt_pack(FFCrtSet js, FFGarner G) {
-    const FFCrtJob &J = js.j[blockIdx.y];
+// One block per CRT_LIMBS output limbs (block bx of job J).
+__device__ __forceinline__ void crt_pack_body(const FFCrtJob &J, uint32_t bx, const FFGarner &G) {
     const uint64_t nblk = (J.nout + CRT_LIMBS - 1) / CRT_LIMBS;
-    if (blockIdx.x >= nblk) return;
+    if (bx >= nblk) return;
     const int bw = J.bw;
-    const uint64_t i0 = (uint64_t)blockIdx.x * CRT_LIMBS;
+    const uint64_t i0 = (uint64_t)bx * CRT_LIMBS;
     const uint64_t i1 = i0 + CRT_LIMBS < J.nout ? i0 + CRT_LIMBS : J.nout;
     // coefficients touching limbs [i0 - 1, i1): limb i0 - 1 gives the block's first hi
     const uint64_t lim = J.conv < J.L ? J.conv : J.L;
@@ -173,15 +172,34 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_crt_pack(FFCrtSet js, FFGarner G)
             Gb = (e & 1) | ((e >> 1) & Gb);
             Pb = Pb & (e >> 1);
         }
-        J.bgp[blockIdx.x] = (uint32_t)(Gb | (Pb << 1));
+        J.bgp[bx] = (uint32_t)(Gb | (Pb << 1));
     }
+}
+
+__global__ __launch_bounds__(CRT_LIMBS) void k_crt_pack(FFCrtSet js, FFGarner G) {
+    crt_pack_body(js.j[blockIdx.y], blockIdx.x, G);
+}
+
+__device__ __forceinline__ int crt_find(const uint32_t *start, int n, uint32_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(CRT_LIMBS) void k_crt_pack_list(const FFCrtJob *jobs, const uint32_t *start, int n,
+                                                             FFGarner G) {
+    const int k = crt_find(start, n, blockIdx.x);
+    crt_pack_body(jobs[k], blockIdx.x - start[k], G);
 }
 
 // Exact carry into every block, only when some block propagates.
 #define SCAN_THREADS 1024
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_crt_scan(FFCrtSet js) {
-    const FFCrtJob &J = js.j[blockIdx.x];
+__device__ __forceinline__ void crt_scan_body(const FFCrtJob &J) {
     const uint64_t nblk = (J.nout + CRT_LIMBS - 1) / CRT_LIMBS;
     int mine = 0;
     for (uint64_t b = threadIdx.x; b < nblk; b += SCAN_THREADS) mine |= (J.bgp[b] >> 1) & 1;
@@ -220,6 +238,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_crt_scan(FFCrtSet js) {
     }
 }
 
+__global__ __launch_bounds__(SCAN_THREADS) void k_crt_scan(FFCrtSet js) { crt_scan_body(js.j[blockIdx.x]); }
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_crt_scan_list(const FFCrtJob *jobs) {
+    crt_scan_body(jobs[blockIdx.x]);
+}
+
 // Top nonzero limb of d[i0..i1) booked into the product's length key.
 __device__ __forceinline__ void book_top(const FFCrtJob &J, uint64_t i0, uint64_t i1) {
     for (uint64_t i = i1; i-- > i0;) {
@@ -232,13 +256,11 @@ __device__ __forceinline__ void book_top(const FFCrtJob &J, uint64_t i0, uint64_
     }
 }
 
-// One thread per block of limbs: adds the block's carry-in (ripple bounded by
-// the block) and, for blocks holding limbs from nout - 3 up (where the top
-// nonzero limb of an na x nb product lies), books the top limb.
-__global__ __launch_bounds__(256) void k_crt_apply(FFCrtSet js) {
-    const FFCrtJob &J = js.j[blockIdx.y];
+// Thread b of a job: adds block b's carry-in (ripple bounded by the block)
+// and, for blocks holding limbs from nout - 3 up (where the top nonzero limb
+// of an na x nb product lies), books the top limb.
+__device__ __forceinline__ void crt_apply_body(const FFCrtJob &J, uint64_t b) {
     const uint64_t nblk = (J.nout + CRT_LIMBS - 1) / CRT_LIMBS;
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nblk) return;
     const uint32_t c = *J.mode ? J.bcin[b] : (b ? J.bgp[b - 1] & 1 : 0);
     const uint64_t i0 = b * CRT_LIMBS, i1 = i0 + CRT_LIMBS < J.nout ? i0 + CRT_LIMBS : J.nout;
@@ -249,6 +271,15 @@ __global__ __launch_bounds__(256) void k_crt_apply(FFCrtSet js) {
     }
     const uint64_t lo_top = J.nout >= 3 ? J.nout - 3 : 0;
     if (i1 > lo_top) book_top(J, i0 > lo_top ? i0 : lo_top, i1);
+}
+
+__global__ __launch_bounds__(256) void k_crt_apply(FFCrtSet js) {
+    crt_apply_body(js.j[blockIdx.y], (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_crt_apply_list(const FFCrtJob *jobs, const uint32_t *astart, int n) {
+    const int k = crt_find(astart, n, blockIdx.x);
+    crt_apply_body(jobs[k], (uint64_t)(blockIdx.x - astart[k]) * blockDim.x + threadIdx.x);
 }
 
 // Algorithmic bytes: pack reads 3 residues per coefficient and writes each
@@ -267,6 +298,23 @@ extern "C" hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_
     dim3 g3((unsigned)((max_blk + 255) / 256), njobs);
     ff_prof_begin(FK_CRT_APPLY, st);
     hipLaunchKernelGGL(k_crt_apply, g3, dim3(256), 0, st, *js);
+    ff_prof_end(FK_CRT_APPLY, 4.0 * out_limbs_total / CRT_LIMBS, st);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *start, const uint32_t *astart, int n,
+                                         uint32_t blocks, uint32_t ablocks, double conv_total,
+                                         double out_limbs_total, hipStream_t st) {
+    if (!n) return hipSuccess;
+    garner_init();
+    ff_prof_begin(FK_CRT_PACK, st);
+    hipLaunchKernelGGL(k_crt_pack_list, dim3(blocks), dim3(CRT_LIMBS), 0, st, jobs, start, n, g_garner);
+    ff_prof_end(FK_CRT_PACK, 12.0 * conv_total + 8.0 * out_limbs_total, st);
+    ff_prof_begin(FK_CRT_SCAN, st);
+    hipLaunchKernelGGL(k_crt_scan_list, dim3(n), dim3(SCAN_THREADS), 0, st, jobs);
+    ff_prof_end(FK_CRT_SCAN, 0.0, st);
+    ff_prof_begin(FK_CRT_APPLY, st);
+    hipLaunchKernelGGL(k_crt_apply_list, dim3(ablocks), dim3(256), 0, st, jobs, astart, n);
     ff_prof_end(FK_CRT_APPLY, 4.0 * out_limbs_total / CRT_LIMBS, st);
     return hipGetLastError();
 }

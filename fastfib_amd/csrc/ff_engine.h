@@ -1,0 +1,51 @@
+// Product schedules shared by the single-product engine (ff_engine.hip) and
+// the batch engine (ff_batch.hip).
+#pragma once
+#include <stdint.h>
+
+// A product's transform: width bw, length L = 2^lg or 3 * 2^lg (r3), conv
+// coefficients; split = 1 when no single transform fits (L > 2^25).
+struct ProdShape {
+    unsigned bw;
+    uint64_t L;
+    int lg, r3;
+    uint64_t conv;
+    int split;
+};
+
+// A schedule choice beside the reference rule: the transform length (0: the
+// shortest, the reference's choice; 1: the next supported length up), the
+// inner block (log2, 0: default) and the outer tile (LDS words, 0: default).
+struct FFSched {
+    int len_up;
+    int inner_log;
+    int tile_words;
+};
+
+ProdShape ff_product_shape(uint64_t bits_a, uint64_t bits_b);
+// The shape with the schedule's length choice applied.
+ProdShape ff_shape_with(const ProdShape &s, const FFSched &sc);
+
+// One product of a launch group (device pointers).
+struct MulItem {
+    const uint64_t *a;
+    uint64_t na, bits_a;
+    const uint64_t *b;  // nullptr: square
+    uint64_t nb, bits_b;
+    uint64_t *out;      // room for na + nb + 1 limbs
+    uint64_t conv;      // product coefficients at the group's width
+    unsigned long long *lenkey;  // where the CRT books the normalized length
+};
+
+// Runs up to FF_MAXB products sharing one shape and schedule, one launch
+// per pass (transform arrays come from the memory pool, stream-ordered).
+void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched &sc);
+
+// Effective schedule knobs (0 in sc: the global options).
+int ff_sched_inner_log(const FFSched &sc);
+int ff_sched_tile_words(const FFSched &sc);
+
+// The schedule a batch gives one product: the reference rule, the
+// autotuner table (FF_BATCH_TUNED) or the learned dispatcher
+// (FF_BATCH_LEARNED); count is the batch's product count.
+FFSched ff_pick_sched(const ProdShape &s0, uint64_t bits_a, uint64_t bits_b, bool square, size_t count, int flags);

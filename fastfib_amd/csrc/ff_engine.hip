@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/fastfib_amd.h"
+#include "ff_engine.h"
 #include "ff_internal.h"
 #include "ff_limbs.h"
 #include "ff_runtime.h"
@@ -39,13 +40,31 @@ static uint64_t ntt_length(uint64_t conv, int *lg_out, int *r3_out) {
     return L;
 }
 
-struct ProdShape {
-    unsigned bw;
-    uint64_t L;
-    int lg, r3;
-    uint64_t conv;
-    int split;  // the product needs pieces (L > 2^25)
-};
+// The next supported length above L (2^k -> 3 * 2^(k-1), 3 * 2^k -> 2^(k+2)).
+static uint64_t next_length(uint64_t L, int *lg, int *r3) {
+    if (*r3) {
+        *r3 = 0;
+        *lg += 2;
+        return L / 3 * 4;
+    }
+    *r3 = 1;
+    *lg -= 1;
+    return L / 2 * 3;
+}
+
+ProdShape ff_shape_with(const ProdShape &s0, const FFSched &sc) {
+    ProdShape s = s0;
+    if (!s.split && sc.len_up) {
+        int lg = s.lg, r3 = s.r3;
+        const uint64_t L = next_length(s.L, &lg, &r3);
+        if (L <= (1u << 25)) s.L = L, s.lg = lg, s.r3 = r3;
+    }
+    return s;
+}
+
+static ProdShape product_shape(uint64_t bits_a, uint64_t bits_b);
+
+ProdShape ff_product_shape(uint64_t bits_a, uint64_t bits_b) { return product_shape(bits_a, bits_b); }
 
 static ProdShape product_shape(uint64_t bits_a, uint64_t bits_b) {
     ProdShape s{};
@@ -153,22 +172,16 @@ static void dnum_free(DNum &x) {
 
 // ------------------------------------------------------------------ products
 
-struct MulItem {
-    const uint64_t *a;
-    uint64_t na, bits_a;
-    const uint64_t *b;  // nullptr: square
-    uint64_t nb, bits_b;
-    uint64_t *out;      // room for out_room() limbs
-    uint64_t conv;      // product coefficients at the group's width
-    int slot;
-};
-
 static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
+int ff_sched_inner_log(const FFSched &sc) { return sc.inner_log ? sc.inner_log : g_inner_log; }
+int ff_sched_tile_words(const FFSched &sc) { return sc.tile_words ? sc.tile_words : g_tile_words; }
+
 // Runs up to FF_MAXB products that share one schedule, one launch per pass.
-static void mul_group(const MulItem *it, int cnt, const ProdShape &s) {
+void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched &sc) {
     FFPlan pl;
-    if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, g_inner_log, g_tile_words) != 0) {
+    if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, ff_sched_inner_log(sc),
+                      ff_sched_tile_words(sc)) != 0) {
         fprintf(stderr, "fastfib_amd: no schedule for L=%llu\n", (unsigned long long)s.L);
         exit(1);
     }
@@ -207,7 +220,7 @@ static void mul_group(const MulItem *it, int cnt, const ProdShape &s) {
                             (uint32_t *)(base + o_bgp),
                             (uint8_t *)(base + o_bcin),
                             (uint32_t *)(base + o_mode),
-                            g_slots + it[i].slot,
+                            it[i].lenkey,
                             nout,
                             it[i].conv,
                             (uint32_t)s.L,
@@ -257,7 +270,7 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
         const int slot = out[i].slot;
         out[i] = dnum_alloc(out_room(shapes[i], a->n, nb));
         out[i].slot = slot;
-        items[i] = MulItem{a->d, a->n, a->bits, sq ? nullptr : b->d, nb, bb, out[i].d, shapes[i].conv, slot};
+        items[i] = MulItem{a->d, a->n, a->bits, sq ? nullptr : b->d, nb, bb, out[i].d, shapes[i].conv, g_slots + slot};
     }
     // one group per (length, width), up to FF_MAXB products per launch
     std::vector<int> done(cnt, 0);
@@ -271,7 +284,7 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
                 done[j] = 1;
             }
         }
-        mul_group(grp.data(), (int)grp.size(), shapes[i]);
+        ff_mul_group(grp.data(), (int)grp.size(), shapes[i], FFSched{0, 0, 0});
     }
     if (pred) {
         for (int i = 0; i < cnt; i++) {
@@ -666,6 +679,17 @@ extern "C" int ff_dbig_upload(ff_dbig *x, const uint64_t *limbs, size_t n) {
     if (!x || (!limbs && n)) return FF_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     ff_ensure_init();
+    const size_t m = limbs ? norm_len(limbs, n) : 1;
+    if (x->d && x->cap >= m) {
+        // in place: device pointers held elsewhere (a batch) stay valid
+        const uint64_t zero = 0;
+        FF_CHECK(hipStreamSynchronize(g_st));
+        FF_CHECK(hipMemcpy(x->d, limbs ? limbs : &zero, m * 8, hipMemcpyHostToDevice));
+        const uint64_t top = limbs ? limbs[m - 1] : 0;
+        x->n = m;
+        x->bits = top ? 64 * (m - 1) + (64 - __builtin_clzll(top)) : 0;
+        return FF_OK;
+    }
     ff_dbig_release(x);
     DNum d = upload(limbs, n);
     to_dbig(d, x);

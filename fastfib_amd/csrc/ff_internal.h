@@ -97,6 +97,18 @@ struct FFCrtSet {
     FFCrtJob j[FF_MAXB];
 };
 
+// Work-list items: one per transform array (outer passes) or product (inner
+// kernels, CRT), each with its own schedule; a flat grid is cut among them
+// by a start array (item k owns blocks [start[k], start[k+1])).
+struct FFOuterItem {
+    FFPass ps;
+    FFXform x;
+};
+struct FFInnerItem {
+    FFInner in;
+    FFConv c;
+};
+
 // Kernel kinds for the live per-launch timer (ff_prof_*).
 enum FFKernelKind {
     FK_FWD_TOP, FK_FWD, FK_INNER, FK_SMALL, FK_INV, FK_INV_TOP,
@@ -119,9 +131,22 @@ hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int nc, int nge
                           hipStream_t st);
 hipError_t ff_launch_inverse(const FFPlan *pl, const FFXformSet *xs, int nx, hipStream_t st);
 
+// Work-list launches (one launch for a heterogeneous list of items).
+hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, const FFOuterItem *items, const uint32_t *start, int n,
+                                uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
+hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerItem *items, const uint32_t *start, int n,
+                                uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
+size_t ff_outer_lds_bytes(const FFPass *ps);
+size_t ff_inner_lds_bytes(const FFInner *in);
+
 // ff_crt.hip
 hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_nout, double conv_total,
                          double out_limbs_total, hipStream_t st);
+// start[k]: first pack block of job k (blocks of 256 limbs); astart[k]:
+// first apply block (256 limb-blocks per apply block).
+hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *start, const uint32_t *astart, int n,
+                              uint32_t blocks, uint32_t ablocks, double conv_total, double out_limbs_total,
+                              hipStream_t st);
 }
 
 #define FF_CHECK(expr)                                                                      \

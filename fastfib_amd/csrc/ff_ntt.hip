@@ -119,16 +119,14 @@ extern "C" int ff_tables_init(void) {
 
 #define FF_THREADS 256
 
-// Outer pass over a tile of R rows x C columns of one block.
+// Outer pass over one tile of R rows x C columns (tile index `tile` among
+// the pass's nblocks * tpb) of array X, prime pi.
 template <bool INV, bool TOP, bool R3>
-__global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, const FFTables *T,
-                                                      FFPrimes PR) {
-    extern __shared__ uint32_t lds[];
-    const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
-    const FFXform X = xs.x[item];
+__device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, int pi, uint32_t tile,
+                                           const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
     const FFPrime P = PR.p[pi];
     const uint32_t p = P.p;
-    const uint32_t blk = blockIdx.x / ps.tpb, ct = blockIdx.x - blk * ps.tpb;
+    const uint32_t blk = tile / ps.tpb, ct = tile - blk * ps.tpb;
     const uint32_t c0 = ct << ps.logC;
     uint32_t *data = X.data + (size_t)pi * ps.L + (size_t)blk * ps.Lb;
     FFTile t{lds, ps.ld, ps.C, ps.logC, ps.rho, R3 ? 3 : 1, 0};
@@ -189,16 +187,46 @@ __global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, 
     }
 }
 
+template <bool INV, bool TOP, bool R3>
+__global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, const FFTables *T,
+                                                      FFPrimes PR) {
+    extern __shared__ uint32_t lds[];
+    const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
+    outer_body<INV, TOP, R3>(ps, xs.x[item], pi, blockIdx.x, T, PR, lds);
+}
+
+// Work-list form: item k owns blocks [start[k], start[k+1]) of the flat
+// grid, 3 * nblocks * tpb of them (the prime is the major index).
+__device__ __forceinline__ int ff_find_item(const uint32_t *start, int n, uint32_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool INV, bool TOP, bool R3>
+__global__ __launch_bounds__(FF_THREADS) void k_outer_list(const FFOuterItem *items, const uint32_t *start, int n,
+                                                           const FFTables *T, FFPrimes PR) {
+    extern __shared__ uint32_t lds[];
+    const int k = ff_find_item(start, n, blockIdx.x);
+    const FFOuterItem &it = items[k];
+    const uint32_t local = blockIdx.x - start[k];
+    const uint32_t ntile = it.ps.nblocks * it.ps.tpb;
+    const int pi = (int)(local / ntile);
+    outer_body<INV, TOP, R3>(it.ps, it.x, pi, local - (uint32_t)pi * ntile, T, PR, lds);
+}
+
 // Inner blocks: forward transform of a (and b), pointwise product, inverse,
 // all in LDS. small = 1: the whole transform, read from the limbs and
 // scaled for the CRT at the end.
 template <bool SMALL, bool R3>
-__global__ __launch_bounds__(FF_THREADS, 4) void k_inner(FFInner in, FFConvSet cs, const FFTables *T, FFPrimes PR) {
-    extern __shared__ uint32_t lds[];
-    const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
+__device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, int pi, uint32_t blkx,
+                                           const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
     const FFPrime P = PR.p[pi];
     const uint32_t p = P.p;
-    const FFConv C = cs.c[item];
     const bool square = SMALL ? (C.bsrc == nullptr) : (C.fb == nullptr);
     const uint32_t n = SMALL ? in.L : in.S;
     const int words = (int)(n + (n >> 4) + 16);
@@ -206,7 +234,7 @@ __global__ __launch_bounds__(FF_THREADS, 4) void k_inner(FFInner in, FFConvSet c
     FFTile ta{A, 1, 1, 0, in.s, R3 ? 3 : 1, 1};
     FFTile tb = ta;
     tb.s = B;
-    const size_t off = (size_t)pi * in.L + (SMALL ? 0 : (size_t)blockIdx.x * in.S);
+    const size_t off = (size_t)pi * in.L + (SMALL ? 0 : (size_t)blkx * in.S);
 
     const uint32_t pinv = P.pinv;
     const bool small = SMALL;
@@ -270,6 +298,26 @@ __global__ __launch_bounds__(FF_THREADS, 4) void k_inner(FFInner in, FFConvSet c
                lds_a, lds_a_st);
     ff_radix3_layer<true>(ta, T, pi, P, in.rstep3);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) store_a((int)i, 0, lds_a((int)i, 0));
+}
+
+template <bool SMALL, bool R3>
+__global__ __launch_bounds__(FF_THREADS, 4) void k_inner(FFInner in, FFConvSet cs, const FFTables *T, FFPrimes PR) {
+    extern __shared__ uint32_t lds[];
+    const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
+    inner_body<SMALL, R3>(in, cs.c[item], pi, blockIdx.x, T, PR, lds);
+}
+
+// Work-list form: item k owns blocks [start[k], start[k+1]), 3 * nblk of them.
+template <bool SMALL, bool R3>
+__global__ __launch_bounds__(FF_THREADS, 4) void k_inner_list(const FFInnerItem *items, const uint32_t *start, int n,
+                                                              const FFTables *T, FFPrimes PR) {
+    extern __shared__ uint32_t lds[];
+    const int k = ff_find_item(start, n, blockIdx.x);
+    const FFInnerItem &it = items[k];
+    const uint32_t local = blockIdx.x - start[k];
+    const uint32_t nblk = SMALL ? 1 : it.in.L / it.in.S;
+    const int pi = (int)(local / nblk);
+    inner_body<SMALL, R3>(it.in, it.c, pi, local - (uint32_t)pi * nblk, T, PR, lds);
 }
 
 // ------------------------------------------------------------------ plans
@@ -365,6 +413,44 @@ extern "C" int ff_plan_build(FFPlan *pl, uint32_t L, int bw, int lg, int r3, uin
 }
 
 static size_t outer_lds(const FFPass &ps) { return (size_t)ps.R * ps.ld * sizeof(uint32_t); }
+
+static size_t inner_lds(const FFInner &in) {
+    const uint32_t n = in.small ? in.L : in.S;
+    return 2 * (n + (n >> 4) + 16) * sizeof(uint32_t);
+}
+
+extern "C" size_t ff_outer_lds_bytes(const FFPass *ps) { return outer_lds(*ps); }
+extern "C" size_t ff_inner_lds_bytes(const FFInner *in) { return inner_lds(*in); }
+
+extern "C" hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, const FFOuterItem *items,
+                                           const uint32_t *start, int n, uint32_t blocks, size_t lds, double bytes,
+                                           hipStream_t st) {
+    if (!blocks) return hipSuccess;
+    const int kind = inv ? (top ? FK_INV_TOP : FK_INV) : (top ? FK_FWD_TOP : FK_FWD);
+    const dim3 g(blocks), b(FF_THREADS);
+    ff_prof_begin(kind, st);
+    if (!inv && top && r3) hipLaunchKernelGGL((k_outer_list<false, true, true>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    else if (!inv && top) hipLaunchKernelGGL((k_outer_list<false, true, false>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    else if (!inv) hipLaunchKernelGGL((k_outer_list<false, false, false>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    else if (top && r3) hipLaunchKernelGGL((k_outer_list<true, true, true>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    else if (top) hipLaunchKernelGGL((k_outer_list<true, true, false>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    else hipLaunchKernelGGL((k_outer_list<true, false, false>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    ff_prof_end(kind, bytes, st);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerItem *items, const uint32_t *start,
+                                           int n, uint32_t blocks, size_t lds, double bytes, hipStream_t st) {
+    if (!blocks) return hipSuccess;
+    const int kind = small ? FK_SMALL : FK_INNER;
+    const dim3 g(blocks), b(FF_THREADS);
+    ff_prof_begin(kind, st);
+    if (!small) hipLaunchKernelGGL((k_inner_list<false, false>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    else if (r3) hipLaunchKernelGGL((k_inner_list<true, true>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    else hipLaunchKernelGGL((k_inner_list<true, false>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    ff_prof_end(kind, bytes, st);
+    return hipGetLastError();
+}
 
 // Algorithmic bytes (per prime, per array): a top forward pass reads the
 // operand's limbs and writes 4L; every other outer pass reads and writes 4L.

@@ -21,5 +21,9 @@ void ff_ensure_init(void);
 void *ff_dalloc(size_t bytes);
 void ff_dfree(void *p);
 
+// Kernel timing is suspended while a graph is being captured (event
+// records and their readback do not belong in a replayed graph).
+void ff_prof_pause(int pause);
+
 // Pinned host buffer for reading small results back.
 uint64_t *ff_pinned_results(size_t words);

@@ -116,6 +116,9 @@ static std::vector<hipEvent_t> g_prof_spare;
 static double g_prof_ms[FK_COUNT], g_prof_bytes[FK_COUNT];
 static long g_prof_n[FK_COUNT];
 static hipEvent_t g_prof_open = nullptr;
+static int g_prof_paused = 0;
+
+void ff_prof_pause(int pause) { g_prof_paused = pause; }
 
 static hipEvent_t prof_event(void) {
     if (!g_prof_spare.empty()) {
@@ -130,13 +133,13 @@ static hipEvent_t prof_event(void) {
 
 extern "C" void ff_prof_begin(int kind, hipStream_t st) {
     (void)kind;
-    if (!g_prof_on) return;
+    if (!g_prof_on || g_prof_paused) return;
     g_prof_open = prof_event();
     FF_CHECK(hipEventRecord(g_prof_open, st));
 }
 
 extern "C" void ff_prof_end(int kind, double bytes, hipStream_t st) {
-    if (!g_prof_on || !g_prof_open) return;
+    if (!g_prof_on || g_prof_paused || !g_prof_open) return;
     hipEvent_t e = prof_event();
     FF_CHECK(hipEventRecord(e, st));
     g_prof_pending.push_back(ProfRec{kind, bytes, g_prof_open, e});

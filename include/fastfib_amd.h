@@ -108,7 +108,8 @@ FF_API const char *ff_last_error(void);
 
 FF_API int ff_dbig_alloc(ff_dbig *x, size_t cap);
 FF_API void ff_dbig_release(ff_dbig *x);
-/* Copies n host limbs in (normalizes n and computes bits). */
+/* Copies n host limbs in (normalizes n and computes bits), into x's own
+   memory when x is allocated with room for them, else into a new block. */
 FF_API int ff_dbig_upload(ff_dbig *x, const uint64_t *limbs, size_t n);
 /* Copies x's n limbs out to host memory with room for cap limbs. */
 FF_API int ff_dbig_download(const ff_dbig *x, uint64_t *limbs, size_t cap);
@@ -119,6 +120,44 @@ FF_API int ff_dmul(ff_dbig *out, const ff_dbig *a, const ff_dbig *b);
 
 /* out = F(n), computed and left in device memory. */
 FF_API int ff_dfibonacci(uint64_t n, ff_dbig *out);
+
+/* ------------------------------------------------------------------------
+ * Batched products (no counterpart in the reference, whose callers loop
+ * over big_mul). A batch is built once for a fixed list of device operands
+ * and can be run many times.
+ * --------------------------------------------------------------------- */
+
+/* One product of a batch. Pointers are device memory; a and b hold na and
+   nb normalized limbs; out has room for na + nb + 1 limbs. b == NULL or
+   b == a squares. bits_a / bits_b may be 0 (the library reads them). nout
+   receives the product's normalized length (ff_batch_finish). Each run
+   re-reads the operands: their values may change between runs as long as
+   na, nb stay and the bit lengths do not exceed bits_a, bits_b. */
+typedef struct {
+    const uint64_t *a;
+    size_t na;
+    const uint64_t *b;
+    size_t nb;
+    uint64_t *out;
+    size_t nout;
+    uint64_t bits_a, bits_b;
+} ff_prod;
+
+/* How a batch runs (flags for ff_batch_create). */
+#define FF_BATCH_PLAIN 0   /* one product at a time, one launch per pass each */
+#define FF_BATCH_GROUPED 1 /* one launch per pass kind for the whole batch (work lists) */
+#define FF_BATCH_GRAPH 2   /* grouped, captured once in a HIP graph and replayed */
+#define FF_BATCH_TUNED 4   /* schedules from the autotuner table (ff_tune_*) */
+#define FF_BATCH_LEARNED 8 /* schedules from the learned dispatcher (ff_mlp_*) */
+
+typedef struct ff_batch ff_batch;
+
+FF_API int ff_batch_create(ff_batch **out, const ff_prod *items, size_t count, int flags);
+/* Queues one run of the whole batch on the library stream. */
+FF_API int ff_batch_run(ff_batch *b);
+/* Waits for the last run and writes every item's nout. */
+FF_API int ff_batch_finish(ff_batch *b, ff_prod *items);
+FF_API void ff_batch_destroy(ff_batch *b);
 
 /* Schedule knobs (the reference's compile-time tunables, Appendix of its
    README: -DNAME=value, here set at run time). Names:

@@ -139,6 +139,42 @@ def big_mul_ntt(a, b=None) -> np.ndarray:
     return _take(lib().ff_big_mul_ntt(ctypes.byref(A), ctypes.byref(B)))
 
 
+_libc = ctypes.CDLL(None)
+_libc.free.argtypes = [ctypes.c_void_p]
+
+
+def _dec_sigs(L) -> None:
+    if getattr(L, "_dec_sigs", False):
+        return
+    L.ff_big_to_string.argtypes = [ctypes.POINTER(Big)]
+    L.ff_big_to_string.restype = ctypes.c_void_p
+    L.ff_dbig_to_string.argtypes = [ctypes.POINTER(DBig), ctypes.POINTER(ctypes.c_void_p),
+                                    ctypes.POINTER(ctypes.c_size_t)]
+    L.ff_big_recip.argtypes = [ctypes.POINTER(Big), ctypes.c_size_t]
+    L.ff_big_recip.restype = Big
+    L._dec_sigs = True
+
+
+def to_string(a) -> str:
+    """ff_big_to_string: decimal digits of a (int or limbs)."""
+    L = lib()
+    _dec_sigs(L)
+    A, ka = _as_big(a)
+    p = L.ff_big_to_string(ctypes.byref(A))
+    try:
+        return ctypes.string_at(p).decode()
+    finally:
+        _libc.free(p)
+
+
+def big_recip(P, s: int) -> np.ndarray:
+    """ff_big_recip: floor(2^(2s) / P) minus at most 4, P of exactly s bits."""
+    L = lib()
+    _dec_sigs(L)
+    A, ka = _as_big(P)
+    return _take(L.ff_big_recip(ctypes.byref(A), s))
+
+
 def fibonacci(n: int) -> np.ndarray:
     return _take(lib().ff_fibonacci(n))
 
@@ -207,6 +243,17 @@ class Device:
 
     def value(self) -> int:
         return from_limbs(self.limbs())
+
+    def to_string(self) -> str:
+        """Decimal digits (ff_dbig_to_string)."""
+        L = lib()
+        _dec_sigs(L)
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(L.ff_dbig_to_string(ctypes.byref(self.h), ctypes.byref(p), ctypes.byref(n)))
+        try:
+            return ctypes.string_at(p, n.value).decode()
+        finally:
+            _libc.free(p)
 
     def release(self) -> None:
         lib().ff_dbig_release(ctypes.byref(self.h))

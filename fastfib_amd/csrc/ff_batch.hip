@@ -310,6 +310,10 @@ static void enqueue_grouped(ff_batch *B) {
 }
 
 extern "C" int ff_batch_create(ff_batch **out, const ff_prod *items, size_t count, int flags) {
+    return ff_batch_create_sched(out, items, count, flags, nullptr);
+}
+
+int ff_batch_create_sched(ff_batch **out, const ff_prod *items, size_t count, int flags, const FFSched *force) {
     if (!out || (!items && count)) return FF_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     ff_ensure_init();
@@ -327,24 +331,37 @@ extern "C" int ff_batch_create(ff_batch **out, const ff_prod *items, size_t coun
     B->shape.resize(count);
     B->sched.resize(count);
     B->zero.assign(count, 0);
-    B->slots = (unsigned long long *)ff_dalloc((count ? count : 1) * 8);
+    std::vector<ProdShape> s0;
+    std::vector<uint64_t> ba, bb;
+    std::vector<uint8_t> sq;
+    std::vector<size_t> who;
     for (size_t i = 0; i < count; i++) {
         ff_prod &p = B->items[i];
-        const bool sq = !p.b || p.b == p.a;
-        const uint64_t bb = sq ? p.bits_a : p.bits_b;
-        if (p.bits_a == 0 || bb == 0) {
+        const bool square = !p.b || p.b == p.a;
+        const uint64_t bits_b = square ? p.bits_a : p.bits_b;
+        if (p.bits_a == 0 || bits_b == 0) {
             B->zero[i] = 1;
             continue;
         }
-        const ProdShape s0 = ff_product_shape(p.bits_a, bb);
-        if (s0.split) {
+        const ProdShape s = ff_product_shape(p.bits_a, bits_b);
+        if (s.split) {
             delete B;
             ff_set_err("batch item %zu needs the split multiply", i);
             return FF_ERR_UNSUPPORTED;
         }
-        B->sched[i] = ff_pick_sched(s0, p.bits_a, bb, sq, count, flags);
-        B->shape[i] = ff_shape_with(s0, B->sched[i]);
+        s0.push_back(s), ba.push_back(p.bits_a), bb.push_back(bits_b), sq.push_back(square), who.push_back(i);
     }
+    std::vector<FFSched> pick(who.size());
+    if (force) {
+        for (size_t j = 0; j < who.size(); j++) pick[j] = force[who[j]];
+    } else {
+        ff_pick_scheds(s0.data(), ba.data(), bb.data(), sq.data(), who.size(), count, flags, pick.data());
+    }
+    for (size_t j = 0; j < who.size(); j++) {
+        B->sched[who[j]] = pick[j];
+        B->shape[who[j]] = ff_shape_with(s0[j], pick[j]);
+    }
+    B->slots = (unsigned long long *)ff_dalloc((count ? count : 1) * 8);
     if (!(flags & (FF_BATCH_GROUPED | FF_BATCH_GRAPH))) {
         for (size_t i = 0; i < count; i++) {
             if (B->zero[i]) continue;

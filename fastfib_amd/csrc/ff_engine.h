@@ -3,6 +3,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "../../include/fastfib_amd.h"
+
 // A product's transform: width bw, length L = 2^lg or 3 * 2^lg (r3), conv
 // coefficients; split = 1 when no single transform fits (L > 2^25).
 struct ProdShape {
@@ -45,7 +47,12 @@ void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched 
 int ff_sched_inner_log(const FFSched &sc);
 int ff_sched_tile_words(const FFSched &sc);
 
-// The schedule a batch gives one product: the reference rule, the
-// autotuner table (FF_BATCH_TUNED) or the learned dispatcher
-// (FF_BATCH_LEARNED); count is the batch's product count.
-FFSched ff_pick_sched(const ProdShape &s0, uint64_t bits_a, uint64_t bits_b, bool square, size_t count, int flags);
+// The schedules of n products (shortest shapes s0, bit lengths, square
+// flags) in a batch of count products: the reference rule, the autotuner
+// table (FF_BATCH_TUNED) or the learned dispatcher (FF_BATCH_LEARNED, one
+// inference launch for all n).
+void ff_pick_scheds(const ProdShape *s0, const uint64_t *bits_a, const uint64_t *bits_b, const uint8_t *square,
+                    size_t n, size_t count, int flags, FFSched *out);
+
+// ff_batch_create with the schedules given (force[i] for item i) instead of picked.
+int ff_batch_create_sched(ff_batch **out, const ff_prod *items, size_t count, int flags, const FFSched *force);

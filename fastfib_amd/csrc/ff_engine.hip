@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/fastfib_amd.h"
+#include "ff_bigops.h"
 #include "ff_engine.h"
 #include "ff_internal.h"
 #include "ff_limbs.h"
@@ -243,9 +244,11 @@ void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched 
 // out[i] = A[i] * B[i] (B[i] == nullptr or == A[i]: square). Results get a
 // length slot each; with pred (bit lengths known in advance) nothing waits
 // for the device, otherwise the lengths are read back.
-static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int cnt, const uint64_t *pred) {
+static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int cnt, const uint64_t *pred,
+                     const FFSched *given = nullptr) {
     std::vector<MulItem> items(cnt);
     std::vector<ProdShape> shapes(cnt);
+    std::vector<FFSched> sched(cnt, FFSched{0, 0, 0});
     std::vector<int> zero(cnt, 0);
     for (int i = 0; i < cnt; i++) {
         const DNum *a = A[i], *b = B[i];
@@ -266,6 +269,14 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
             fprintf(stderr, "fastfib_amd: products past 2^25-word transforms are not supported yet\n");
             exit(1);
         }
+        if (given) {
+            sched[i] = given[i];
+        } else if (g_dispatch && !g_force_len) {
+            const uint64_t ba = a->bits;
+            const uint8_t s1 = sq;
+            ff_pick_scheds(&shapes[i], &ba, &bb, &s1, 1, (size_t)cnt, g_dispatch, &sched[i]);
+        }
+        shapes[i] = ff_shape_with(shapes[i], sched[i]);
         const uint64_t nb = sq ? a->n : b->n;
         const int slot = out[i].slot;
         out[i] = dnum_alloc(out_room(shapes[i], a->n, nb));
@@ -279,12 +290,13 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
         std::vector<MulItem> grp;
         for (int j = i; j < cnt; j++) {
             if (zero[j] || done[j]) continue;
-            if (shapes[j].L == shapes[i].L && shapes[j].bw == shapes[i].bw && (int)grp.size() < FF_MAXB) {
+            if (shapes[j].L == shapes[i].L && shapes[j].bw == shapes[i].bw && (int)grp.size() < FF_MAXB &&
+                !memcmp(&sched[j], &sched[i], sizeof(FFSched))) {
                 grp.push_back(items[j]);
                 done[j] = 1;
             }
         }
-        ff_mul_group(grp.data(), (int)grp.size(), shapes[i], FFSched{0, 0, 0});
+        ff_mul_group(grp.data(), (int)grp.size(), shapes[i], sched[i]);
     }
     if (pred) {
         for (int i = 0; i < cnt; i++) {
@@ -455,12 +467,40 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
                 predict = false;
         }
     }
-    for (int i = top - 1; i >= 1; i--) {
+    // With a tuned or learned dispatcher and predicted lengths, every
+    // product's schedule is picked up front in one call (one inference
+    // launch): two squarings per step, then the final product.
+    std::vector<FFSched> ds;
+    if (predict && g_dispatch && !g_force_len && top > 0) {
+        std::vector<ProdShape> s0;
+        std::vector<uint64_t> ba, bb;
+        std::vector<uint8_t> sqf;
+        uint64_t kk = 1, b;
+        for (int i = top - 1; i >= 1; i--) {
+            for (int j = 0; j < 2; j++) {
+                fib_bits_pred(kk - j, FB_F, &b);
+                b = b ? b : 1;
+                s0.push_back(product_shape(b, b)), ba.push_back(b), bb.push_back(b), sqf.push_back(1);
+            }
+            kk = 2 * kk + ((n >> i) & 1);
+        }
+        ds.resize(s0.size() + 1);
+        ff_pick_scheds(s0.data(), ba.data(), bb.data(), sqf.data(), s0.size(), 2, g_dispatch, ds.data());
+        uint64_t bx, by;
+        if (n & 1) fib_bits_pred(kk + 2, FB_F, &bx), fib_bits_pred(kk - 1, FB_L, &by);
+        else fib_bits_pred(kk, FB_F, &bx), fib_bits_pred(kk, FB_L, &by);
+        bx = bx ? bx : 1, by = by ? by : 1;
+        const ProdShape sf = product_shape(bx, by);
+        const uint8_t gen = 0;
+        if (!sf.split) ff_pick_scheds(&sf, &bx, &by, &gen, 1, 1, g_dispatch, &ds[s0.size()]);
+    }
+    int step = 0;
+    for (int i = top - 1; i >= 1; i--, step++) {
         DNum sq[2];
         const DNum *A[2] = {&fk, &fkm1}, *B[2] = {nullptr, nullptr};
         uint64_t ps[2] = {0, 0};
         if (predict) ps[0] = P(k, FB_F2), ps[1] = P(k - 1, FB_F2);
-        mul_many(A, B, sq, 2, predict ? ps : nullptr);
+        mul_many(A, B, sq, 2, predict ? ps : nullptr, ds.empty() ? nullptr : &ds[2 * step]);
         if (predict) {
             checks.push_back({sq[0].slot, ps[0]});
             if (k > 1) checks.push_back({sq[1].slot, ps[1]});
@@ -498,7 +538,7 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         dnum_free(fkm1);
         const DNum *A[1] = {&x}, *B[1] = {&y};
         uint64_t pr = predict ? P(n, FB_F) : 0;
-        mul_many(A, B, &result, 1, predict ? &pr : nullptr);
+        mul_many(A, B, &result, 1, predict ? &pr : nullptr, ds.empty() ? nullptr : &ds.back());
         dnum_free(x);
         dnum_free(y);
         const uint64_t len = result.n + 1 <= result.cap ? result.n + 1 : result.cap;
@@ -521,7 +561,7 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         dnum_free(fkm1);
         const DNum *A[1] = {&fk}, *B[1] = {&x};
         uint64_t pr = predict ? P(n, FB_F) : 0;
-        mul_many(A, B, &result, 1, predict ? &pr : nullptr);
+        mul_many(A, B, &result, 1, predict ? &pr : nullptr, ds.empty() ? nullptr : &ds.back());
         if (predict) checks.push_back({result.slot, pr});
         dnum_free(x);
         dnum_free(fk);
@@ -546,6 +586,18 @@ static DNum dfib_checked(uint64_t n) {
     }
     dfib(n, false, &r);
     return r;
+}
+
+// a * b for the limb helpers (ff_bigops.h): lengths read back.
+FFNum ff_num_mul(const FFNum &a, const FFNum *b) {
+    slots_reset(16);
+    DNum A{a.d, a.n, a.bits, a.cap, -1}, B{};
+    const bool sq = !b || b == &a || b->d == a.d;
+    if (!sq) B = DNum{b->d, b->n, b->bits, b->cap, -1};
+    const DNum *pa[1] = {&A}, *pb[1] = {sq ? nullptr : &B};
+    DNum R;
+    mul_many(pa, pb, &R, 1, nullptr);
+    return FFNum{R.d, R.n, R.bits, R.cap};
 }
 
 // ------------------------------------------------------------------ C-ABI

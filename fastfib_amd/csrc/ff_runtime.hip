@@ -21,8 +21,9 @@ std::recursive_mutex g_mu;  // the drop-in calls may come from several threads
 static char g_err[512];
 
 int g_inner_log = 12;
-int g_tile_words = 16384;
+int g_tile_words = 4096;
 uint64_t g_force_len = 0;
+int g_dispatch = 0;
 
 void ff_set_err(const char *fmt, ...) {
     va_list ap;
@@ -163,7 +164,7 @@ static void prof_collect(void) {
 
 static const char *kKindNames[FK_COUNT] = {"ntt_fwd_top", "ntt_fwd", "ntt_inner", "ntt_small", "ntt_inv",
                                            "ntt_inv_top", "crt_pack", "crt_scan", "crt_apply", "fib_step",
-                                           "carry", "normlen"};
+                                           "carry", "normlen", "bigops", "dec_leaf"};
 
 extern "C" int ff_prof_enable(int on) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -250,6 +251,9 @@ extern "C" int ff_set_option(const char *name, long long v) {
     } else if (!strcmp(name, "force_len")) {
         if (v < 0) return FF_ERR_ARG;
         g_force_len = (uint64_t)v;
+    } else if (!strcmp(name, "dispatch")) {
+        if (v != 0 && v != FF_BATCH_TUNED && v != FF_BATCH_LEARNED) return FF_ERR_ARG;
+        g_dispatch = (int)v;
     } else {
         return FF_ERR_ARG;
     }

@@ -61,6 +61,17 @@ FF_API Big ff_big_mul_ntt(const Big *a, const Big *b);
    with every product and linear step on the GPU. */
 FF_API Big ff_fibonacci(uint64_t n);
 
+/* Replaces big_to_string (src/decimal.c:229): the decimal digits of a,
+   without leading zeros ("0" for zero), as a malloc'ed NUL-terminated
+   string the caller frees with free(). Divide and conquer by powers
+   10^(19 * 2^i) with Barrett division, every level on the GPU. */
+FF_API char *ff_big_to_string(const Big *a);
+
+/* Replaces big_recip (src/decimal.c:39): floor(2^(2s) / P) minus at most
+   4, for P of exactly s bits, by the reference's Newton recursion with
+   every product on the GPU (exported for its parity tests). */
+FF_API Big ff_big_recip(const Big *P, size_t s);
+
 /* Replaces big_free (src/bigint.c:26): frees x's limbs, sets d = NULL, n = 0. */
 FF_API void ff_big_free(Big *x);
 
@@ -121,6 +132,15 @@ FF_API int ff_dmul(ff_dbig *out, const ff_dbig *a, const ff_dbig *b);
 /* out = F(n), computed and left in device memory. */
 FF_API int ff_dfibonacci(uint64_t n, ff_dbig *out);
 
+/* The decimal digits of a device number (as ff_big_to_string): *out gets
+   a malloc'ed string, *len its length. */
+FF_API int ff_dbig_to_string(const ff_dbig *a, char **out, size_t *len);
+
+/* The same conversion with the digits left in device memory: *dev_digits
+   points at the first of *len digits (no terminator), in a buffer the
+   library owns and reuses at the next conversion. */
+FF_API int ff_dbig_to_decimal(const ff_dbig *a, const char **dev_digits, size_t *len);
+
 /* ------------------------------------------------------------------------
  * Batched products (no counterpart in the reference, whose callers loop
  * over big_mul). A batch is built once for a fixed list of device operands
@@ -159,11 +179,72 @@ FF_API int ff_batch_run(ff_batch *b);
 FF_API int ff_batch_finish(ff_batch *b, ff_prod *items);
 FF_API void ff_batch_destroy(ff_batch *b);
 
+/* ------------------------------------------------------------------------
+ * Schedule autotuner and learned dispatcher (no counterpart in the
+ * reference, which always takes the shortest length and fixed block sizes:
+ * mul_ntt.c:103-118, ntt_conv.c:34-62).
+ *
+ * Candidate k (0 .. ff_tune_candidates()-1) is a schedule: the transform
+ * length (the shortest, or the next supported one up), the LDS-resident
+ * inner block (log2) and the outer-pass tile (LDS words). A batch created
+ * with FF_BATCH_TUNED takes each product's candidate from the autotuner
+ * table, with FF_BATCH_LEARNED from the MLP; when neither was set, both are
+ * read from dispatch.bin beside the library on first use. Without them a
+ * batch uses the reference rule (ff_tune_default_candidate()).
+ * --------------------------------------------------------------------- */
+
+FF_API int ff_tune_candidates(void);
+FF_API int ff_tune_default_candidate(void);
+FF_API int ff_tune_candidate(int k, int *len_up, int *inner_log, int *tile_words);
+
+/* The dispatcher's 8 features of a product in a batch of count products
+   (log2 bits of a and b, square, log2 count, free HBM fraction, log2 and
+   radix-3 flag of the shortest length, width - 32) and the mask of
+   candidates that apply to it. */
+FF_API int ff_tune_features(uint64_t bits_a, uint64_t bits_b, int square, size_t count, float *x, uint32_t *mask);
+
+/* Times every candidate for one product shape: per-product device
+   microseconds of a grouped batch of `copies` such products with random
+   operands, averaged over at least reps runs (more, up to 32, while the
+   runs take under a millisecond) after one warm run (NaN: candidate does
+   not apply). us has room for ff_tune_candidates() values. */
+FF_API int ff_tune_measure(uint64_t bits_a, uint64_t bits_b, int square, size_t copies, int reps, uint64_t seed,
+                           float *us);
+
+/* The autotuner table: one candidate per size class; ff_tune_class gives a
+   product's class (index into the table of ff_tune_table_size() entries). */
+FF_API int ff_tune_table_size(void);
+FF_API int ff_tune_class(uint64_t bits_a, uint64_t bits_b, int square, size_t count);
+FF_API int ff_tune_set_table(const int32_t *table, size_t n);
+
+/* Loads a dispatcher file (table and MLP parameters). */
+FF_API int ff_dispatch_load(const char *path);
+
+/* The MLP 8 -> 64 -> 64 -> K (FP32, ReLU): parameters in the order
+   mu[8] sd[8] (input normalization) W1[64][8] b1[64] W2[64][64] b2[64]
+   W3[K][64] b3[K]; outputs are predicted relative log-times. */
+FF_API size_t ff_mlp_param_count(int K);
+FF_API int ff_mlp_set(int K, const float *params, size_t count);
+FF_API int ff_mlp_get(int *K, float *params, size_t count);
+/* Mean-squared-error loss over n rows (x: n x 8, t: n x K) and its
+   gradient with respect to every parameter (mu, sd get 0), on the GPU. */
+FF_API int ff_mlp_grad(const float *x, const float *t, size_t n, float *loss, float *grad);
+/* Adam training on the GPU from the current parameters: epochs over the n
+   rows in shuffled minibatches of `batch`; loss_out gets the last epoch's
+   mean loss. */
+FF_API int ff_mlp_train(const float *x, const float *t, size_t n, int epochs, int batch, float lr, uint64_t seed,
+                        float *loss_out);
+/* Inference on the GPU: outputs (y, n x K) and argmin over mask bits
+   (choice; mask NULL allows all), either may be NULL. */
+FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float *y, int *choice);
+
 /* Schedule knobs (the reference's compile-time tunables, Appendix of its
    README: -DNAME=value, here set at run time). Names:
      "inner_log"   log2 of the LDS-resident inner block (4..12, default 12)
-     "tile_words"  LDS words of an outer-pass tile (1024..16384, default 16384)
+     "tile_words"  LDS words of an outer-pass tile (1024..16384, default 4096)
      "force_len"   transform length to use instead of the shortest (0: off)
+     "dispatch"    schedules of single products and F(n): 0 the reference
+                   rule (default), FF_BATCH_TUNED or FF_BATCH_LEARNED
    Returns FF_ERR_ARG for an unknown name or value. */
 FF_API int ff_set_option(const char *name, long long value);
 

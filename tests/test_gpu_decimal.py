@@ -1,0 +1,92 @@
+"""Decimal output on the GPU (ff_big_to_string, ff_big_recip), exact.
+
+Reciprocals are compared bit for bit with the oracle's restatement of
+big_recip (decimal.c:39-71). Strings are compared with Python's own int to
+str at sizes it finishes quickly, and with the reference's big_to_string
+(oracle/_ref) beyond. Cases cover every tree height from a single leaf up,
+the boundaries of the tree (10^(19 * 2^i) and its neighbours, 10^k - 1 with
+all nines, powers of two) and F(n).
+"""
+import numpy as np
+import pytest
+
+from fastfib_amd.workloads import operand_limbs
+from oracle.oracle import fib_int, from_limbs, to_limbs
+
+pytestmark = pytest.mark.gpu
+
+
+def test_recip_matches_oracle(ff, oracle):
+    rng = np.random.default_rng(7)
+    sizes = [2, 30, 62, 63, 64, 65, 127, 128, 129, 1000, 2020, 4039, 20000, 100000, 400000]
+    for s in sizes:
+        for _ in range(3):
+            P = from_limbs(operand_limbs(rng, s))
+            got = from_limbs(ff.big_recip(P, s))
+            want = from_limbs(oracle.recip(to_limbs(P), s))
+            assert got == want, s
+            exact = (1 << (2 * s)) // P
+            assert exact - 4 <= got <= exact, s
+
+
+def test_recip_of_decimal_powers(ff, oracle):
+    for i in range(0, 14):
+        P = 10 ** (19 * 2 ** i)
+        s = P.bit_length()
+        got = from_limbs(ff.big_recip(P, s))
+        assert got == from_limbs(oracle.recip(to_limbs(P), s)), i
+
+
+EDGE = [0, 1, 9, 10, 99, 10 ** 18, 10 ** 19 - 1, 10 ** 19, 10 ** 19 + 1, (1 << 64) - 1, 1 << 64, (1 << 128) + 1]
+
+
+def test_small_values(ff):
+    for x in EDGE:
+        assert ff.to_string(x) == str(x), x
+
+
+def test_tree_boundaries(ff):
+    """10^(19 * 2^i) and 10^(38 * 2^i) with their neighbours, all-nines and
+    powers of two, for every tree height up to 11."""
+    for i in range(0, 12):
+        for k in (19 * 2 ** i, 38 * 2 ** i):
+            P = 10 ** k
+            for x in (P - 1, P, P + 1, 2 * P - 1, P * P - 1 if k < 20000 else P - 2):
+                assert ff.to_string(x) == str(x), (k, x.bit_length())
+        b = 64 * 2 ** i
+        for x in ((1 << b) - 1, 1 << b, (1 << (2 * b - 2)) - 1):
+            assert ff.to_string(x) == str(x), b
+
+
+@pytest.mark.parametrize("bits", [100, 1000, 2019, 2021, 4040, 9000, 30000, 70000, 150000, 330000])
+def test_random_vs_python(ff, bits):
+    rng = np.random.default_rng(bits)
+    for _ in range(3):
+        x = from_limbs(operand_limbs(rng, bits))
+        assert ff.to_string(x) == str(x), bits
+
+
+@pytest.mark.parametrize("bits", [1_000_000, 4_000_000, 25_000_000])
+def test_large_vs_reference(ff, ref, bits):
+    rng = np.random.default_rng(bits)
+    A = operand_limbs(rng, bits)
+    assert ff.to_string(A) == ref.to_string(A)
+
+
+def test_fibonacci_digits(ff, ref):
+    """F(10^6) against Python, F(10^7) against the reference, both through
+    the device-resident entry (the number never leaves the GPU)."""
+    d = ff.dfibonacci(10 ** 6)
+    assert d.to_string() == str(fib_int(10 ** 6))
+    d = ff.dfibonacci(10 ** 7)
+    assert d.to_string() == ref.to_string(d.limbs())
+
+
+def test_repeat_and_mixed_heights(ff):
+    """Cached powers, reciprocals and level graphs: conversions of different
+    heights interleaved, each repeated, stay exact."""
+    rng = np.random.default_rng(3)
+    xs = [from_limbs(operand_limbs(rng, b)) for b in (5000, 200000, 3000, 200000, 60000)]
+    want = [str(x) for x in xs]
+    for _ in range(2):
+        assert [ff.to_string(x) for x in xs] == want

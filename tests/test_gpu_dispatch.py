@@ -1,0 +1,169 @@
+"""The autotuner and the learned dispatcher on the GPU.
+
+The MLP's HIP kernels (gradient, inference, training) are checked against
+the float64 numpy restatement in oracle/mlp.py; every schedule candidate is
+run through batches and checked bit-exact against Python ints; F(n) with a
+tuned or learned dispatcher stays bit-exact.
+"""
+import numpy as np
+import pytest
+
+from fastfib_amd import dispatch as dp
+from fastfib_amd.workloads import operand_limbs
+from oracle import mlp as om
+from oracle.oracle import FINGERPRINT_PRIMES, fib_fingerprint, from_limbs
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_problem(K, n, seed):
+    rng = np.random.default_rng(seed)
+    X = np.column_stack([rng.uniform(6, 29, n), rng.uniform(6, 29, n), rng.integers(0, 2, n),
+                         rng.uniform(0, 12, n), rng.uniform(0.5, 1, n), rng.uniform(4, 25, n),
+                         rng.integers(0, 2, n), rng.integers(0, 4, n)]).astype(np.float32)
+    T = rng.normal(size=(n, K)).astype(np.float32)
+    sd = X.std(0)
+    p = om.init_params(K, X.mean(0), np.where(sd > 1e-6, sd, 1.0), seed)
+    # non-zero biases so every path of the gradient is exercised
+    L = om.layout(K)
+    for name in ("b1", "b2", "b3"):
+        o, s = L[name]
+        p[o:o + s] = rng.normal(scale=0.1, size=s)
+    return X, T, p
+
+
+@pytest.mark.parametrize("n", [1, 37, 300])
+def test_mlp_gradient_matches_oracle(ff, n):
+    K = dp.candidates()
+    X, T, p = _random_problem(K, n, 3 + n)
+    dp.set_params(K, p)
+    L = ff.lib()
+    g = np.zeros(p.size, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    rc = L.ff_mlp_grad(dp._fp(X), dp._fp(T), n, dp._fp(loss), dp._fp(g))
+    assert rc == 0
+    want = om.grad(p, X, T, K)
+    # float32 kernels against a float64 restatement
+    assert np.max(np.abs(g - want)) <= 1e-4 * max(1.0, np.max(np.abs(want)))
+    assert abs(loss[0] - om.loss(p, X, T, K)) <= 1e-4 * max(1.0, om.loss(p, X, T, K))
+
+
+def test_mlp_inference_matches_oracle(ff):
+    K = dp.candidates()
+    X, T, p = _random_problem(K, 2000, 9)
+    dp.set_params(K, p)
+    rng = np.random.default_rng(1)
+    mask = rng.integers(1, 1 << K, size=X.shape[0], dtype=np.uint32)
+    y, c = dp.predict(X, mask)
+    want = om.forward(p, X, K)[3]
+    assert np.max(np.abs(y - want)) <= 1e-4 * max(1.0, np.max(np.abs(want)))
+    wc = om.choose(p, X, K, mask)
+    # the argmin agrees wherever the two best allowed outputs are not a float32 tie
+    ys = np.where(((mask[:, None].astype(np.int64) >> np.arange(K)) & 1) == 1, want, np.inf)
+    srt = np.sort(ys, axis=1)
+    clear = (srt[:, 1] - srt[:, 0]) > 1e-4
+    assert np.all(c[clear] == wc[clear])
+    assert np.all(((mask.astype(np.int64) >> c) & 1) == 1)
+
+
+def test_mlp_training_fits(ff):
+    """Adam on the GPU fits a smooth synthetic target far below its variance,
+    and the oracle's loss of the trained parameters agrees with the kernel's."""
+    K = dp.candidates()
+    rng = np.random.default_rng(4)
+    n = 4096
+    X = np.column_stack([rng.uniform(6, 29, (n, 2)), rng.integers(0, 2, n), rng.uniform(0, 12, n),
+                         rng.uniform(0.5, 1, n), rng.uniform(4, 25, n), rng.integers(0, 2, n),
+                         rng.integers(0, 4, n)]).astype(np.float32)
+    W = rng.normal(size=(8, K)) / 8
+    T = np.tanh((X - X.mean(0)) / X.std(0) @ W).astype(np.float32)
+    p0 = dp.init_params(K, X, 0)
+    dp.set_params(K, p0)
+    before = om.loss(p0, X, T, K)
+    last = dp.train(X, T, epochs=60, batch=256, lr=3e-3, seed=1)
+    _, p1 = dp.get_params()
+    after = om.loss(p1, X, T, K)
+    assert after < 0.1 * before, (before, after)
+    assert abs(after - last) < 0.5 * max(after, last) + 1e-3
+
+
+def test_measure_every_candidate(ff):
+    K = dp.candidates()
+    for ba, bb, sq, cnt in ((5000, 5000, 1, 64), (300000, 120000, 0, 4), (1 << 22, 1 << 22, 1, 1)):
+        us = dp.measure(ba, bb, sq, cnt, reps=1)
+        x, mask = dp.features(ba, bb, sq, cnt)
+        for k in range(K):
+            if (mask >> k) & 1:
+                assert np.isfinite(us[k]) and us[k] > 0, (ba, bb, k, us)
+            else:
+                assert np.isnan(us[k])
+
+
+def _pairs(ff, seed, sizes):
+    rng = np.random.default_rng(seed)
+    pairs, want = [], []
+    for ba, bb in sizes:
+        A = operand_limbs(rng, ba)
+        if bb is None:
+            pairs.append((ff.Device(A), None))
+            want.append(from_limbs(A) ** 2)
+        else:
+            B = operand_limbs(rng, bb)
+            pairs.append((ff.Device(A), ff.Device(B)))
+            want.append(from_limbs(A) * from_limbs(B))
+    return pairs, want
+
+
+SIZES = [(100, None), (3000, 2500), (9000, None), (40000, 70000), (200000, None), (640000, 500000),
+         (1 << 21, None), (3000000, 2900000)]
+
+
+def test_every_candidate_exact(ff):
+    """A table that names candidate k for every class runs every product of
+    a mixed batch under candidate k (where it applies): all exact."""
+    L = dp._L()
+    size = L.ff_tune_table_size()
+    pairs, want = _pairs(ff, 8, SIZES)
+    for k in range(dp.candidates()):
+        tab = np.full(size, k, dtype=np.int32)
+        assert L.ff_tune_set_table(tab.ctypes.data_as(dp.ctypes.POINTER(dp.ctypes.c_int32)), size) == 0
+        for flags in (ff.BATCH_GROUPED | ff.BATCH_TUNED, ff.BATCH_GRAPH | ff.BATCH_TUNED):
+            bt = ff.Batch(pairs, flags)
+            bt.run()
+            bt.finish()
+            assert bt.results() == want, (k, flags)
+            bt.close()
+
+
+def test_learned_batch_exact(ff):
+    """Random network parameters pick arbitrary candidates: still exact."""
+    K = dp.candidates()
+    X, T, p = _random_problem(K, 10, 21)
+    dp.set_params(K, p)
+    pairs, want = _pairs(ff, 9, SIZES)
+    bt = ff.Batch(pairs, ff.BATCH_GRAPH | ff.BATCH_LEARNED)
+    bt.run()
+    bt.finish()
+    assert bt.results() == want
+    bt.close()
+
+
+@pytest.mark.parametrize("mode", [4, 8])
+def test_fibonacci_with_dispatcher(ff, mode):
+    K = dp.candidates()
+    X, T, p = _random_problem(K, 10, 33)
+    dp.set_params(K, p)
+    L = dp._L()
+    size = L.ff_tune_table_size()
+    tab = (np.arange(size) % K).astype(np.int32)
+    L.ff_tune_set_table(tab.ctypes.data_as(dp.ctypes.POINTER(dp.ctypes.c_int32)), size)
+    ff.set_option("dispatch", mode)
+    try:
+        for n in (10 ** 6 + 1, 3 * 10 ** 6):
+            x = from_limbs(ff.dfibonacci(n).limbs())
+            fp = fib_fingerprint(n)
+            assert x.bit_length() == fp["bits"]
+            assert x % (1 << 64) == fp["low64"]
+            assert all(x % q == r for q, r in zip(FINGERPRINT_PRIMES, fp["mods"]))
+    finally:
+        ff.set_option("dispatch", 0)

@@ -155,7 +155,7 @@ def test_schedule_knobs(ff, inner_log, tile):
             assert from_limbs(ff.big_mul_ntt(a)) == a * a
         finally:
             ff.set_option("inner_log", 12)
-            ff.set_option("tile_words", 16384)
+            ff.set_option("tile_words", 4096)
 
 
 @pytest.mark.parametrize("x,y", [(64 * 1000, 64 * 1000), (64 * 100000, 64 * 100000), (64 * 542377, 64 * 542377),

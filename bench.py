@@ -135,7 +135,8 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float) -> dict
         pairs.append((ff.Device(A), None if B is None else ff.Device(B)))
         host.append((A, B))
     ff.sync()
-    forms = [("plain", ff.BATCH_PLAIN), ("grouped", ff.BATCH_GROUPED), ("graph", ff.BATCH_GRAPH)]
+    forms = [("plain", ff.BATCH_PLAIN), ("grouped", ff.BATCH_GROUPED), ("graph", ff.BATCH_GRAPH),
+             ("graph_tuned", ff.BATCH_GRAPH | ff.BATCH_TUNED), ("graph_learned", ff.BATCH_GRAPH | ff.BATCH_LEARNED)]
     out, ref_limbs, ok = {}, None, True
     for name, flags in forms:
         bt = ff.Batch(pairs, flags)
@@ -159,12 +160,112 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float) -> dict
         out[name] = {"ms": round(dt * 1e3, 3), "gbit_products_per_s": round(gbit / dt, 2), "bit_exact": good,
                      "steps": n_steps}
         bt.close()
-    best = max(("grouped", "graph"), key=lambda k: out[k]["gbit_products_per_s"])
+    plain = out["plain"]["gbit_products_per_s"]
     leg = {"workload": f"{count} uneven products, operands 2^12..2^24 bits log-uniform, 30% squares, seed {seed}",
            "operand_gbit": round(gbit, 3), "forms": out, "bit_exact": ok,
-           "speedup_vs_plain": round(out[best]["gbit_products_per_s"] / out["plain"]["gbit_products_per_s"], 2)}
+           "speedup_vs_plain": {k: round(v["gbit_products_per_s"] / plain, 2) for k, v in out.items() if k != "plain"}}
     leg["cpu_baseline"] = batch_cpu_baseline(host, gbit, cpu_seconds)
     return leg
+
+
+def dispatcher_regret() -> dict:
+    """Regret on the committed held-out autotuner samples (1000 mixes) of the
+    learned dispatcher (inference by the library's HIP kernel), the
+    autotuner table and the reference rule, against the exhaustive best."""
+    import numpy as np
+    from fastfib_amd import dispatch as dp
+
+    here = os.path.dirname(os.path.abspath(dp.__file__))
+    with np.load(os.path.join(here, "dispatch_data", "heldout.npz")) as z:
+        ho = {k: z[k] for k in z.files}
+    K, table, params = dp.read_bin(os.path.join(here, "dispatch.bin"))
+    dp.set_params(K, params)
+    _, choice = dp.predict(ho["X"], ho["mask"])
+    r = {"learned": dp.regret(ho["T_us"], choice, ho["mix"]),
+         "table": dp.regret(ho["T_us"], dp.table_choice(table, ho), ho["mix"]),
+         "reference_rule": dp.regret(ho["T_us"], np.full(ho["X"].shape[0], dp.default_candidate()), ho["mix"])}
+    return {k: {"mean": round(v["mean"], 4), "aggregate": round(v["aggregate"], 4), "p95": round(v["p95"], 4)}
+            for k, v in r.items()} | {"mixes": int(np.unique(ho["mix"]).size)}
+
+
+DEC_PRIMES = (2147483647, 2147483629, 2147483587, 1000000007)
+
+
+def digits_residue(s: str, p: int) -> int:
+    """The value of a decimal string modulo p < 2^31, by pairwise combination
+    of 9-digit chunks (numpy, no big integers)."""
+    import numpy as np
+
+    d = np.frombuffer(s.encode(), dtype=np.uint8).astype(np.int64) - 48
+    pad = (-d.size) % 9
+    d = np.concatenate([np.zeros(pad, np.int64), d]).reshape(-1, 9)
+    v = (d * (10 ** np.arange(8, -1, -1, dtype=np.int64))).sum(1) % p
+    m = 10 ** 9 % p
+    while v.size > 1:
+        if v.size % 2:
+            v = np.concatenate([np.zeros(1, np.int64), v])
+        v = (v[0::2] * m + v[1::2]) % p
+        m = m * m % p
+    return int(v[0])
+
+
+def fib_digits(n: int) -> int:
+    from mpmath import floor, log10, mp, sqrt
+
+    mp.dps = 50
+    phi = (1 + sqrt(5)) / 2
+    return int(floor(n * log10(phi) - log10(sqrt(5)))) + 1
+
+
+def decimal_leg(ff, n: int, steps: int, cpu_seconds: float) -> dict:
+    """Decimal digits of F(n) (device-resident input): the first conversion
+    (powers, reciprocals and level graphs built), then warm conversions with
+    the digits left in HBM, and one with the string copied to the host. The
+    string is checked by its length (Binet) and its value modulo four primes
+    against F(n) modulo them (fast doubling on residues)."""
+    L = ff.lib()
+    ff._dec_sigs(L)
+    L.ff_dbig_to_decimal.argtypes = [ctypes.POINTER(ff.DBig), ctypes.POINTER(ctypes.c_void_p),
+                                     ctypes.POINTER(ctypes.c_size_t)]
+    x = ff.dfibonacci(n)
+    ff.sync()
+    p, ln = ctypes.c_void_p(), ctypes.c_size_t()
+    t0 = time.perf_counter()
+    ff._check(L.ff_dbig_to_decimal(ctypes.byref(x.h), ctypes.byref(p), ctypes.byref(ln)))
+    cold = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ff._check(L.ff_dbig_to_decimal(ctypes.byref(x.h), ctypes.byref(p), ctypes.byref(ln)))
+    warm = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    s = x.to_string()
+    host = time.perf_counter() - t0
+    ok = len(s) == fib_digits(n) and s[0] != "0" and all(digits_residue(s, q) == fib_mod(n, q) for q in DEC_PRIMES)
+    leg = {"workload": f"decimal digits of F({n})", "digits": len(s), "bit_exact": ok,
+           "cold_ms": round(cold * 1e3, 3), "warm_device_ms": round(warm * 1e3, 3),
+           "warm_to_host_string_ms": round(host * 1e3, 3)}
+    limbs = x.limbs()
+    x.release()
+    leg["cpu_baseline"] = decimal_cpu_baseline(limbs, n, cpu_seconds)
+    return leg
+
+
+def decimal_cpu_baseline(limbs, n: int, seconds: float) -> dict | None:
+    """The reference's big_to_string of the same number on this host."""
+    try:
+        from oracle.oracle import Reference
+        ref = Reference()
+    except Exception as e:
+        print(f"cpu baseline unavailable: {e}", file=sys.stderr)
+        return None
+    times = []
+    t_end = time.perf_counter() + seconds
+    while not times or time.perf_counter() < t_end:
+        times.append(ref.time_to_string(limbs))
+        if len(times) >= 5:
+            break
+    return {"value": round(min(times) * 1e3, 2), "unit": "ms", "cores": ref.pool_cpus(), "kind": "reference",
+            "sample": f"big_to_string(F({n})) x {len(times)}, best"}
 
 
 def batch_cpu_baseline(host, gbit: float, seconds: float) -> dict | None:
@@ -199,6 +300,7 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=4096, help="uneven batch size (0: skip the batch leg)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--decimal-steps", type=int, default=5, help="warm decimal conversions of F(n) (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (products shard by prime or "
@@ -245,7 +347,12 @@ def main() -> int:
     batch = None
     if args.batch_count > 0:
         batch = batch_leg(ff, args.batch_count, args.batch_seed, args.batch_steps, args.cpu_seconds)
+        batch["dispatcher_regret"] = dispatcher_regret()
         ok &= batch["bit_exact"]
+    decimal = None
+    if args.decimal_steps > 0:
+        decimal = decimal_leg(ff, n, args.decimal_steps, args.cpu_seconds)
+        ok &= decimal["bit_exact"]
     line = {
         "metric": METRIC, "value": round(ms, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False, "scaling": "weak",
@@ -256,6 +363,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(cpu["value"] / ms, 2) if cpu else None,
         "batch": batch,
+        "decimal": decimal,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 4),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
                     for k, v in stats.items()},

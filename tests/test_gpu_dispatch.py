@@ -167,3 +167,34 @@ def test_fibonacci_with_dispatcher(ff, mode):
             assert all(x % q == r for q, r in zip(FINGERPRINT_PRIMES, fp["mods"]))
     finally:
         ff.set_option("dispatch", 0)
+
+
+def test_learned_batch_gain(ff):
+    """Batched with the committed learned dispatcher against one product per
+    launch sequence on an uneven batch: at least 1.5x, results identical."""
+    import os
+    import time
+
+    from fastfib_amd.workloads import uneven_mix
+
+    dp.load(os.path.join(os.path.dirname(dp.__file__), "dispatch.bin"))
+    rng = np.random.default_rng(5)
+    pairs = []
+    for ba, bb in uneven_mix(5, 512, 12.0, 22.0):
+        A = ff.Device(operand_limbs(rng, ba))
+        pairs.append((A, None if bb is None else ff.Device(operand_limbs(rng, bb))))
+    times, outs = {}, {}
+    for name, flags in (("plain", ff.BATCH_PLAIN), ("learned", ff.BATCH_GRAPH | ff.BATCH_LEARNED)):
+        bt = ff.Batch(pairs, flags)
+        bt.run()
+        bt.finish()
+        reps = 2 if name == "plain" else 10
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bt.run()
+        bt.finish()
+        times[name] = (time.perf_counter() - t0) / reps
+        outs[name] = [o.limbs() for o in bt.outs]
+        bt.close()
+    assert all(np.array_equal(x, y) for x, y in zip(outs["plain"], outs["learned"]))
+    assert times["plain"] / times["learned"] >= 1.5, times

@@ -65,6 +65,22 @@ def fingerprint_ok(n: int, limbs) -> bool:
     return all(x % p == fib_mod(n, p) for p in FP_PRIMES)
 
 
+def profiled_traffic(kind: str):
+    """HBM bytes per launch of a kernel kind from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy), or
+    None when the summary does not cover it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "fib_1e8_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == kind:
+                    return int(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def kernel_stats(ff) -> dict:
     L = ff.lib()
     L.ff_prof_read.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_long),
@@ -340,7 +356,8 @@ def main() -> int:
     per_launch_ms = d["ms"] / d["launches"]
     achieved = d["bytes"] / d["launches"] / (per_launch_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom),
+                "algorithmic_bytes_per_launch": round(d["bytes"] / d["launches"]),
                 "avg_launch_us": round(per_launch_ms * 1e3, 2), "launches": d["launches"]}
 
     cpu = cpu_baseline(n, args.cpu_seconds)

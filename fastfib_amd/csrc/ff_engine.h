@@ -25,6 +25,8 @@ struct FFSched {
 };
 
 ProdShape ff_product_shape(uint64_t bits_a, uint64_t bits_b);
+// The CRT bound (mul_ntt.c:44-53): min(ca, cb) * 2^(2w) < p1 p2 p3.
+int ff_width_ok(uint64_t bits_a, uint64_t bits_b, unsigned w);
 // The shape with the schedule's length choice applied.
 ProdShape ff_shape_with(const ProdShape &s, const FFSched &sc);
 
@@ -38,6 +40,9 @@ struct MulItem {
     uint64_t conv;      // product coefficients at the group's width
     unsigned long long *lenkey;  // where the CRT books the normalized length
 };
+
+// One product past the longest transform (split into transformed-once pieces).
+void ff_mul_split(const MulItem &it);
 
 // Runs up to FF_MAXB products sharing one shape and schedule, one launch
 // per pass (transform arrays come from the memory pool, stream-ordered).

@@ -31,6 +31,8 @@ static int width_ok(uint64_t bits_a, uint64_t bits_b, unsigned w) {
     return ((u128)(ca < cb ? ca : cb) << (2 * w)) < M;
 }
 
+int ff_width_ok(uint64_t bits_a, uint64_t bits_b, unsigned w) { return width_ok(bits_a, bits_b, w); }
+
 static uint64_t ntt_length(uint64_t conv, int *lg_out, int *r3_out) {
     uint64_t L = 16;
     int lg = 4, r3 = 0;
@@ -58,7 +60,7 @@ ProdShape ff_shape_with(const ProdShape &s0, const FFSched &sc) {
     if (!s.split && sc.len_up) {
         int lg = s.lg, r3 = s.r3;
         const uint64_t L = next_length(s.L, &lg, &r3);
-        if (L <= (1u << 25)) s.L = L, s.lg = lg, s.r3 = r3;
+        if (L <= (1u << g_max_lg)) s.L = L, s.lg = lg, s.r3 = r3;
     }
     return s;
 }
@@ -76,7 +78,7 @@ static ProdShape product_shape(uint64_t bits_a, uint64_t bits_b) {
         const uint64_t L = ntt_length(ca + cb - 1, &lg, &r3);
         if (s.L == 0 || L < s.L) s.L = L, s.lg = lg, s.r3 = r3, s.bw = w, s.conv = ca + cb - 1;
     }
-    if (s.L == 0 || s.L > (1u << 25)) s.split = 1;
+    if (s.L == 0 || s.L > (1u << g_max_lg)) s.split = 1;
     if (!s.split && g_force_len) {
         // a forced length must be supported (2^k or 3*2^k <= 2^25) and long enough
         const uint64_t F = g_force_len;
@@ -266,8 +268,12 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
         }
         shapes[i] = product_shape(a->bits, bb);
         if (shapes[i].split) {
-            fprintf(stderr, "fastfib_amd: products past 2^25-word transforms are not supported yet\n");
-            exit(1);
+            const uint64_t nb = sq ? a->n : b->n;
+            const int slot = out[i].slot;
+            out[i] = dnum_alloc(out_room(shapes[i], a->n, nb));
+            out[i].slot = slot;
+            items[i] = MulItem{a->d, a->n, a->bits, sq ? nullptr : b->d, nb, bb, out[i].d, 0, g_slots + slot};
+            continue;
         }
         if (given) {
             sched[i] = given[i];
@@ -287,9 +293,14 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
     std::vector<int> done(cnt, 0);
     for (int i = 0; i < cnt; i++) {
         if (zero[i] || done[i]) continue;
+        if (shapes[i].split) {
+            ff_mul_split(items[i]);
+            done[i] = 1;
+            continue;
+        }
         std::vector<MulItem> grp;
         for (int j = i; j < cnt; j++) {
-            if (zero[j] || done[j]) continue;
+            if (zero[j] || done[j] || shapes[j].split) continue;
             if (shapes[j].L == shapes[i].L && shapes[j].bw == shapes[i].bw && (int)grp.size() < FF_MAXB &&
                 !memcmp(&sched[j], &sched[i], sizeof(FFSched))) {
                 grp.push_back(items[j]);

@@ -136,6 +136,9 @@ hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, const FFOuterItem *
                                 uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
 hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerItem *items, const uint32_t *start, int n,
                                 uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
+// Inner blocks alone, in place (non-small plans): forward or inverse levels.
+hipError_t ff_launch_inner_part_list(bool inv, const FFInnerItem *items, const uint32_t *start, int n,
+                                     uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
 size_t ff_outer_lds_bytes(const FFPass *ps);
 size_t ff_inner_lds_bytes(const FFInner *in);
 

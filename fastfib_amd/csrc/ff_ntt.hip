@@ -320,6 +320,34 @@ __global__ __launch_bounds__(FF_THREADS, 4) void k_inner_list(const FFInnerItem 
     inner_body<SMALL, R3>(it.in, it.c, pi, local - (uint32_t)pi * nblk, T, PR, lds);
 }
 
+// Inner blocks of a transform on their own, in place: the forward levels
+// (INV false) or the inverse levels (INV true), no pointwise product (the
+// split multiply forms its pointwise sums between the two).
+template <bool INV>
+__global__ __launch_bounds__(FF_THREADS, 4) void k_inner_part_list(const FFInnerItem *items, const uint32_t *start,
+                                                                   int n, const FFTables *T, FFPrimes PR) {
+    extern __shared__ uint32_t lds[];
+    const int k = ff_find_item(start, n, blockIdx.x);
+    const FFInnerItem &it = items[k];
+    const uint32_t local = blockIdx.x - start[k];
+    const uint32_t nblk = it.in.L / it.in.S;
+    const int pi = (int)(local / nblk);
+    const uint32_t blkx = local - (uint32_t)pi * nblk;
+    const FFPrime P = PR.p[pi];
+    FFTile ta{lds, 1, 1, 0, it.in.s, 1, 1};
+    uint32_t *d = it.c.fa + (size_t)pi * it.in.L + (size_t)blkx * it.in.S;
+    auto ld = [&](int pos, int c) -> uint32_t {
+        (void)c;
+        return d[pos];
+    };
+    auto st = [&](int pos, int c, uint32_t v) {
+        (void)c;
+        d[pos] = v;
+    };
+    if (!INV) ff_forward_io(ta, T->rt[pi], T->rtp[pi], P.p, P.pinv, true, true, ld, st);
+    else ff_inverse_io(ta, T->irt[pi], T->irtp[pi], P.p, P.pinv, true, true, ld, st);
+}
+
 // ------------------------------------------------------------------ plans
 
 static int ilog2(uint64_t x) { return 63 - __builtin_clzll(x); }
@@ -448,6 +476,19 @@ extern "C" hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerIte
     if (!small) hipLaunchKernelGGL((k_inner_list<false, false>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
     else if (r3) hipLaunchKernelGGL((k_inner_list<true, true>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
     else hipLaunchKernelGGL((k_inner_list<true, false>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
+    ff_prof_end(kind, bytes, st);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ff_launch_inner_part_list(bool inv, const FFInnerItem *items, const uint32_t *start, int n,
+                                                uint32_t blocks, size_t lds, double bytes, hipStream_t st) {
+    if (!blocks) return hipSuccess;
+    const int kind = inv ? FK_INV : FK_FWD;
+    ff_prof_begin(kind, st);
+    if (!inv) hipLaunchKernelGGL((k_inner_part_list<false>), dim3(blocks), dim3(FF_THREADS), lds, st, items, start, n,
+                                 g_tables, g_primes_arg);
+    else hipLaunchKernelGGL((k_inner_part_list<true>), dim3(blocks), dim3(FF_THREADS), lds, st, items, start, n,
+                            g_tables, g_primes_arg);
     ff_prof_end(kind, bytes, st);
     return hipGetLastError();
 }

@@ -12,6 +12,7 @@ extern std::recursive_mutex g_mu;
 extern int g_inner_log;
 extern int g_tile_words;
 extern uint64_t g_force_len;
+extern int g_max_lg;    // longest single transform 2^max_lg; longer products split
 extern int g_dispatch;  // schedule source of the drop-in products and F(n): 0, FF_BATCH_TUNED or FF_BATCH_LEARNED
 
 void ff_set_err(const char *fmt, ...);

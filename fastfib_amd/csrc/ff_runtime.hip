@@ -24,6 +24,7 @@ int g_inner_log = 12;
 int g_tile_words = 4096;
 uint64_t g_force_len = 0;
 int g_dispatch = 0;
+int g_max_lg = 25;
 
 void ff_set_err(const char *fmt, ...) {
     va_list ap;
@@ -251,6 +252,9 @@ extern "C" int ff_set_option(const char *name, long long v) {
     } else if (!strcmp(name, "force_len")) {
         if (v < 0) return FF_ERR_ARG;
         g_force_len = (uint64_t)v;
+    } else if (!strcmp(name, "max_lg")) {
+        if (v < 12 || v > 25) return FF_ERR_ARG;
+        g_max_lg = (int)v;
     } else if (!strcmp(name, "dispatch")) {
         if (v != 0 && v != FF_BATCH_TUNED && v != FF_BATCH_LEARNED) return FF_ERR_ARG;
         g_dispatch = (int)v;

@@ -243,6 +243,9 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
      "inner_log"   log2 of the LDS-resident inner block (4..12, default 12)
      "tile_words"  LDS words of an outer-pass tile (1024..16384, default 4096)
      "force_len"   transform length to use instead of the shortest (0: off)
+     "max_lg"      longest single transform 2^max_lg (12..25, default 25);
+                   longer products split into transformed-once pieces (the
+                   reference's table bound NTT_MAX_LG, ntt_tables.c)
      "dispatch"    schedules of single products and F(n): 0 the reference
                    rule (default), FF_BATCH_TUNED or FF_BATCH_LEARNED
    Returns FF_ERR_ARG for an unknown name or value. */

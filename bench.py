@@ -266,6 +266,118 @@ def decimal_leg(ff, n: int, steps: int, cpu_seconds: float) -> dict:
     return leg
 
 
+def product_leg(ff, steps: int) -> dict:
+    """The single large product: two 2^26-bit operands (2^20 limbs from the
+    xorshift64 generator and seed of the reference's fuzzer, top bit set),
+    square and general, device-resident; L = 2^22 at b = 32 by the reference
+    rule, and the forced 3 * 2^21. Results checked against the committed
+    digests of the reference build's products; per-pass GB/s from the live
+    event timer."""
+    import hashlib
+
+    import numpy as np
+
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "golden.json")) as f:
+        golden = json.load(f)["product_2p26"]
+    state = [88172645463325252]
+
+    def limbs(n):
+        out = np.empty(n, dtype=np.uint64)
+        s = state[0]
+        for i in range(n):
+            s ^= (s << 13) & 0xFFFFFFFFFFFFFFFF
+            s ^= s >> 7
+            s ^= (s << 17) & 0xFFFFFFFFFFFFFFFF
+            out[i] = s
+        state[0] = s
+        out[-1] |= np.uint64(1 << 63)
+        return out
+
+    A, B = limbs(1 << 20), limbs(1 << 20)
+    da, db = ff.Device(A), ff.Device(B)
+    L = ff.lib()
+    out = {"workload": "2^26-bit operands (2^20 limbs), square and general product, device-resident"}
+    ok = True
+    for force in (0, 3 << 21):
+        ff.set_option("force_len", force)
+        try:
+            for name, other, key in (("square", None, "square_sha256"), ("general", db, "general_sha256")):
+                r = ff.dmul(da, other)
+                ok &= hashlib.sha256(r.limbs().tobytes()).hexdigest() == golden[key]
+                r.release()
+                ff.sync()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    ff.dmul(da, other).release()
+                ff.sync()
+                ms = (time.perf_counter() - t0) / steps * 1e3
+                L.ff_prof_reset()
+                L.ff_prof_enable(1)
+                ff.dmul(da, other).release()
+                ff.sync()
+                st = kernel_stats(ff)
+                L.ff_prof_enable(0)
+                passes = {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) for k, v in st.items()
+                          if v["ms"] > 0 and v["bytes"] > 0}
+                out[f"{name}_L{force or 1 << 22}"] = {"ms": round(ms, 4), "GBps_per_kernel": passes}
+        finally:
+            ff.set_option("force_len", 0)
+    out["bit_exact"] = ok
+    da.release()
+    db.release()
+    return out
+
+
+def large_fib_leg(ff, n: int, steps: int, cpu_seconds: float) -> dict:
+    """F(n) at a large n (10^9: 29 NTT products up to 3 * 2^23 words) with
+    the reference rule, the autotuner table and the learned dispatcher
+    choosing every product's schedule; each result checked by its bit
+    length and residues (fast doubling on residues)."""
+    out = {"workload": f"F({n}) fast doubling, device-resident", "n": n}
+    ok = True
+    for name, mode in (("reference_rule", 0), ("tuned", ff.BATCH_TUNED), ("learned", ff.BATCH_LEARNED)):
+        ff.set_option("dispatch", mode)
+        try:
+            ff.dfibonacci(n).release()
+            ff.sync()
+            t0 = time.perf_counter()
+            last = None
+            for _ in range(steps):
+                if last is not None:
+                    last.release()
+                last = ff.dfibonacci(n)
+            ff.sync()
+            ms = (time.perf_counter() - t0) / steps * 1e3
+        finally:
+            ff.set_option("dispatch", 0)
+        good = fingerprint_ok(n, last.limbs())
+        last.release()
+        ok &= good
+        out[name] = {"ms": round(ms, 3), "bit_exact": good}
+    out["bit_exact"] = ok
+    out["learned_vs_reference_rule"] = round(out["reference_rule"]["ms"] / out["learned"]["ms"], 3)
+    out["cpu_baseline"] = large_fib_cpu_baseline(n, cpu_seconds)
+    return out
+
+
+def large_fib_cpu_baseline(n: int, seconds: float) -> dict | None:
+    """The reference's fibonacci(n) on this host, best of a few runs."""
+    try:
+        from oracle.oracle import Reference
+        ref = Reference()
+    except Exception as e:
+        print(f"cpu baseline unavailable: {e}", file=sys.stderr)
+        return None
+    times = []
+    t_end = time.perf_counter() + seconds
+    while not times or time.perf_counter() < t_end:
+        times.append(ref.time_fibonacci(n))
+        if len(times) >= 5:
+            break
+    return {"value": round(min(times) * 1e3, 2), "unit": "ms", "cores": ref.pool_cpus(), "kind": "reference",
+            "sample": f"fibonacci({n}) x {len(times)}, best"}
+
+
 def decimal_cpu_baseline(limbs, n: int, seconds: float) -> dict | None:
     """The reference's big_to_string of the same number on this host."""
     try:
@@ -317,6 +429,9 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--decimal-steps", type=int, default=5, help="warm decimal conversions of F(n) (0: skip)")
+    ap.add_argument("--large-n", type=int, default=10 ** 9, help="F(n) leg with dispatcher schedules (0: skip)")
+    ap.add_argument("--large-steps", type=int, default=3)
+    ap.add_argument("--product-steps", type=int, default=10, help="2^26-bit product leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (products shard by prime or "
@@ -366,6 +481,14 @@ def main() -> int:
         batch = batch_leg(ff, args.batch_count, args.batch_seed, args.batch_steps, args.cpu_seconds)
         batch["dispatcher_regret"] = dispatcher_regret()
         ok &= batch["bit_exact"]
+    product = None
+    if args.product_steps > 0:
+        product = product_leg(ff, args.product_steps)
+        ok &= product["bit_exact"]
+    large = None
+    if args.large_n > 0:
+        large = large_fib_leg(ff, args.large_n, args.large_steps, args.cpu_seconds)
+        ok &= large["bit_exact"]
     decimal = None
     if args.decimal_steps > 0:
         decimal = decimal_leg(ff, n, args.decimal_steps, args.cpu_seconds)
@@ -381,6 +504,8 @@ def main() -> int:
         "speedup_vs_cpu": round(cpu["value"] / ms, 2) if cpu else None,
         "batch": batch,
         "decimal": decimal,
+        "large_fib": large,
+        "product_2p26": product,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 4),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
                     for k, v in stats.items()},

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <set>
 #include <tuple>
 #include <vector>
 
@@ -154,7 +155,24 @@ static void build_lists(ff_batch *B) {
     };
     std::vector<Pending> pend;
 
-    auto outer_list = [&](int pass, bool inv, int r3sel) {
+    // A launch allocates its largest item's LDS for every workgroup, so
+    // items are grouped by how many workgroups their LDS lets a CU hold.
+    auto occ = [](size_t lds) -> int {
+        const size_t k = lds ? (size_t)163840 / lds : 8;
+        return (int)(k < 8 ? k : 8);
+    };
+    auto outer_occs = [&](int pass, int r3sel) {
+        std::set<int> out;
+        for (size_t i : order) {
+            const FFPlan &pl = *plan[i];
+            if (pl.npass <= pass) continue;
+            const FFPass &ps = pl.pass[pass];
+            if (pass == 0 && (int)ps.r3 != r3sel) continue;
+            out.insert(occ(ff_outer_lds_bytes(&ps)));
+        }
+        return out;
+    };
+    auto outer_list = [&](int pass, bool inv, int r3sel, int oc) {
         std::vector<FFOuterItem> its;
         std::vector<uint32_t> start;
         uint32_t blocks = 0;
@@ -165,6 +183,7 @@ static void build_lists(ff_batch *B) {
             if (pl.npass <= pass) continue;
             const FFPass &ps = pl.pass[pass];
             if (pass == 0 && (int)ps.r3 != r3sel) continue;
+            if (occ(ff_outer_lds_bytes(&ps)) != oc) continue;
             const ff_prod &p = B->items[i];
             const bool sq = !p.b || p.b == p.a;
             for (int op = 0; op < (sq ? 1 : 2); op++) {
@@ -193,7 +212,16 @@ static void build_lists(ff_batch *B) {
         pd.start_off = put(start.data(), start.size() * 4);
         pend.push_back(pd);
     };
-    auto inner_list = [&](int small, int r3sel) {
+    auto inner_occs = [&](int small, int r3sel) {
+        std::set<int> out;
+        for (size_t i : order) {
+            const FFPlan &pl = *plan[i];
+            if (pl.inner.small != small || (small && pl.inner.r3 != r3sel)) continue;
+            out.insert(occ(ff_inner_lds_bytes(&pl.inner)));
+        }
+        return out;
+    };
+    auto inner_list = [&](int small, int r3sel, int oc) {
         std::vector<FFInnerItem> its;
         std::vector<uint32_t> start;
         uint32_t blocks = 0;
@@ -202,6 +230,7 @@ static void build_lists(ff_batch *B) {
         for (size_t i : order) {
             const FFPlan &pl = *plan[i];
             if (pl.inner.small != small || (small && pl.inner.r3 != r3sel)) continue;
+            if (occ(ff_inner_lds_bytes(&pl.inner)) != oc) continue;
             const ff_prod &p = B->items[i];
             const bool sq = !p.b || p.b == p.a;
             FFInnerItem ii;
@@ -225,23 +254,30 @@ static void build_lists(ff_batch *B) {
         pend.push_back(pd);
     };
 
+    // every (pass, radix-3) list once per occupancy class
+    auto outer_all = [&](int pass, bool inv, int r3sel) {
+        for (int oc : outer_occs(pass, r3sel)) outer_list(pass, inv, r3sel, oc);
+    };
+    auto inner_all = [&](int small, int r3sel) {
+        for (int oc : inner_occs(small, r3sel)) inner_list(small, r3sel, oc);
+    };
     for (int pass = 0; pass < 3; pass++) {
         if (pass == 0) {
-            outer_list(0, false, 1);
-            outer_list(0, false, 0);
+            outer_all(0, false, 1);
+            outer_all(0, false, 0);
         } else {
-            outer_list(pass, false, 0);
+            outer_all(pass, false, 0);
         }
     }
-    inner_list(0, 0);
-    inner_list(1, 1);
-    inner_list(1, 0);
+    inner_all(0, 0);
+    inner_all(1, 1);
+    inner_all(1, 0);
     for (int pass = 2; pass >= 0; pass--) {
         if (pass == 0) {
-            outer_list(0, true, 1);
-            outer_list(0, true, 0);
+            outer_all(0, true, 1);
+            outer_all(0, true, 0);
         } else {
-            outer_list(pass, true, 0);
+            outer_all(pass, true, 0);
         }
     }
     {

@@ -497,6 +497,11 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         }
         ds.resize(s0.size() + 1);
         ff_pick_scheds(s0.data(), ba.data(), bb.data(), sqf.data(), s0.size(), 2, g_dispatch, ds.data());
+        // the two squarings of a step run as one launch group: the smaller
+        // takes the larger one's schedule where their shapes agree
+        for (size_t j = 0; j + 1 < s0.size(); j += 2) {
+            if (s0[j + 1].L == s0[j].L && s0[j + 1].bw == s0[j].bw) ds[j + 1] = ds[j];
+        }
         uint64_t bx, by;
         if (n & 1) fib_bits_pred(kk + 2, FB_F, &bx), fib_bits_pred(kk - 1, FB_L, &by);
         else fib_bits_pred(kk, FB_F, &bx), fib_bits_pred(kk, FB_L, &by);

@@ -15,6 +15,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <chrono>
 #include <mutex>
 #include <vector>
 
@@ -51,10 +52,17 @@ int table_class(const ProdShape &s, bool sq, size_t count) {
     return (((lg - 4) * 2 + s.r3) * 2 + (sq ? 1 : 0)) * T_CNT + lg_count(count);
 }
 
+// Free HBM fraction, queried at most once a second (the query goes to the
+// driver and costs more than an inference).
 double free_hbm_fraction() {
+    static double cached = -1;
+    static std::chrono::steady_clock::time_point when;
+    const auto now = std::chrono::steady_clock::now();
+    if (cached >= 0 && now - when < std::chrono::seconds(1)) return cached;
     size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) != hipSuccess || tot == 0) return 1.0;
-    return (double)fr / (double)tot;
+    cached = (hipMemGetInfo(&fr, &tot) != hipSuccess || tot == 0) ? 1.0 : (double)fr / (double)tot;
+    when = now;
+    return cached;
 }
 
 void features(uint64_t bits_a, uint64_t bits_b, bool sq, size_t count, const ProdShape &s, double hbm, float *x) {

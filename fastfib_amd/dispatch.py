@@ -7,8 +7,8 @@
 A sample is one product drawn from a seeded random size mix (batch size,
 operand bit range, square fraction), with the autotuner's measured
 per-product time of every schedule candidate (ff_tune_measure: a grouped
-batch of copies of that product, copies = min(batch size, 512, 2^26 bits /
-product bits)). Training fits the library's MLP (ff_mlp_train, on the GPU)
+batch of copies of that product, copies = min(batch size, 1024, 2^29 bits /
+product bits): a product's marginal cost in a batch that keeps the GPU busy). Training fits the library's MLP (ff_mlp_train, on the GPU)
 to the candidates' relative log-times; the autotuner table takes, per size
 class, the candidate with the least mean log-time. Regret on held-out mixes
 compares, per mix, the summed time of the chosen candidates with the summed
@@ -29,8 +29,8 @@ import numpy as np
 from . import _check, lib
 
 NI = 8
-MAX_COPIES = 512
-FILL_BITS = 1 << 26
+MAX_COPIES = 1024
+FILL_BITS = 1 << 29
 
 
 def _sigs(L) -> None:

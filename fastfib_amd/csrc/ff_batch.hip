@@ -111,6 +111,10 @@ static void build_lists(ff_batch *B) {
     std::vector<const FFPlan *> plan(n, nullptr);
     B->work.assign(n, nullptr);
     std::vector<uint32_t *> fa(n), fb(n);
+    std::vector<char> own_b(n, 0);  // the item transforms its b (else it reads an earlier item's)
+    // products sharing their b operand (same pointer, limbs, width and
+    // schedule) share one transform of it
+    std::map<std::tuple<const uint64_t *, uint64_t, uint64_t, unsigned, const FFPlan *>, uint32_t *> shared_b;
     std::vector<FFCrtJob> jobs(n);
     for (size_t i : order) {
         const ProdShape &s = B->shape[i];
@@ -131,12 +135,21 @@ static void build_lists(ff_batch *B) {
         const size_t words = 3 * (size_t)s.L;
         const uint64_t nout = p.na + (sq ? p.na : p.nb) + 1;
         const uint64_t nblk = (nout + 255) / 256;
-        const size_t o_fb = al(words * 4), o_bgp = o_fb + (sq ? 0 : al(words * 4));
+        uint32_t *sb = nullptr;
+        if (!sq) {
+            auto bk = std::make_tuple(p.b, (uint64_t)p.nb, (uint64_t)p.bits_b, s.bw, plan[i]);
+            auto f = shared_b.find(bk);
+            if (f != shared_b.end()) sb = f->second;
+        }
+        const bool ownb = !sq && !sb;
+        const size_t o_fb = al(words * 4), o_bgp = o_fb + (ownb ? al(words * 4) : 0);
         const size_t o_bcin = o_bgp + al(nblk * 4), o_mode = o_bcin + al(nblk);
         char *w = (char *)ff_dalloc(o_mode + 256);
         B->work[i] = w;
         fa[i] = (uint32_t *)w;
-        fb[i] = sq ? nullptr : (uint32_t *)(w + o_fb);
+        fb[i] = sq ? nullptr : (ownb ? (uint32_t *)(w + o_fb) : sb);
+        own_b[i] = ownb;
+        if (ownb) shared_b[std::make_tuple(p.b, (uint64_t)p.nb, (uint64_t)p.bits_b, s.bw, plan[i])] = fb[i];
         jobs[i] = FFCrtJob{fa[i], p.out, (uint32_t *)(w + o_bgp), (uint8_t *)(w + o_bcin), (uint32_t *)(w + o_mode),
                            B->slots + i, nout, s.conv, (uint32_t)s.L, (int)s.bw};
     }
@@ -187,6 +200,7 @@ static void build_lists(ff_batch *B) {
             const ff_prod &p = B->items[i];
             const bool sq = !p.b || p.b == p.a;
             for (int op = 0; op < (sq ? 1 : 2); op++) {
+                if (op == 1 && !inv && !own_b[i]) continue;  // b transformed once for its sharers
                 FFOuterItem oi;
                 oi.ps = ps;
                 oi.x = FFXform{nullptr, 0, op ? fb[i] : fa[i]};

@@ -318,3 +318,167 @@ extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *s
     ff_prof_end(FK_CRT_APPLY, 4.0 * out_limbs_total / CRT_LIMBS, st);
     return hipGetLastError();
 }
+
+// ------------------------------------------------------------------ fused small step
+//
+// A doubling step whose two squarings are one-block transforms (L <= 4096)
+// is latency-bound: its CRT and linear pass would be six launches of a few
+// microseconds each. Here one workgroup of 1024 threads does all of it from
+// the squares' residues: the CRT of F(k)^2 and F(k-1)^2 into registers
+// (thread t owns limbs t, t + 1024, t + 2048), then A = s1 + s2,
+// B = 4 s1 - s2 + cb0, C = B - A (fib.c:36-62) by workgroup-wide carry
+// lookahead (wavefront ballots, 16 wave summaries composed in LDS, chunks
+// of 1024 limbs in order), writing K (A or B) and C and booking their top
+// limbs.
+
+#define SS_T 1024
+#define SS_CH 3
+#define SS_MAXC 2112  // coefficients touching 1025 limbs at bw >= 32, with margin
+
+struct SSShared {
+    uint64_t xlo[SS_MAXC];
+    uint32_t xhi[SS_MAXC];
+    uint64_t ex[SS_T];
+    uint32_t wg[SS_T / 64], wp[SS_T / 64], wc[SS_T / 64];
+    uint32_t carry;
+    uint64_t hi_last;
+};
+
+// Carry into this thread's limb for the chunk: u (limb sum), g (generates),
+// p (propagates), cin (carry into the chunk); returns u + carry and updates
+// S.carry to the chunk's carry-out. Every thread of the block calls it.
+__device__ __forceinline__ uint64_t ss_lookahead(SSShared &S, uint64_t u, bool g, bool p) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t G = __ballot(g), B = G | __ballot(p);
+    if (lane == 0) {
+        S.wg[w] = (G + B) < G;
+        S.wp[w] = (B == ~0ull && G == 0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = S.carry;
+        for (int k = 0; k < SS_T / 64; k++) {
+            S.wc[k] = c;
+            c = S.wg[k] | (S.wp[k] & c);
+        }
+        S.carry = c;
+    }
+    __syncthreads();
+    const uint64_t C = (G + B + (uint64_t)S.wc[w]) ^ G ^ B;
+    const uint64_t r = u + ((C >> lane) & 1);
+    __syncthreads();  // S.wc / S.carry are rewritten by the next call
+    return r;
+}
+
+// Limbs [0, len) of the product whose residues are y ([3][L]) into out[c]
+// (limb c * 1024 + threadIdx.x).
+__device__ void ss_crt(SSShared &S, const uint32_t *y, uint32_t L, int bw, uint32_t conv, uint32_t len,
+                       const FFGarner &G, uint64_t out[SS_CH]) {
+    const int t = threadIdx.x;
+    if (threadIdx.x == 0) S.carry = 0, S.hi_last = 0;
+    __syncthreads();
+    for (int c = 0; c < SS_CH; c++) {
+        out[c] = 0;
+        const uint64_t i0 = (uint64_t)c * SS_T;
+        if (i0 >= len || conv == 0) continue;  // uniform across the block
+        const uint64_t i1 = i0 + SS_T < len ? i0 + SS_T : len;
+        const uint64_t jbase = first_coef(i0 ? i0 - 1 : 0, bw);
+        uint64_t jend = (64 * i1 + bw - 1) / bw;
+        if (jend > conv) jend = conv;
+        for (uint64_t j = jbase + t; j < jend; j += SS_T) {
+            const u128 X = ff_garner(y[j], y[L + j], y[2 * (size_t)L + j], G);
+            S.xlo[j - jbase] = (uint64_t)X;
+            S.xhi[j - jbase] = (uint32_t)(X >> 64);
+        }
+        __syncthreads();
+        const uint64_t i = i0 + t;
+        const bool live = i < i1;
+        const u128 T = live ? limb_sum(i, bw, jbase, S.xlo, S.xhi, jbase, jend) : 0;
+        S.ex[t] = (uint64_t)(T >> 64);
+        __syncthreads();
+        const uint64_t hprev = t ? S.ex[t - 1] : S.hi_last;
+        const u128 u = (u128)(uint64_t)T + hprev;
+        const bool g = live && (u >> 64) != 0;
+        const bool p = !live || (!g && (uint64_t)u == ~0ull);
+        __syncthreads();
+        if (t == SS_T - 1) S.hi_last = S.ex[t];
+        const uint64_t v = ss_lookahead(S, (uint64_t)u, g, p);
+        out[c] = live ? v : 0;
+    }
+}
+
+// out = a + b' + cin0 over len limbs in registers (b' = b or ~b).
+__device__ void ss_add(SSShared &S, const uint64_t a[SS_CH], const uint64_t b[SS_CH], bool neg, uint32_t cin0,
+                       uint32_t len, uint64_t out[SS_CH]) {
+    if (threadIdx.x == 0) S.carry = cin0;
+    __syncthreads();
+    for (int c = 0; c < SS_CH; c++) {
+        const uint64_t i = (uint64_t)c * SS_T + threadIdx.x;
+        if ((uint64_t)c * SS_T >= len) {
+            out[c] = 0;
+            continue;
+        }
+        const bool live = i < len;
+        const uint64_t bv = neg ? ~b[c] : b[c];
+        const uint64_t u = a[c] + bv;
+        const bool g = live && u < a[c];
+        const bool p = !live || u == ~0ull;
+        const uint64_t v = ss_lookahead(S, u, g, p);
+        out[c] = live ? v : 0;
+    }
+}
+
+__global__ __launch_bounds__(SS_T) void k_small_step(FFSmallStep a, FFGarner G) {
+    __shared__ SSShared S;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        atomicExch(a.keyK, 0ull);
+        atomicExch(a.keyC, 0ull);
+        __threadfence();
+    }
+    uint64_t s1[SS_CH], s2[SS_CH], A[SS_CH], D[SS_CH], B[SS_CH], C[SS_CH];
+    ss_crt(S, a.y1, a.L1, a.bw1, a.conv1, a.len, G, s1);
+    ss_crt(S, a.y2, a.L2, a.bw2, a.conv2, a.len, G, s2);
+    ss_add(S, s1, s2, false, 0, a.len, A);
+    // 4 s1 (limb i takes the top two bits of limb i - 1), then + cb0
+    for (int c = 0; c < SS_CH; c++) {
+        S.ex[t] = s1[c];
+        __syncthreads();
+        const uint64_t prev = t ? S.ex[t - 1] : (c ? S.hi_last : 0);
+        __syncthreads();
+        if (t == SS_T - 1) S.hi_last = s1[c];
+        D[c] = (s1[c] << 2) | (prev >> 62);
+        __syncthreads();
+    }
+    {
+        // cb0 = +2: add 2; cb0 = -2: add 2^(64 len) - 2 (limb 0 = ~1, the rest all ones)
+        uint64_t k[SS_CH];
+        for (int c = 0; c < SS_CH; c++) {
+            const uint64_t i = (uint64_t)c * SS_T + t;
+            k[c] = a.cb0 >= 0 ? (i == 0 ? (uint64_t)a.cb0 : 0) : (i == 0 ? ~(uint64_t)(-a.cb0 - 1) : ~0ull);
+        }
+        ss_add(S, D, k, false, 0, a.len, D);
+    }
+    ss_add(S, D, s2, true, 1, a.len, B);
+    ss_add(S, B, A, true, 1, a.len, C);
+    const uint64_t lo_top = a.len >= 3 ? a.len - 3 : 0;
+    for (int c = 0; c < SS_CH; c++) {
+        const uint64_t i = (uint64_t)c * SS_T + t;
+        if (i >= a.len) break;
+        const uint64_t kv = a.bit ? B[c] : A[c];
+        a.K[i] = kv;
+        a.C[i] = C[c];
+        if (i >= lo_top) {
+            if (kv) atomicMax(a.keyK, ((unsigned long long)(i + 1) << 8) | (unsigned)(64 - __clzll((long long)kv)));
+            if (C[c]) atomicMax(a.keyC, ((unsigned long long)(i + 1) << 8) | (unsigned)(64 - __clzll((long long)C[c])));
+        }
+    }
+}
+
+extern "C" hipError_t ff_launch_small_step(const FFSmallStep *a, hipStream_t st) {
+    garner_init();
+    ff_prof_begin(FK_STEP, st);
+    hipLaunchKernelGGL(k_small_step, dim3(1), dim3(SS_T), 0, st, *a, g_garner);
+    ff_prof_end(FK_STEP, 12.0 * (a->conv1 + a->conv2) + 16.0 * a->len, st);
+    return hipGetLastError();
+}

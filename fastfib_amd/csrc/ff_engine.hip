@@ -364,6 +364,77 @@ static void fib_step(const DNum &s1, const DNum &s2, int k_odd, int bit, DNum *l
     else *lo = K, *hi = C;
 }
 
+// A step whose squarings are one-block transforms (L <= 4096), with
+// predicted lengths: the two transforms (one launch when their shapes
+// agree), then the CRT of both squares and the linear step fused in one
+// workgroup (ff_launch_small_step) instead of six launches.
+static bool small_fusable(const DNum &fk) {
+    if (g_force_len || g_dispatch || fk.bits == 0) return false;
+    const ProdShape s = product_shape(fk.bits, fk.bits);
+    return !s.split && s.L <= FF_TW && s.bw >= 32;
+}
+
+static void small_step(const DNum &fk, const DNum &fkm1, int k_odd, int bit, DNum *lo, DNum *hi,
+                       uint64_t bits_sq1, const uint64_t pred[2]) {
+    const ProdShape s1 = product_shape(fk.bits, fk.bits);
+    const bool z2 = fkm1.bits == 0;
+    const ProdShape s2 = z2 ? s1 : product_shape(fkm1.bits, fkm1.bits);
+    const uint64_t len = (bits_sq1 + 63) / 64 + 1;  // s2 <= s1, results < 4 s1 + 2
+    uint32_t *fa1 = (uint32_t *)ff_dalloc(3 * (size_t)s1.L * 4);
+    uint32_t *fa2 = z2 ? nullptr : (uint32_t *)ff_dalloc(3 * (size_t)s2.L * 4);
+    FFPlan p1;
+    if (ff_plan_build(&p1, (uint32_t)s1.L, (int)s1.bw, s1.lg, s1.r3, s1.conv, g_inner_log, g_tile_words) != 0) {
+        fprintf(stderr, "fastfib_amd: no schedule for L=%llu\n", (unsigned long long)s1.L);
+        exit(1);
+    }
+    FFConvSet cv = {};
+    cv.c[0] = FFConv{(const uint32_t *)fk.d, nullptr, (uint32_t)(2 * fk.n), 0, fa1, nullptr};
+    int nc = 1;
+    const bool same = !z2 && s2.L == s1.L && s2.bw == s1.bw;
+    if (same) {
+        cv.c[1] = FFConv{(const uint32_t *)fkm1.d, nullptr, (uint32_t)(2 * fkm1.n), 0, fa2, nullptr};
+        nc = 2;
+    }
+    FF_CHECK(ff_launch_conv(&p1, &cv, nc, 0, 8.0 * (fk.n + (same ? fkm1.n : 0)), g_st));
+    if (!z2 && !same) {
+        FFPlan p2;
+        if (ff_plan_build(&p2, (uint32_t)s2.L, (int)s2.bw, s2.lg, s2.r3, s2.conv, g_inner_log, g_tile_words) != 0) {
+            fprintf(stderr, "fastfib_amd: no schedule for L=%llu\n", (unsigned long long)s2.L);
+            exit(1);
+        }
+        FFConvSet c2 = {};
+        c2.c[0] = FFConv{(const uint32_t *)fkm1.d, nullptr, (uint32_t)(2 * fkm1.n), 0, fa2, nullptr};
+        FF_CHECK(ff_launch_conv(&p2, &c2, 1, 0, 8.0 * fkm1.n, g_st));
+    }
+    DNum K = dnum_alloc(len), C = dnum_alloc(len);
+    K.slot = slot_new();
+    C.slot = slot_new();
+    FFSmallStep a{fa1,
+                  z2 ? fa1 : fa2,
+                  (uint32_t)s1.L,
+                  (uint32_t)s2.L,
+                  (int)s1.bw,
+                  (int)s2.bw,
+                  (uint32_t)s1.conv,
+                  z2 ? 0u : (uint32_t)s2.conv,
+                  (uint32_t)len,
+                  k_odd ? -2 : 2,
+                  bit,
+                  K.d,
+                  C.d,
+                  g_slots + K.slot,
+                  g_slots + C.slot};
+    FF_CHECK(ff_launch_small_step(&a, g_st));
+    ff_dfree(fa1);
+    if (fa2) ff_dfree(fa2);
+    // pred[0], pred[1]: bit lengths of (lo, hi)
+    const uint64_t bk = bit ? pred[1] : pred[0], bc = bit ? pred[0] : pred[1];
+    K.bits = bk, K.n = bk ? (bk + 63) / 64 : 1;
+    C.bits = bc, C.n = bc ? (bc + 63) / 64 : 1;
+    if (bit) *lo = C, *hi = K;
+    else *lo = K, *hi = C;
+}
+
 // X = 2u + v and, if Y, Y = 2u - v (fib.c:145-161).
 static void last_factors(const DNum &u, const DNum &v, DNum *X, DNum *Y, const uint64_t *pred) {
     const uint64_t len = (u.n > v.n ? u.n : v.n) + 1;
@@ -512,6 +583,21 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
     }
     int step = 0;
     for (int i = top - 1; i >= 1; i--, step++) {
+        if (predict && small_fusable(fk)) {
+            const int bit = (int)((n >> i) & 1);
+            const uint64_t bsq = P(k, FB_F2);
+            uint64_t pl[2] = {P(2 * k - 1 + bit, FB_F), P(2 * k + bit, FB_F)};
+            DNum nlo, nhi;
+            small_step(fk, fkm1, k_odd, bit, &nlo, &nhi, bsq, pl);
+            checks.push_back({nlo.slot, pl[0]});
+            checks.push_back({nhi.slot, pl[1]});
+            dnum_free(fk);
+            dnum_free(fkm1);
+            fkm1 = nlo, fk = nhi;
+            k = 2 * k + bit;
+            k_odd = bit;
+            continue;
+        }
         DNum sq[2];
         const DNum *A[2] = {&fk, &fkm1}, *B[2] = {nullptr, nullptr};
         uint64_t ps[2] = {0, 0};

@@ -136,6 +136,22 @@ hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, const FFOuterItem *
                                 uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
 hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerItem *items, const uint32_t *start, int n,
                                 uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
+// A doubling step whose squarings are one-block transforms, fused: the CRT
+// of both squares and the linear step (fib.c:36-62) in one workgroup.
+// keyK / keyC must read zero before the launch.
+struct FFSmallStep {
+    const uint32_t *y1, *y2;  // residues of F(k)^2 and F(k-1)^2 ([3][L], CRT-ready); y2 unused if conv2 = 0
+    uint32_t L1, L2;
+    int bw1, bw2;
+    uint32_t conv1, conv2;    // coefficients of each square (0: zero)
+    uint32_t len;             // limbs of the step: n(F(k)^2) + 1, at most 3072
+    int cb0;                  // 2 (-1)^k
+    int bit;
+    uint64_t *K, *C;          // K = bit ? B : A, C = B - A (len limbs each)
+    unsigned long long *keyK, *keyC;
+};
+hipError_t ff_launch_small_step(const FFSmallStep *a, hipStream_t st);
+
 // Inner blocks alone, in place (non-small plans): forward or inverse levels.
 hipError_t ff_launch_inner_part_list(bool inv, const FFInnerItem *items, const uint32_t *start, int n,
                                      uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);

@@ -144,3 +144,25 @@ def test_uneven_4096_residues(ff):
             v = o.value()
             assert [v % p for p in PRIMES] == res[i], f"product {i} {mix[i]} flags {flags}"
         bt.close()
+
+
+def test_shared_operand(ff):
+    """Many products with one common b (transformed once for all of them),
+    mixed with squares and other products, in every form."""
+    rng = np.random.default_rng(31)
+    B = operand_limbs(rng, 300000)
+    db = ff.Device(B)
+    b = from_limbs(B)
+    pairs, want = [], []
+    for bits in (1000, 5000, 90000, 300000, 700000, 300000, 64):
+        A = operand_limbs(rng, bits)
+        pairs.append((ff.Device(A), db))
+        want.append(from_limbs(A) * b)
+    pairs.append((db, None))
+    want.append(b * b)
+    for flags in (0, 1, 2):
+        bt = ff.Batch(pairs, flags)
+        bt.run()
+        bt.finish()
+        assert bt.results() == want, flags
+        bt.close()

@@ -196,7 +196,8 @@ def dispatcher_regret() -> dict:
         ho = {k: z[k] for k in z.files}
     K, table, params = dp.read_bin(os.path.join(here, "dispatch.bin"))
     dp.set_params(K, params)
-    _, choice = dp.predict(ho["X"], ho["mask"])
+    y, _ = dp.predict(ho["X"], ho["mask"])
+    choice = dp.dispatch_choice(y, ho["mask"], dp.default_candidate())
     r = {"learned": dp.regret(ho["T_us"], choice, ho["mix"]),
          "table": dp.regret(ho["T_us"], dp.table_choice(table, ho), ho["mix"]),
          "reference_rule": dp.regret(ho["T_us"], np.full(ho["X"].shape[0], dp.default_candidate()), ho["mix"])}
@@ -331,31 +332,33 @@ def product_leg(ff, steps: int) -> dict:
 def large_fib_leg(ff, n: int, steps: int, cpu_seconds: float) -> dict:
     """F(n) at a large n (10^9: 29 NTT products up to 3 * 2^23 words) with
     the reference rule, the autotuner table and the learned dispatcher
-    choosing every product's schedule; each result checked by its bit
-    length and residues (fast doubling on residues)."""
-    out = {"workload": f"F({n}) fast doubling, device-resident", "n": n}
-    ok = True
-    for name, mode in (("reference_rule", 0), ("tuned", ff.BATCH_TUNED), ("learned", ff.BATCH_LEARNED)):
-        ff.set_option("dispatch", mode)
-        try:
-            ff.dfibonacci(n).release()
-            ff.sync()
-            t0 = time.perf_counter()
-            last = None
-            for _ in range(steps):
-                if last is not None:
-                    last.release()
-                last = ff.dfibonacci(n)
-            ff.sync()
-            ms = (time.perf_counter() - t0) / steps * 1e3
-        finally:
-            ff.set_option("dispatch", 0)
-        good = fingerprint_ok(n, last.limbs())
-        last.release()
-        ok &= good
-        out[name] = {"ms": round(ms, 3), "bit_exact": good}
-    out["bit_exact"] = ok
-    out["learned_vs_reference_rule"] = round(out["reference_rule"]["ms"] / out["learned"]["ms"], 3)
+    choosing every product's schedule, run interleaved (one of each per
+    round, best of the rounds); each result checked by its bit length and
+    residues (fast doubling on residues)."""
+    modes = (("reference_rule", 0), ("tuned", ff.BATCH_TUNED), ("learned", ff.BATCH_LEARNED))
+    best = {name: float("inf") for name, _ in modes}
+    good = {name: True for name, _ in modes}
+    try:
+        for r in range(steps + 1):
+            for name, mode in modes:
+                ff.set_option("dispatch", mode)
+                ff.sync()
+                t0 = time.perf_counter()
+                x = ff.dfibonacci(n)
+                ff.sync()
+                dt = time.perf_counter() - t0
+                if r == 0:  # warm-up round, checked
+                    good[name] = fingerprint_ok(n, x.limbs())
+                else:
+                    best[name] = min(best[name], dt)
+                x.release()
+    finally:
+        ff.set_option("dispatch", 0)
+    out = {"workload": f"F({n}) fast doubling, device-resident", "n": n, "rounds": steps}
+    for name, _ in modes:
+        out[name] = {"ms": round(best[name] * 1e3, 3), "bit_exact": good[name]}
+    out["bit_exact"] = all(good.values())
+    out["learned_vs_reference_rule"] = round(best["reference_rule"] / best["learned"], 3)
     out["cpu_baseline"] = large_fib_cpu_baseline(n, cpu_seconds)
     return out
 
@@ -430,7 +433,7 @@ def main() -> int:
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--decimal-steps", type=int, default=5, help="warm decimal conversions of F(n) (0: skip)")
     ap.add_argument("--large-n", type=int, default=10 ** 9, help="F(n) leg with dispatcher schedules (0: skip)")
-    ap.add_argument("--large-steps", type=int, default=3)
+    ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--product-steps", type=int, default=10, help="2^26-bit product leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:

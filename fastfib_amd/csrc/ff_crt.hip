@@ -331,13 +331,14 @@ extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *s
 // of 1024 limbs in order), writing K (A or B) and C and booking their top
 // limbs.
 
-#define SS_T 1024
-#define SS_CH 3
-#define SS_MAXC 2112  // coefficients touching 1025 limbs at bw >= 32, with margin
+#define SS_CH 3  // chunks of SS_T limbs: len <= 3 SS_T
 
+// Coefficients touching SS_T + 1 limbs at bw >= 32, with margin.
+template <int SS_T>
 struct SSShared {
-    uint64_t xlo[SS_MAXC];
-    uint32_t xhi[SS_MAXC];
+    static constexpr int MAXC = ((SS_T + 1) * 64 + 93) / 32 + 64;
+    uint64_t xlo[MAXC];
+    uint32_t xhi[MAXC];
     uint64_t ex[SS_T];
     uint32_t wg[SS_T / 64], wp[SS_T / 64], wc[SS_T / 64];
     uint32_t carry;
@@ -347,7 +348,8 @@ struct SSShared {
 // Carry into this thread's limb for the chunk: u (limb sum), g (generates),
 // p (propagates), cin (carry into the chunk); returns u + carry and updates
 // S.carry to the chunk's carry-out. Every thread of the block calls it.
-__device__ __forceinline__ uint64_t ss_lookahead(SSShared &S, uint64_t u, bool g, bool p) {
+template <int SS_T>
+__device__ __forceinline__ uint64_t ss_lookahead(SSShared<SS_T> &S, uint64_t u, bool g, bool p) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t G = __ballot(g), B = G | __ballot(p);
     if (lane == 0) {
@@ -372,7 +374,8 @@ __device__ __forceinline__ uint64_t ss_lookahead(SSShared &S, uint64_t u, bool g
 
 // Limbs [0, len) of the product whose residues are y ([3][L]) into out[c]
 // (limb c * 1024 + threadIdx.x).
-__device__ void ss_crt(SSShared &S, const uint32_t *y, uint32_t L, int bw, uint32_t conv, uint32_t len,
+template <int SS_T>
+__device__ void ss_crt(SSShared<SS_T> &S, const uint32_t *y, uint32_t L, int bw, uint32_t conv, uint32_t len,
                        const FFGarner &G, uint64_t out[SS_CH]) {
     const int t = threadIdx.x;
     if (threadIdx.x == 0) S.carry = 0, S.hi_last = 0;
@@ -408,7 +411,8 @@ __device__ void ss_crt(SSShared &S, const uint32_t *y, uint32_t L, int bw, uint3
 }
 
 // out = a + b' + cin0 over len limbs in registers (b' = b or ~b).
-__device__ void ss_add(SSShared &S, const uint64_t a[SS_CH], const uint64_t b[SS_CH], bool neg, uint32_t cin0,
+template <int SS_T>
+__device__ void ss_add(SSShared<SS_T> &S, const uint64_t a[SS_CH], const uint64_t b[SS_CH], bool neg, uint32_t cin0,
                        uint32_t len, uint64_t out[SS_CH]) {
     if (threadIdx.x == 0) S.carry = cin0;
     __syncthreads();
@@ -428,8 +432,9 @@ __device__ void ss_add(SSShared &S, const uint64_t a[SS_CH], const uint64_t b[SS
     }
 }
 
+template <int SS_T>
 __global__ __launch_bounds__(SS_T) void k_small_step(FFSmallStep a, FFGarner G) {
-    __shared__ SSShared S;
+    __shared__ SSShared<SS_T> S;
     const int t = threadIdx.x;
     if (t == 0) {
         atomicExch(a.keyK, 0ull);
@@ -478,7 +483,10 @@ __global__ __launch_bounds__(SS_T) void k_small_step(FFSmallStep a, FFGarner G) 
 extern "C" hipError_t ff_launch_small_step(const FFSmallStep *a, hipStream_t st) {
     garner_init();
     ff_prof_begin(FK_STEP, st);
-    hipLaunchKernelGGL(k_small_step, dim3(1), dim3(SS_T), 0, st, *a, g_garner);
+    // a quarter of the threads while every chunk fits 256 limbs (the early,
+    // smallest steps), where barriers over 16 wavefronts would dominate
+    if (a->len <= 3 * 256) hipLaunchKernelGGL(k_small_step<256>, dim3(1), dim3(256), 0, st, *a, g_garner);
+    else hipLaunchKernelGGL(k_small_step<1024>, dim3(1), dim3(1024), 0, st, *a, g_garner);
     ff_prof_end(FK_STEP, 12.0 * (a->conv1 + a->conv2) + 16.0 * a->len, st);
     return hipGetLastError();
 }

@@ -6,6 +6,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
+#include <tuple>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -552,8 +554,12 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
     // With a tuned or learned dispatcher and predicted lengths, every
     // product's schedule is picked up front in one call (one inference
     // launch): two squarings per step, then the final product.
+    // the picks depend only on n and the source: kept for repeated F(n)
+    static std::map<std::tuple<uint64_t, int, int>, std::vector<FFSched>> picked;
     std::vector<FFSched> ds;
-    if (predict && g_dispatch && !g_force_len && top > 0) {
+    auto memo = picked.find(std::make_tuple(n, g_dispatch, g_dispatch_gen));
+    if (predict && g_dispatch && !g_force_len && top > 0 && memo != picked.end()) ds = memo->second;
+    if (predict && g_dispatch && !g_force_len && top > 0 && ds.empty()) {
         std::vector<ProdShape> s0;
         std::vector<uint64_t> ba, bb;
         std::vector<uint8_t> sqf;
@@ -580,6 +586,7 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         const ProdShape sf = product_shape(bx, by);
         const uint8_t gen = 0;
         if (!sf.split) ff_pick_scheds(&sf, &bx, &by, &gen, 1, 1, g_dispatch, &ds[s0.size()]);
+        picked[std::make_tuple(n, g_dispatch, g_dispatch_gen)] = ds;
     }
     int step = 0;
     for (int i = top - 1; i >= 1; i--, step++) {

@@ -47,6 +47,8 @@ __host__ __device__ inline Layout layout(int K) {
 
 std::vector<float> g_params;  // host copy of the current parameters (empty: none)
 int g_K = 0;
+float *g_dparams = nullptr;   // device copy for inference (uploaded when stale)
+bool g_dstale = true;
 
 }  // namespace
 
@@ -137,7 +139,8 @@ __global__ void k_gather(const float *src, const int *idx, float *dst, int rows,
 // the K outputs (if Y) and the argmin over candidates allowed by mask bit k
 // (if choice).
 __global__ __launch_bounds__(256) void k_mlp_infer(const float *X, const float *P, int n, int K,
-                                                   const uint32_t *mask, float *Y, int *choice) {
+                                                   const uint32_t *mask, float *Y, int *choice, float margin,
+                                                   int dflt) {
     extern __shared__ float sp[];
     const Layout l = layout(K);
     for (size_t i = threadIdx.x; i < l.total; i += blockDim.x) sp[i] = P[i];
@@ -163,10 +166,13 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *X, const float *
         const float h2 = fmaxf(acc, 0.0f);
         for (int k = 0; k < K; k++) y[k] = fmaf(h2, sp[l.W3 + k * NH + o], y[k]);
     }
+    float ydef = INFINITY;
     for (int k = 0; k < K; k++) {
         if (Y) Y[(size_t)s * K + k] = y[k];
         if (((mk >> k) & 1) && y[k] < best) best = y[k], arg = k;
+        if (k == dflt) ydef = y[k];
     }
+    if (margin > 0 && dflt >= 0 && ((mk >> dflt) & 1) && ydef - best <= margin) arg = dflt;
     if (choice) choice[s] = arg;
 }
 
@@ -241,6 +247,8 @@ extern "C" int ff_mlp_set(int K, const float *params, size_t count) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     g_params.assign(params, params + count);
     g_K = K;
+    g_dstale = true;
+    g_dispatch_gen++;
     return FF_OK;
 }
 
@@ -330,21 +338,28 @@ extern "C" int ff_mlp_train(const float *x, const float *t, size_t n, int epochs
     }
     FF_CHECK(hipStreamSynchronize(g_st));
     FF_CHECK(hipMemcpy(g_params.data(), D.P, l.total * 4, hipMemcpyDeviceToHost));
+    g_dstale = true;
+    g_dispatch_gen++;
     if (loss_out) *loss_out = (float)last_loss;
     return FF_OK;
 }
 
 // Inference on device for n feature rows; y (n x K) and/or choice (argmin
-// over mask bits; mask NULL allows all).
-int ff_mlp_infer_dev(const float *dx, size_t n, const uint32_t *dmask, float *dy, int *dchoice) {
+// over mask bits; mask NULL allows all; with margin > 0 the candidate dflt
+// is kept unless another is predicted faster by more than margin).
+int ff_mlp_infer_dev(const float *dx, size_t n, const uint32_t *dmask, float *dy, int *dchoice, float margin,
+                     int dflt) {
     if (!g_K) return FF_ERR_ARG;
     const Layout l = layout(g_K);
-    float *dp = (float *)ff_dalloc(l.total * 4);
-    FF_CHECK(hipMemcpyAsync(dp, g_params.data(), l.total * 4, hipMemcpyHostToDevice, g_st));
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n)), dim3(256), l.total * 4, g_st, dx, dp, (int)n, g_K, dmask,
-                       dy, dchoice);
+    if (!g_dparams) g_dparams = (float *)ff_dalloc(layout(FF_MLP_MAXK).total * 4);
+    if (g_dstale) {
+        FF_CHECK(hipStreamSynchronize(g_st));
+        FF_CHECK(hipMemcpy(g_dparams, g_params.data(), l.total * 4, hipMemcpyHostToDevice));
+        g_dstale = false;
+    }
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n)), dim3(256), l.total * 4, g_st, dx, (const float *)g_dparams,
+                       (int)n, g_K, dmask, dy, dchoice, margin, dflt);
     FF_CHECK(hipGetLastError());
-    ff_dfree(dp);
     return FF_OK;
 }
 
@@ -361,7 +376,7 @@ extern "C" int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, fl
     uint32_t *dm = (uint32_t *)(dc + n);
     FF_CHECK(hipMemcpyAsync(dx, x, n * NI * 4, hipMemcpyHostToDevice, g_st));
     if (mask) FF_CHECK(hipMemcpyAsync(dm, mask, n * 4, hipMemcpyHostToDevice, g_st));
-    int rc = ff_mlp_infer_dev(dx, n, mask ? dm : nullptr, dy, dc);
+    int rc = ff_mlp_infer_dev(dx, n, mask ? dm : nullptr, dy, dc, 0.0f, -1);
     if (rc == FF_OK) {
         FF_CHECK(hipStreamSynchronize(g_st));
         if (y) FF_CHECK(hipMemcpy(y, dy, n * K * 4, hipMemcpyDeviceToHost));

@@ -13,7 +13,8 @@ extern int g_inner_log;
 extern int g_tile_words;
 extern uint64_t g_force_len;
 extern int g_max_lg;    // longest single transform 2^max_lg; longer products split
-extern int g_dispatch;  // schedule source of the drop-in products and F(n): 0, FF_BATCH_TUNED or FF_BATCH_LEARNED
+extern int g_dispatch;
+extern int g_dispatch_gen;  // bumped whenever the table or the network changes  // schedule source of the drop-in products and F(n): 0, FF_BATCH_TUNED or FF_BATCH_LEARNED
 
 void ff_set_err(const char *fmt, ...);
 void ff_ensure_init(void);
@@ -22,6 +23,8 @@ void ff_ensure_init(void);
 // order (all device work runs on g_st), hipMalloc only on a miss.
 void *ff_dalloc(size_t bytes);
 void ff_dfree(void *p);
+// Bytes the pool holds for reuse (free for the library, not for the driver).
+size_t ff_pool_cached(void);
 
 // Kernel timing is suspended while a graph is being captured (event
 // records and their readback do not belong in a replayed graph).

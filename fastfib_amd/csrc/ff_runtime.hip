@@ -25,6 +25,7 @@ int g_tile_words = 4096;
 uint64_t g_force_len = 0;
 int g_dispatch = 0;
 int g_max_lg = 25;
+int g_dispatch_gen = 0;
 
 void ff_set_err(const char *fmt, ...) {
     va_list ap;
@@ -45,6 +46,9 @@ extern "C" void ff_fail(const char *file, int line, const char *what, hipError_t
 
 static std::multimap<size_t, void *> g_free;
 static std::unordered_map<void *, size_t> g_sizes;
+static size_t g_cached = 0;  // bytes held in g_free
+
+size_t ff_pool_cached(void) { return g_cached; }
 
 static size_t size_class(size_t b) {
     if (b < 4096) return 4096;
@@ -59,6 +63,7 @@ void *ff_dalloc(size_t bytes) {
     auto it = g_free.lower_bound(sz);
     if (it != g_free.end() && it->first <= sz + sz / 2) {
         void *p = it->second;
+        g_cached -= it->first;
         g_free.erase(it);
         return p;
     }
@@ -72,6 +77,7 @@ void *ff_dalloc(size_t bytes) {
             g_sizes.erase(kv.second);
         }
         g_free.clear();
+        g_cached = 0;
         (void)hipGetLastError();
         e = hipMalloc(&p, sz);
         if (e != hipSuccess) {
@@ -88,6 +94,7 @@ void ff_dfree(void *p) {
     auto it = g_sizes.find(p);
     if (it == g_sizes.end()) return;
     g_free.emplace(it->second, p);
+    g_cached += it->second;
 }
 
 static uint64_t *g_res_h = nullptr;

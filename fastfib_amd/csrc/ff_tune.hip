@@ -52,17 +52,20 @@ int table_class(const ProdShape &s, bool sq, size_t count) {
     return (((lg - 4) * 2 + s.r3) * 2 + (sq ? 1 : 0)) * T_CNT + lg_count(count);
 }
 
-// Free HBM fraction, queried at most once a second (the query goes to the
-// driver and costs more than an inference).
+// HBM available to the library as a fraction of the whole: free memory
+// plus the blocks its pool holds for reuse. The driver query is made at
+// most once a second (it costs more than an inference).
 double free_hbm_fraction() {
-    static double cached = -1;
+    static size_t fr = 0, tot = 0;
     static std::chrono::steady_clock::time_point when;
     const auto now = std::chrono::steady_clock::now();
-    if (cached >= 0 && now - when < std::chrono::seconds(1)) return cached;
-    size_t fr = 0, tot = 0;
-    cached = (hipMemGetInfo(&fr, &tot) != hipSuccess || tot == 0) ? 1.0 : (double)fr / (double)tot;
-    when = now;
-    return cached;
+    if (tot == 0 || now - when >= std::chrono::seconds(1)) {
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0, tot = 0;
+        when = now;
+    }
+    if (tot == 0) return 1.0;
+    const double f = (double)(fr + ff_pool_cached()) / (double)tot;
+    return f < 1.0 ? f : 1.0;
 }
 
 void features(uint64_t bits_a, uint64_t bits_b, bool sq, size_t count, const ProdShape &s, double hbm, float *x) {
@@ -114,6 +117,7 @@ int load_file(const char *path) {
         return FF_ERR_ARG;
     }
     g_table = table;
+    g_dispatch_gen++;
     if (np && ff_mlp_set(K, params.data(), params.size()) != FF_OK) return FF_ERR_ARG;
     return FF_OK;
 }
@@ -150,16 +154,24 @@ void ff_pick_scheds(const ProdShape *s0, const uint64_t *bits_a, const uint64_t 
             features(bits_a[i], bits_b[i], square[i], count, s0[i], hbm, &x[i * FF_MLP_IN]);
             mask[i] = cand_mask(s0[i]);
         }
-        char *buf = (char *)ff_dalloc(n * (FF_MLP_IN + 2) * 4 + 512);
+        // features, masks and choices staged through pinned memory
+        const size_t words = n * (FF_MLP_IN + 2);
+        uint64_t *h = ff_pinned_results((words + 1) / 2);
+        float *hx = (float *)h;
+        uint32_t *hm = (uint32_t *)(hx + n * FF_MLP_IN);
+        memcpy(hx, x.data(), x.size() * 4);
+        memcpy(hm, mask.data(), n * 4);
+        char *buf = (char *)ff_dalloc(words * 4 + 512);
         float *dx = (float *)buf;
         uint32_t *dm = (uint32_t *)(dx + n * FF_MLP_IN);
         int *dc = (int *)(dm + n);
-        FF_CHECK(hipMemcpyAsync(dx, x.data(), x.size() * 4, hipMemcpyHostToDevice, g_st));
-        FF_CHECK(hipMemcpyAsync(dm, mask.data(), n * 4, hipMemcpyHostToDevice, g_st));
+        FF_CHECK(hipMemcpyAsync(dx, hx, (n * FF_MLP_IN + n) * 4, hipMemcpyHostToDevice, g_st));
         std::vector<int> choice(n, K_DEFAULT);
-        if (ff_mlp_infer_dev(dx, n, dm, nullptr, dc) == FF_OK) {
-            FF_CHECK(hipMemcpyAsync(choice.data(), dc, n * 4, hipMemcpyDeviceToHost, g_st));
+        if (ff_mlp_infer_dev(dx, n, dm, nullptr, dc, FF_MLP_MARGIN, K_DEFAULT) == FF_OK) {
+            int *hc = (int *)(hm + n);
+            FF_CHECK(hipMemcpyAsync(hc, dc, n * 4, hipMemcpyDeviceToHost, g_st));
             FF_CHECK(hipStreamSynchronize(g_st));
+            memcpy(choice.data(), hc, n * 4);
         }
         ff_dfree(buf);
         for (size_t i = 0; i < n; i++) out[i] = cand_sched(choice[i] >= 0 ? choice[i] : K_DEFAULT);
@@ -214,6 +226,7 @@ extern "C" int ff_tune_set_table(const int32_t *table, size_t n) {
     if (n != (size_t)T_SIZE || !table) return FF_ERR_ARG;
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     g_table.assign(table, table + n);
+    g_dispatch_gen++;
     return FF_OK;
 }
 

@@ -156,8 +156,10 @@ def init_params(K: int, X: np.ndarray, seed: int = 0) -> np.ndarray:
     """Normalization from the data, He-uniform weights, zero biases."""
     rng = np.random.default_rng(seed)
     mu = X.mean(0)
-    sd = X.std(0)
-    sd = np.where(sd > 1e-6, sd, 1.0)
+    # a feature that barely varies in the samples (free HBM) must not blow
+    # up when it does vary in use: at least 0.1 (features are logs, flags
+    # and fractions)
+    sd = np.maximum(X.std(0), 0.1)
     parts = [mu, sd]
     for fan_in, shape in ((NI, (64, NI)), (None, (64,)), (64, (64, 64)), (None, (64,)), (64, (K, 64)),
                           (None, (K,))):
@@ -221,6 +223,21 @@ def build_table(d: dict, size: int, K: int, default: int) -> np.ndarray:
             continue
         table[c] = int(np.argmin(np.where(np.isfinite(m), m, np.inf)))
     return table
+
+
+MARGIN = 0.02  # the library's FF_MLP_MARGIN
+
+
+def dispatch_choice(y: np.ndarray, mask: np.ndarray, default: int, margin: float = MARGIN) -> np.ndarray:
+    """The dispatcher's pick from the network's outputs: the argmin over the
+    applicable candidates, unless the default is predicted within margin of
+    it (the rule of the library's inference kernel)."""
+    K = y.shape[1]
+    allowed = ((np.asarray(mask, dtype=np.int64)[:, None] >> np.arange(K)) & 1) == 1
+    yy = np.where(allowed, y, np.inf)
+    best = np.argmin(yy, axis=1)
+    gain = yy[np.arange(len(yy)), default] - yy[np.arange(len(yy)), best]
+    return np.where(allowed[:, default] & (gain <= margin), default, best)
 
 
 def table_choice(table: np.ndarray, d: dict) -> np.ndarray:
@@ -314,7 +331,8 @@ def main(argv=None) -> int:
     K, params = get_params()
     size = int(_L().ff_tune_table_size())
     table = build_table(tr, size, K, default_candidate())
-    _, choice = predict(ho["X"], ho["mask"])
+    y, _ = predict(ho["X"], ho["mask"])
+    choice = dispatch_choice(y, ho["mask"], default_candidate())
     rep = {"train_samples": int(tr["X"].shape[0]), "heldout_samples": int(ho["X"].shape[0]),
            "train_mixes": int(np.unique(tr["mix"]).size), "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
            "train_seconds": round(dt, 2), "final_loss": loss,

@@ -107,16 +107,30 @@ __global__ __launch_bounds__(256) void k_add_scan(FFAddNodes J) {
     }
 }
 
-// Adds each block's carry-in, the ripple bounded by the block.
-__global__ __launch_bounds__(256) void k_add_apply(FFAddNodes J, uint32_t total) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total || !J.bcin[t]) return;
-    const uint32_t node = t / J.nblk, blk = t - node * J.nblk;
+// Adds each block's carry-in, one thread per limb: limb k of the block
+// takes the carry when the limbs below it in the block are all ones (a
+// subtraction of nearly equal numbers ripples through whole blocks, so a
+// sequential ripple would be slow); wavefront ballots find the first limb
+// that stops it.
+__global__ __launch_bounds__(ADD_T) void k_add_apply(FFAddNodes J) {
+    const uint32_t b = blockIdx.x;
+    if (!J.bcin[b]) return;  // uniform over the block
+    __shared__ uint32_t all1[ADD_T / 64];
+    const uint32_t node = b / J.nblk, blk = b - node * J.nblk;
     uint64_t *o = J.out + node * J.so;
-    const uint64_t i0 = (uint64_t)blk * ADD_T, i1 = i0 + ADD_T < J.n ? i0 + ADD_T : J.n;
-    for (uint64_t i = i0; i < i1; i++) {
-        if (++o[i] != 0) break;
-    }
+    const uint64_t i = (uint64_t)blk * ADD_T + threadIdx.x;
+    const bool live = i < J.n;
+    const uint64_t v = live ? o[i] : 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t ones = __ballot(live && v == ~0ull);
+    if (lane == 0) all1[w] = ones == ~0ull;
+    __syncthreads();
+    bool cw = true;  // the carry reaches this wave
+    for (int k = 0; k < w; k++) cw = cw && all1[k];
+    // lanes up to the first that is not all ones take the carry
+    const uint64_t below = lane ? (ones & ((1ull << lane) - 1)) : 0;
+    const bool take = cw && below == ((lane ? (1ull << lane) - 1 : 0));
+    if (live && take) o[i] = v + 1;
 }
 
 size_t ff_add_nodes_scratch(uint32_t N, uint64_t n) {
@@ -134,8 +148,7 @@ hipError_t ff_add_nodes(FFAddNodes J, void *scratch, hipStream_t st) {
     hipLaunchKernelGGL(k_add_local, dim3((unsigned)blocks), dim3(ADD_T), 0, st, J);
     if (J.nblk > 1) {
         hipLaunchKernelGGL(k_add_scan, dim3(J.N), dim3(256), 0, st, J);
-        hipLaunchKernelGGL(k_add_apply, dim3((unsigned)((blocks + 255) / 256)), dim3(256), 0, st, J,
-                           (uint32_t)blocks);
+        hipLaunchKernelGGL(k_add_apply, dim3((unsigned)blocks), dim3(ADD_T), 0, st, J);
     }
     ff_prof_end(FK_LINEAR, 8.0 * J.N * (3.0 * J.n), st);
     return hipGetLastError();

@@ -14,44 +14,87 @@
 #pragma once
 #include "ff_arith.h"
 
+// Twiddles of one group and level: the butterflies of level b = b_lo + lb
+// use rt[2^b + (low | k << b_lo)] = w^low * w_(2^(lb+1))^k, k < 2^lb, with
+// w = w_(2^(b+1)). Only the base w^low is read from the table; the others
+// are products with the 16th roots w_16^m = rt[8 + m] (companions at
+// rtp[8 + m]), the same for every group of the prime: one gathered load
+// per level instead of 2^lb, at the price of cheap multiplies.
+template <int LB>
+__device__ __forceinline__ void ff_group_twiddles(uint32_t tw[1 << LB], uint32_t twp[1 << LB], int h, int low,
+                                                  const uint32_t *rt, const uint32_t *rtp, uint32_t p,
+                                                  uint32_t pinv) {
+    tw[0] = rt[h + low];
+    twp[0] = rtp[h + low];
+    // the 16th roots sit at the same address for every lane: read through
+    // the constant address space, so they arrive by scalar loads
+    typedef const __attribute__((address_space(4))) uint32_t *cptr;
+    const cptr crt = (cptr)rt, crtp = (cptr)rtp;
+#pragma unroll
+    for (int k = 1; k < (1 << LB); k++) {
+        const int m = k << (3 - LB);  // w_(2^(LB+1))^k = w_16^(k 2^(3-LB))
+        tw[k] = ff_mont_w(tw[0], crt[8 + m], crtp[8 + m], p);
+        twp[k] = tw[k] * pinv;
+    }
+}
+
 template <int NL>
 __device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, const uint32_t *rt, const uint32_t *rtp,
-                                             uint32_t p) {
+                                             uint32_t p, uint32_t pinv) {
 #pragma unroll
     for (int lb = NL - 1; lb >= 0; lb--) {
         const int b = b_lo + lb;
         const int h = 1 << b;
+        if (b == 0) {
+#pragma unroll
+            for (int j = 0; j < (1 << NL); j++) {
+                if (j & (1 << lb)) continue;
+                const int k = j | (1 << lb);
+                const uint32_t x = v[j], y = v[k];
+                v[j] = ff_add(x, y, p);
+                v[k] = ff_sub(x, y, p);
+            }
+            continue;
+        }
+        uint32_t tw[8], twp[8];
+        if (lb == 0) ff_group_twiddles<0>(tw, twp, h, low, rt, rtp, p, pinv);
+        else if (lb == 1) ff_group_twiddles<1>(tw, twp, h, low, rt, rtp, p, pinv);
+        else if (lb == 2) ff_group_twiddles<2>(tw, twp, h, low, rt, rtp, p, pinv);
+        else ff_group_twiddles<3>(tw, twp, h, low, rt, rtp, p, pinv);
 #pragma unroll
         for (int j = 0; j < (1 << NL); j++) {
             if (j & (1 << lb)) continue;
             const int k = j | (1 << lb);
-            uint32_t x = v[j], y = v[k];
+            const uint32_t x = v[j], y = v[k];
+            const int q = j & ((1 << lb) - 1);
             v[j] = ff_add(x, y, p);
-            if (b == 0) {
-                v[k] = ff_sub(x, y, p);
-            } else {
-                const int t = low | ((j & ((1 << lb) - 1)) << b_lo);
-                v[k] = ff_mont_w(x - y + p, rt[h + t], rtp[h + t], p);
-            }
+            v[k] = ff_mont_w(x - y + p, tw[q], twp[q], p);
         }
     }
 }
 
 template <int NL>
 __device__ __forceinline__ void ff_dit_group(uint32_t *v, int low, int b_lo, const uint32_t *irt, const uint32_t *irtp,
-                                             uint32_t p) {
+                                             uint32_t p, uint32_t pinv) {
 #pragma unroll
     for (int lb = 0; lb < NL; lb++) {
         const int b = b_lo + lb;
         const int h = 1 << b;
+        uint32_t tw[8], twp[8];
+        if (b != 0) {
+            if (lb == 0) ff_group_twiddles<0>(tw, twp, h, low, irt, irtp, p, pinv);
+            else if (lb == 1) ff_group_twiddles<1>(tw, twp, h, low, irt, irtp, p, pinv);
+            else if (lb == 2) ff_group_twiddles<2>(tw, twp, h, low, irt, irtp, p, pinv);
+            else ff_group_twiddles<3>(tw, twp, h, low, irt, irtp, p, pinv);
+        }
 #pragma unroll
         for (int j = 0; j < (1 << NL); j++) {
             if (j & (1 << lb)) continue;
             const int k = j | (1 << lb);
             uint32_t x = v[j], y = v[k];
             if (b != 0) {
-                const int t = low | ((j & ((1 << lb) - 1)) << b_lo);
-                y = ff_mont_w(y, irt[h + t], irtp[h + t], p);
+                const int q = j & ((1 << lb) - 1);
+                y = ff_mont_w(y, tw[q], twp[q], p);
             }
             v[j] = ff_add(x, y, p);
             v[k] = ff_sub(x, y, p);
@@ -76,29 +119,6 @@ __device__ __forceinline__ int ff_pos_addr(const FFTile &t, int pos, int c) {
 
 __device__ __forceinline__ int ff_tile_addr(const FFTile &t, int sub, int row, int c) {
     return ff_pos_addr(t, (sub << t.rho) + row, c);
-}
-
-// One round of NL levels over the whole tile (all threads of the block).
-template <int NL, bool INV>
-__device__ void ff_round(const FFTile &t, int b_lo, const uint32_t *rt, const uint32_t *rtp, uint32_t p) {
-    const int G = 1 << NL;
-    const int per_sub = (1 << (t.rho - NL)) << t.logC;  // groups per sub-transform
-    const int ngroups = per_sub * t.nsub;
-    for (int gi = threadIdx.x; gi < ngroups; gi += blockDim.x) {
-        const int sub = gi / per_sub;
-        const int g = gi - sub * per_sub;
-        const int c = g & (t.C - 1);
-        const int rest = g >> t.logC;
-        const int low = rest & ((1 << b_lo) - 1);
-        const int base = low | ((rest >> b_lo) << (b_lo + NL));
-        uint32_t v[G];
-#pragma unroll
-        for (int j = 0; j < G; j++) v[j] = t.s[ff_tile_addr(t, sub, base | (j << b_lo), c)];
-        if (INV) ff_dit_group<NL>(v, low, b_lo, rt, rtp, p);
-        else ff_dif_group<NL>(v, low, b_lo, rt, rtp, p);
-#pragma unroll
-        for (int j = 0; j < G; j++) t.s[ff_tile_addr(t, sub, base | (j << b_lo), c)] = v[j];
-    }
 }
 
 // A round whose groups may come from a loader instead of LDS and go to a
@@ -132,17 +152,17 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
             v[j] = FROM_G ? ld(pos, c) : t.s[ff_pos_addr(t, pos, c)];
         }
         if (KIND == FF_R_DIF) {
-            ff_dif_group<NL>(v, low, b_lo, rt, rtp, p);
+            ff_dif_group<NL>(v, low, b_lo, rt, rtp, p, pinv);
         } else if (KIND == FF_R_DIT) {
-            ff_dit_group<NL>(v, low, b_lo, irt, irtp, p);
+            ff_dit_group<NL>(v, low, b_lo, irt, irtp, p, pinv);
         } else {
-            ff_dif_group<NL>(v, low, b_lo, rt, rtp, p);
+            ff_dif_group<NL>(v, low, b_lo, rt, rtp, p, pinv);
 #pragma unroll
             for (int j = 0; j < G; j++) {
                 const int pos = pos0 + (base | (j << b_lo));
                 v[j] = ff_mont(v[j], B ? B[ff_pos_addr(t, pos, c)] : v[j], p, pinv);
             }
-            ff_dit_group<NL>(v, low, b_lo, irt, irtp, p);
+            ff_dit_group<NL>(v, low, b_lo, irt, irtp, p, pinv);
         }
 #pragma unroll
         for (int j = 0; j < G; j++) {
@@ -257,43 +277,6 @@ __device__ void ff_conv_io(const FFTile &t, const uint32_t *rt, const uint32_t *
     }
 }
 
-// All DIF levels (forward) or DIT levels (inverse) of the tile's columns.
-template <bool INV>
-__device__ void ff_tile_transform(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, uint32_t p) {
-    if (t.rho == 0) return;
-    if (!INV) {
-        // DIF: top bits first, 4 levels per round, the remainder last
-        int b = t.rho;
-        while (b > 0) {
-            const int nl = b >= 4 ? 4 : b;
-            b -= nl;
-            __syncthreads();
-            switch (nl) {
-            case 4: ff_round<4, false>(t, b, rt, rtp, p); break;
-            case 3: ff_round<3, false>(t, b, rt, rtp, p); break;
-            case 2: ff_round<2, false>(t, b, rt, rtp, p); break;
-            default: ff_round<1, false>(t, b, rt, rtp, p); break;
-            }
-        }
-    } else {
-        // DIT: bottom bits first, mirror of the DIF rounds
-        int b = 0;
-        const int first = t.rho % 4 == 0 ? 4 : t.rho % 4;
-        int nl = first;
-        while (b < t.rho) {
-            __syncthreads();
-            switch (nl) {
-            case 4: ff_round<4, true>(t, b, rt, rtp, p); break;
-            case 3: ff_round<3, true>(t, b, rt, rtp, p); break;
-            case 2: ff_round<2, true>(t, b, rt, rtp, p); break;
-            default: ff_round<1, true>(t, b, rt, rtp, p); break;
-            }
-            b += nl;
-            nl = 4;
-        }
-    }
-    __syncthreads();
-}
 
 // Radix-3 layer over a tile of nsub = 3 stacked thirds (ntt_radix3.c:8-22,
 // reference): forward, for n1 < M with a_r = (third r, row n1):

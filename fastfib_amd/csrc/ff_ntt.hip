@@ -119,6 +119,43 @@ extern "C" int ff_tables_init(void) {
 
 #define FF_THREADS 256
 
+// Four-step twiddles of an outer pass: element (row position pos, tile
+// column c), column col = c0 + c, is scaled by w^(col * f(pos) * rootstep),
+// f the row's frequency (ff_row_freq), w the 3 * 2^25-th root; INV: by the
+// inverse root, times the CRT scale on the top pass. group() serves the 2^NL
+// consecutive rows of one butterfly group: their frequencies are
+// f0 + step * rev(j), so the twiddles are base * g^rev(j) from two table
+// roots (base = w^(col f0 rootstep), g = w^(col step rootstep)).
+template <bool INV>
+struct FFOuterTw {
+    static constexpr bool on = true;
+    const FFTables *T;
+    int pi, rho, r3;
+    bool top;
+    uint32_t c0, rootstep, scm, p, pinv;
+
+    __device__ __forceinline__ uint32_t root(uint32_t e) const {
+        if (INV) e = e ? FF_NBASE - e : 0;
+        return ff_root(T, pi, e, p, pinv);
+    }
+    __device__ __forceinline__ uint32_t one(int pos, int c) const {
+        uint32_t tw = root((c0 + c) * ff_row_freq(pos, rho, r3) * rootstep);
+        if (INV && top) tw = ff_mont(tw, scm, p, pinv);
+        return tw;
+    }
+    template <int NL>
+    __device__ __forceinline__ void group(int pos, int c, uint32_t *tw) const {
+        const uint32_t cs = (c0 + c) * rootstep;
+        const uint32_t step = (r3 ? 3u : 1u) << (rho - NL);
+        uint32_t base = root(cs * ff_row_freq(pos, rho, r3));
+        if (INV && top) base = ff_mont(base, scm, p, pinv);
+        uint32_t pw[1 << NL];
+        ff_geo<NL>(base, root(cs * step), pw, p, pinv);
+#pragma unroll
+        for (int j = 0; j < (1 << NL); j++) tw[j] = pw[ff_rev_c(j, NL)];
+    }
+};
+
 // Outer pass over one tile of R rows x C columns (tile index `tile` among
 // the pass's nblocks * tpb) of array X, prime pi.
 template <bool INV, bool TOP, bool R3>
@@ -145,26 +182,17 @@ __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, i
         const size_t g = (size_t)pos * Sb + c0 + c;
         return top ? ff_load_coef(src, nw, g, bw, P) : data[g];
     };
-    auto store_fwd = [&](int pos, int c, uint32_t v) {
-        const uint32_t col = c0 + c;
-        const uint32_t e = col * ff_row_freq(pos, rho, r3) * rootstep;
-        data[(size_t)pos * Sb + col] = e ? ff_mont(v, ff_root(T, pi, e, p, pinv), p, pinv) : v;
-    };
-    auto load_inv = [&](int pos, int c) -> uint32_t {
-        const uint32_t col = c0 + c;
-        uint32_t e = col * ff_row_freq(pos, rho, r3) * rootstep;
-        e = e ? FF_NBASE - e : 0;
-        uint32_t tw = ff_root(T, pi, e, p, pinv);
-        if (top) tw = ff_mont(tw, scm, p, pinv);
-        return ff_mont(data[(size_t)pos * Sb + col], tw, p, pinv);
-    };
+    // the twiddles are applied by the rounds (tw), not by these accessors
+    auto store_fwd = [&](int pos, int c, uint32_t v) { data[(size_t)pos * Sb + c0 + c] = v; };
+    auto load_inv = [&](int pos, int c) -> uint32_t { return data[(size_t)pos * Sb + c0 + c]; };
+    const FFOuterTw<INV> tw{T, pi, rho, r3, top, c0, rootstep, scm, p, pinv};
     auto store_inv = [&](int pos, int c, uint32_t v) { data[(size_t)pos * Sb + c0 + c] = v; };
     auto lds_load = [&](int pos, int c) -> uint32_t { return lds[ff_pos_addr(t, pos, c)]; };
     auto lds_store = [&](int pos, int c, uint32_t v) { lds[ff_pos_addr(t, pos, c)] = v; };
 
     if (!INV) {
         if (!R3 && !TOP) {
-            ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, true, true, load_fwd, store_fwd);
+            ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, true, true, load_fwd, store_fwd, tw);
         } else {
             // limbs are unpacked element by element into the LDS tile (16
             // unpacks in flight per thread would not fit in registers), and
@@ -173,13 +201,13 @@ __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, i
                                                                         load_fwd(i >> ps.logC, i & (ps.C - 1)));
             if (R3) ff_radix3_layer<false>(t, T, pi, P, ps.rstep3);
             else __syncthreads();
-            ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, false, true, lds_load, store_fwd);
+            ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, false, true, lds_load, store_fwd, tw);
         }
     } else {
         if (!R3) {
-            ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, true, load_inv, store_inv);
+            ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, true, load_inv, store_inv, tw);
         } else {
-            ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, false, load_inv, lds_store);
+            ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, false, load_inv, lds_store, tw);
             ff_radix3_layer<true>(t, T, pi, P, ps.rstep3);
             for (int i = threadIdx.x; i < n; i += blockDim.x) store_inv(i >> ps.logC, i & (ps.C - 1),
                                                                         lds_load(i >> ps.logC, i & (ps.C - 1)));

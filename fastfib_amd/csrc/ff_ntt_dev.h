@@ -130,10 +130,42 @@ __device__ __forceinline__ int ff_tile_addr(const FFTile &t, int sub, int row, i
 // round, which cover the same row bits and therefore the same groups.
 enum { FF_R_DIF = 0, FF_R_DIT = 1, FF_R_MID = 2 };
 
-template <int NL, int KIND, bool FROM_G, bool TO_G, class Ld, class St>
+// Twiddles of the outer four-step passes, applied in the round that touches
+// HBM (the forward's last, the inverse's first). Those rounds cover the
+// lowest row bits, so a group holds 2^NL consecutive rows, whose twiddles
+// are base * g^rev(j): the provider computes them from two table roots per
+// group instead of one per element. FFNoTw: no twiddles (inner blocks).
+struct FFNoTw {
+    static constexpr bool on = false;
+    template <int NL>
+    __device__ void group(int, int, uint32_t *) const {}
+    __device__ uint32_t one(int, int) const { return 0; }
+};
+
+// nl-bit reversal of j, folded at compile time in unrolled loops.
+__host__ __device__ constexpr int ff_rev_c(int j, int nl) {
+    return nl == 0 ? 0 : ((j & 1) << (nl - 1)) | ff_rev_c(j >> 1, nl - 1);
+}
+
+// T[k] = base * g^k for k < 2^NL (Montgomery form), in a tree of depth NL.
+template <int NL>
+__device__ __forceinline__ void ff_geo(uint32_t base, uint32_t g, uint32_t *T, uint32_t p, uint32_t pinv) {
+    T[0] = base;
+    uint32_t gk = g;  // g^(2^i)
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        const int h = 1 << i;
+        const uint32_t gp = gk * pinv;
+#pragma unroll
+        for (int k = 0; k < h; k++) T[h + k] = ff_mont_w(T[k], gk, gp, p);
+        if (i + 1 < NL) gk = ff_mont(gk, gk, p, pinv);
+    }
+}
+
+template <int NL, int KIND, bool FROM_G, bool TO_G, class Ld, class St, class Gt = FFNoTw>
 __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uint32_t *rt, const uint32_t *rtp,
                                             const uint32_t *irt, const uint32_t *irtp, const uint32_t *B,
-                                            uint32_t p, uint32_t pinv, Ld ld, St st) {
+                                            uint32_t p, uint32_t pinv, Ld ld, St st, const Gt &gt = Gt()) {
     const int G = 1 << NL;
     const int per_sub = (1 << (t.rho - NL)) << t.logC;
     const int ngroups = per_sub * t.nsub;
@@ -151,6 +183,13 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
             const int pos = pos0 + (base | (j << b_lo));
             v[j] = FROM_G ? ld(pos, c) : t.s[ff_pos_addr(t, pos, c)];
         }
+        if constexpr (Gt::on && KIND == FF_R_DIT && FROM_G) {
+            // b_lo = 0 here: rows pos0 + base + j
+            uint32_t tw[G];
+            gt.template group<NL>(pos0 + base, c, tw);
+#pragma unroll
+            for (int j = 0; j < G; j++) v[j] = ff_mont(v[j], tw[j], p, pinv);
+        }
         if (KIND == FF_R_DIF) {
             ff_dif_group<NL>(v, low, b_lo, rt, rtp, p, pinv);
         } else if (KIND == FF_R_DIT) {
@@ -164,6 +203,12 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
             }
             ff_dit_group<NL>(v, low, b_lo, irt, irtp, p, pinv);
         }
+        if constexpr (Gt::on && KIND == FF_R_DIF && TO_G) {
+            uint32_t tw[G];
+            gt.template group<NL>(pos0 + base, c, tw);
+#pragma unroll
+            for (int j = 0; j < G; j++) v[j] = ff_mont(v[j], tw[j], p, pinv);
+        }
 #pragma unroll
         for (int j = 0; j < G; j++) {
             const int pos = pos0 + (base | (j << b_lo));
@@ -174,16 +219,16 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
 }
 
 // Dispatch of one round by its level count and ends.
-template <int KIND, class Ld, class St>
+template <int KIND, class Ld, class St, class Gt = FFNoTw>
 __device__ void ff_round_any(int nl, bool from_g, bool to_g, const FFTile &t, int b_lo, const uint32_t *rt,
                              const uint32_t *rtp, const uint32_t *irt, const uint32_t *irtp, const uint32_t *B,
-                             uint32_t p, uint32_t pinv, Ld ld, St st) {
-#define FF_RCASE(N)                                                                                              \
-    case N:                                                                                                      \
-        if (from_g && to_g) ff_round_io<N, KIND, true, true>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st);   \
-        else if (from_g) ff_round_io<N, KIND, true, false>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st);     \
-        else if (to_g) ff_round_io<N, KIND, false, true>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st);       \
-        else ff_round_io<N, KIND, false, false>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st);                \
+                             uint32_t p, uint32_t pinv, Ld ld, St st, const Gt &gt = Gt()) {
+#define FF_RCASE(N)                                                                                                \
+    case N:                                                                                                        \
+        if (from_g && to_g) ff_round_io<N, KIND, true, true>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st, gt); \
+        else if (from_g) ff_round_io<N, KIND, true, false>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st, gt);   \
+        else if (to_g) ff_round_io<N, KIND, false, true>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st, gt);     \
+        else ff_round_io<N, KIND, false, false>(t, b_lo, rt, rtp, irt, irtp, B, p, pinv, ld, st, gt);              \
         break;
     switch (nl) {
         FF_RCASE(4)
@@ -203,22 +248,26 @@ __device__ __forceinline__ int ff_last_nl(int rho) { return rho % 4 == 0 ? 4 : r
 // from_g, the last writes through st when to_g.
 // A tile of one-row sub-transforms (rho = 0, under a radix-3 layer) has no
 // levels: its elements only pass through.
-template <class Ld, class St>
-__device__ void ff_pass_through(const FFTile &t, bool from_g, bool to_g, Ld ld, St st) {
+template <class Ld, class St, class Gt = FFNoTw>
+__device__ void ff_pass_through(const FFTile &t, bool from_g, bool to_g, Ld ld, St st, const Gt &gt = Gt(),
+                                uint32_t p = 0, uint32_t pinv = 0) {
     const int n = t.nsub << t.logC;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int pos = i >> t.logC, c = i & (t.C - 1);
-        const uint32_t v = from_g ? ld(pos, c) : t.s[ff_pos_addr(t, pos, c)];
+        uint32_t v = from_g ? ld(pos, c) : t.s[ff_pos_addr(t, pos, c)];
+        if constexpr (Gt::on) {
+            if (from_g || to_g) v = ff_mont(v, gt.one(pos, c), p, pinv);
+        }
         if (to_g) st(pos, c, v);
         else t.s[ff_pos_addr(t, pos, c)] = v;
     }
 }
 
-template <class Ld, class St>
+template <class Ld, class St, class Gt = FFNoTw>
 __device__ void ff_forward_io(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, uint32_t p, uint32_t pinv,
-                              bool from_g, bool to_g, Ld ld, St st) {
+                              bool from_g, bool to_g, Ld ld, St st, const Gt &gt = Gt()) {
     if (t.rho == 0) {
-        ff_pass_through(t, from_g, to_g, ld, st);
+        ff_pass_through(t, from_g, to_g, ld, st, gt, p, pinv);
         return;
     }
     const int R = ff_nrounds(t.rho);
@@ -228,16 +277,16 @@ __device__ void ff_forward_io(const FFTile &t, const uint32_t *rt, const uint32_
         b -= nl;
         if (r) __syncthreads();
         ff_round_any<FF_R_DIF>(nl, from_g && r == 0, to_g && r == R - 1, t, b, rt, rtp, nullptr, nullptr, nullptr,
-                               p, pinv, ld, st);
+                               p, pinv, ld, st, gt);
     }
 }
 
 // Inverse (DIT) over the tile, mirror of ff_forward_io.
-template <class Ld, class St>
+template <class Ld, class St, class Gt = FFNoTw>
 __device__ void ff_inverse_io(const FFTile &t, const uint32_t *irt, const uint32_t *irtp, uint32_t p, uint32_t pinv,
-                              bool from_g, bool to_g, Ld ld, St st) {
+                              bool from_g, bool to_g, Ld ld, St st, const Gt &gt = Gt()) {
     if (t.rho == 0) {
-        ff_pass_through(t, from_g, to_g, ld, st);
+        ff_pass_through(t, from_g, to_g, ld, st, gt, p, pinv);
         return;
     }
     const int R = ff_nrounds(t.rho);
@@ -246,7 +295,7 @@ __device__ void ff_inverse_io(const FFTile &t, const uint32_t *irt, const uint32
         const int nl = r == 0 ? ff_last_nl(t.rho) : 4;
         if (r) __syncthreads();
         ff_round_any<FF_R_DIT>(nl, from_g && r == 0, to_g && r == R - 1, t, b, nullptr, nullptr, irt, irtp, nullptr,
-                               p, pinv, ld, st);
+                               p, pinv, ld, st, gt);
         b += nl;
     }
 }

@@ -26,16 +26,8 @@
 // (crt.c:326-336) with a parallel carry-lookahead over all limbs.
 #include <stdint.h>
 
+#include "ff_crt_dev.h"
 #include "ff_internal.h"
-
-typedef unsigned __int128 u128;
-
-struct FFGarner {
-    uint32_t p1, p2, p3;
-    uint32_t pinv2, pinv3;
-    uint32_t i12, i12p, i13, i13p, i23, i23p;  // pi^-1 mod pj, Montgomery form, and * pinv
-    uint64_t p12;
-};
 
 static FFGarner g_garner;
 static int g_garner_ready = 0;
@@ -68,18 +60,6 @@ static void garner_init(void) {
     g_garner_ready = 1;
 }
 
-// X = x1 + x2 p1 + x3 p1 p2 from canonical residues (crt.c:180-206).
-__device__ __forceinline__ u128 ff_garner(uint32_t r1, uint32_t r2, uint32_t r3, const FFGarner &G) {
-    const uint32_t x1 = r1;
-    const uint32_t x1m2 = x1 >= G.p2 ? x1 - G.p2 : x1;  // p1 < 2 p2
-    const uint32_t x2 = ff_mont_w(r2 - x1m2 + G.p2, G.i12, G.i12p, G.p2);
-    const uint32_t x1m3 = x1 >= G.p3 ? x1 - G.p3 : x1;  // p1 < 2 p3
-    const uint32_t t3 = ff_mont_w(r3 - x1m3 + G.p3, G.i13, G.i13p, G.p3);
-    const uint32_t x2m3 = x2 >= G.p3 ? x2 - G.p3 : x2;  // p2 < 2 p3
-    const uint32_t x3 = ff_mont_w(t3 - x2m3 + G.p3, G.i23, G.i23p, G.p3);
-    const uint64_t lo = (uint64_t)x2 * G.p1 + x1;  // < 2^62
-    return (u128)x3 * G.p12 + lo;
-}
 
 #define CRT_LIMBS 256  // limbs per block, one per thread
 // Coefficients a block can touch: limbs [i0 - 1, i0 + 256) span 257 * 64 bits
@@ -87,28 +67,6 @@ __device__ __forceinline__ u128 ff_garner(uint32_t r1, uint32_t r2, uint32_t r3,
 // 2070 at the narrowest width, 8 (mul_split.c:151-152).
 #define CRT_MAXC 2112
 
-// First coefficient touching limb i: coefficient j spans bits [j bw, j bw + 93).
-__device__ __forceinline__ uint64_t first_coef(uint64_t i, int bw) {
-    const uint64_t lo = 64 * i;
-    return lo >= 93 ? (lo - 93) / bw + 1 : 0;
-}
-
-// T_i = sum over the coefficients touching limb i of digit i of X_j 2^(j bw).
-__device__ __forceinline__ u128 limb_sum(uint64_t i, int bw, uint64_t j0, const uint64_t *xlo, const uint32_t *xhi,
-                                         uint64_t jbase, uint64_t jend) {
-    u128 T = 0;
-    const uint64_t bit = 64 * i;
-    for (uint64_t j = first_coef(i, bw) > j0 ? first_coef(i, bw) : j0; j < jend; j++) {
-        const uint64_t jb = j * (uint64_t)bw;
-        if (jb >= bit + 64) break;
-        const u128 X = ((u128)xhi[j - jbase] << 64) | xlo[j - jbase];
-        uint64_t d;
-        if (jb >= bit) d = (uint64_t)(X << (unsigned)(jb - bit));
-        else d = (uint64_t)(X >> (unsigned)(bit - jb));
-        T += d;
-    }
-    return T;
-}
 
 // One block per CRT_LIMBS output limbs (block bx of job J).
 __device__ __forceinline__ void crt_pack_body(const FFCrtJob &J, uint32_t bx, const FFGarner &G) {
@@ -323,115 +281,10 @@ extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *s
 //
 // A doubling step whose two squarings are one-block transforms (L <= 4096)
 // is latency-bound: its CRT and linear pass would be six launches of a few
-// microseconds each. Here one workgroup of 1024 threads does all of it from
-// the squares' residues: the CRT of F(k)^2 and F(k-1)^2 into registers
-// (thread t owns limbs t, t + 1024, t + 2048), then A = s1 + s2,
-// B = 4 s1 - s2 + cb0, C = B - A (fib.c:36-62) by workgroup-wide carry
-// lookahead (wavefront ballots, 16 wave summaries composed in LDS, chunks
-// of 1024 limbs in order), writing K (A or B) and C and booking their top
-// limbs.
-
-#define SS_CH 3  // chunks of SS_T limbs: len <= 3 SS_T
-
-// Coefficients touching SS_T + 1 limbs at bw >= 32, with margin.
-template <int SS_T>
-struct SSShared {
-    static constexpr int MAXC = ((SS_T + 1) * 64 + 93) / 32 + 64;
-    uint64_t xlo[MAXC];
-    uint32_t xhi[MAXC];
-    uint64_t ex[SS_T];
-    uint32_t wg[SS_T / 64], wp[SS_T / 64], wc[SS_T / 64];
-    uint32_t carry;
-    uint64_t hi_last;
-};
-
-// Carry into this thread's limb for the chunk: u (limb sum), g (generates),
-// p (propagates), cin (carry into the chunk); returns u + carry and updates
-// S.carry to the chunk's carry-out. Every thread of the block calls it.
-template <int SS_T>
-__device__ __forceinline__ uint64_t ss_lookahead(SSShared<SS_T> &S, uint64_t u, bool g, bool p) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint64_t G = __ballot(g), B = G | __ballot(p);
-    if (lane == 0) {
-        S.wg[w] = (G + B) < G;
-        S.wp[w] = (B == ~0ull && G == 0);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t c = S.carry;
-        for (int k = 0; k < SS_T / 64; k++) {
-            S.wc[k] = c;
-            c = S.wg[k] | (S.wp[k] & c);
-        }
-        S.carry = c;
-    }
-    __syncthreads();
-    const uint64_t C = (G + B + (uint64_t)S.wc[w]) ^ G ^ B;
-    const uint64_t r = u + ((C >> lane) & 1);
-    __syncthreads();  // S.wc / S.carry are rewritten by the next call
-    return r;
-}
-
-// Limbs [0, len) of the product whose residues are y ([3][L]) into out[c]
-// (limb c * 1024 + threadIdx.x).
-template <int SS_T>
-__device__ void ss_crt(SSShared<SS_T> &S, const uint32_t *y, uint32_t L, int bw, uint32_t conv, uint32_t len,
-                       const FFGarner &G, uint64_t out[SS_CH]) {
-    const int t = threadIdx.x;
-    if (threadIdx.x == 0) S.carry = 0, S.hi_last = 0;
-    __syncthreads();
-    for (int c = 0; c < SS_CH; c++) {
-        out[c] = 0;
-        const uint64_t i0 = (uint64_t)c * SS_T;
-        if (i0 >= len || conv == 0) continue;  // uniform across the block
-        const uint64_t i1 = i0 + SS_T < len ? i0 + SS_T : len;
-        const uint64_t jbase = first_coef(i0 ? i0 - 1 : 0, bw);
-        uint64_t jend = (64 * i1 + bw - 1) / bw;
-        if (jend > conv) jend = conv;
-        for (uint64_t j = jbase + t; j < jend; j += SS_T) {
-            const u128 X = ff_garner(y[j], y[L + j], y[2 * (size_t)L + j], G);
-            S.xlo[j - jbase] = (uint64_t)X;
-            S.xhi[j - jbase] = (uint32_t)(X >> 64);
-        }
-        __syncthreads();
-        const uint64_t i = i0 + t;
-        const bool live = i < i1;
-        const u128 T = live ? limb_sum(i, bw, jbase, S.xlo, S.xhi, jbase, jend) : 0;
-        S.ex[t] = (uint64_t)(T >> 64);
-        __syncthreads();
-        const uint64_t hprev = t ? S.ex[t - 1] : S.hi_last;
-        const u128 u = (u128)(uint64_t)T + hprev;
-        const bool g = live && (u >> 64) != 0;
-        const bool p = !live || (!g && (uint64_t)u == ~0ull);
-        __syncthreads();
-        if (t == SS_T - 1) S.hi_last = S.ex[t];
-        const uint64_t v = ss_lookahead(S, (uint64_t)u, g, p);
-        out[c] = live ? v : 0;
-    }
-}
-
-// out = a + b' + cin0 over len limbs in registers (b' = b or ~b).
-template <int SS_T>
-__device__ void ss_add(SSShared<SS_T> &S, const uint64_t a[SS_CH], const uint64_t b[SS_CH], bool neg, uint32_t cin0,
-                       uint32_t len, uint64_t out[SS_CH]) {
-    if (threadIdx.x == 0) S.carry = cin0;
-    __syncthreads();
-    for (int c = 0; c < SS_CH; c++) {
-        const uint64_t i = (uint64_t)c * SS_T + threadIdx.x;
-        if ((uint64_t)c * SS_T >= len) {
-            out[c] = 0;
-            continue;
-        }
-        const bool live = i < len;
-        const uint64_t bv = neg ? ~b[c] : b[c];
-        const uint64_t u = a[c] + bv;
-        const bool g = live && u < a[c];
-        const bool p = !live || u == ~0ull;
-        const uint64_t v = ss_lookahead(S, u, g, p);
-        out[c] = live ? v : 0;
-    }
-}
-
+// microseconds each. Here one workgroup does all of it from the squares'
+// residues: the CRT of F(k)^2 and F(k-1)^2 into registers (thread t owns
+// limbs t, t + SS_T, t + 2 SS_T), the linear step by workgroup-wide carry
+// lookahead, writing K (A or B) and C and booking their top limbs.
 template <int SS_T>
 __global__ __launch_bounds__(SS_T) void k_small_step(FFSmallStep a, FFGarner G) {
     __shared__ SSShared<SS_T> S;
@@ -441,43 +294,23 @@ __global__ __launch_bounds__(SS_T) void k_small_step(FFSmallStep a, FFGarner G) 
         atomicExch(a.keyC, 0ull);
         __threadfence();
     }
-    uint64_t s1[SS_CH], s2[SS_CH], A[SS_CH], D[SS_CH], B[SS_CH], C[SS_CH];
+    uint64_t s1[SS_CH], s2[SS_CH], K[SS_CH], C[SS_CH];
     ss_crt(S, a.y1, a.L1, a.bw1, a.conv1, a.len, G, s1);
     ss_crt(S, a.y2, a.L2, a.bw2, a.conv2, a.len, G, s2);
-    ss_add(S, s1, s2, false, 0, a.len, A);
-    // 4 s1 (limb i takes the top two bits of limb i - 1), then + cb0
-    for (int c = 0; c < SS_CH; c++) {
-        S.ex[t] = s1[c];
-        __syncthreads();
-        const uint64_t prev = t ? S.ex[t - 1] : (c ? S.hi_last : 0);
-        __syncthreads();
-        if (t == SS_T - 1) S.hi_last = s1[c];
-        D[c] = (s1[c] << 2) | (prev >> 62);
-        __syncthreads();
-    }
-    {
-        // cb0 = +2: add 2; cb0 = -2: add 2^(64 len) - 2 (limb 0 = ~1, the rest all ones)
-        uint64_t k[SS_CH];
-        for (int c = 0; c < SS_CH; c++) {
-            const uint64_t i = (uint64_t)c * SS_T + t;
-            k[c] = a.cb0 >= 0 ? (i == 0 ? (uint64_t)a.cb0 : 0) : (i == 0 ? ~(uint64_t)(-a.cb0 - 1) : ~0ull);
-        }
-        ss_add(S, D, k, false, 0, a.len, D);
-    }
-    ss_add(S, D, s2, true, 1, a.len, B);
-    ss_add(S, B, A, true, 1, a.len, C);
-    const uint64_t lo_top = a.len >= 3 ? a.len - 3 : 0;
+    ss_linear(S, s1, s2, a.cb0, a.len, K, C, a.bit);
     for (int c = 0; c < SS_CH; c++) {
         const uint64_t i = (uint64_t)c * SS_T + t;
         if (i >= a.len) break;
-        const uint64_t kv = a.bit ? B[c] : A[c];
-        a.K[i] = kv;
+        a.K[i] = K[c];
         a.C[i] = C[c];
-        if (i >= lo_top) {
-            if (kv) atomicMax(a.keyK, ((unsigned long long)(i + 1) << 8) | (unsigned)(64 - __clzll((long long)kv)));
-            if (C[c]) atomicMax(a.keyC, ((unsigned long long)(i + 1) << 8) | (unsigned)(64 - __clzll((long long)C[c])));
-        }
+        ss_book(a.keyK, i, K[c], a.len);
+        ss_book(a.keyC, i, C[c], a.len);
     }
+}
+
+FFGarner ff_garner_consts(void) {
+    garner_init();
+    return g_garner;
 }
 
 extern "C" hipError_t ff_launch_small_step(const FFSmallStep *a, hipStream_t st) {

@@ -519,6 +519,20 @@ struct FibPlanVal {
 // lengths come from fib_bits_pred and nothing waits for the device until
 // the end, where the lengths the kernels booked are checked against the
 // predictions (a mismatch returns false; the caller reruns reading lengths).
+// Transform of F(k)^2 in the chain of first steps (ff_launch_fib_early): a
+// power-of-2 length, the widest coefficients that fit (as product_shape).
+static bool early_shape(uint64_t bits, int *lg, int *bw, uint64_t *conv) {
+    for (int w = 35; w >= 32; w--) {
+        if (!width_ok(bits, bits, (unsigned)w)) continue;
+        const uint64_t cv = 2 * ((bits + w - 1) / w) - 1;
+        int g = 4;
+        while ((1ull << g) < cv) g++;
+        *lg = g, *bw = w, *conv = cv;
+        return g <= FF_EARLY_LG;
+    }
+    return false;
+}
+
 static bool dfib(uint64_t n, bool predict, DNum *res) {
     slots_reset(8 * 64 + 16);
     if (n == 0) {
@@ -526,7 +540,7 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         return true;
     }
     std::vector<FibPlanVal> checks;
-    DNum fkm1 = dnum_u64(0), fk = dnum_u64(1);
+    DNum fkm1{}, fk{};
     uint64_t k = 1;
     int k_odd = 1;
     const int top = 63 - __builtin_clzll(n);
@@ -588,8 +602,65 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         if (!sf.split) ff_pick_scheds(&sf, &bx, &by, &gen, 1, 1, g_dispatch, &ds[s0.size()]);
         picked[std::make_tuple(n, g_dispatch, g_dispatch_gen)] = ds;
     }
-    int step = 0;
-    for (int i = top - 1; i >= 1; i--, step++) {
+    int i = top - 1, step = 0;
+    if (predict && !g_force_len && top >= 7) {
+        // The top seven bits on the host (F(k0) < 2^88 for k0 < 128), then
+        // the steps while F(k)^2 is a transform of at most 2^10 words in
+        // one workgroup, the pair never leaving LDS.
+        const int i0 = top - 6;
+        const uint64_t k0 = n >> i0;
+        u128 fa = 0, fb = 1;  // F(0), F(1)
+        for (uint64_t j = 1; j < k0; j++) {
+            const u128 t = fa + fb;
+            fa = fb, fb = t;
+        }
+        FFEarlyArgs ea;
+        memset(&ea, 0, sizeof ea);
+        ea.f[0][0] = (uint64_t)fa, ea.f[0][1] = (uint64_t)(fa >> 64);
+        ea.f[1][0] = (uint64_t)fb, ea.f[1][1] = (uint64_t)(fb >> 64);
+        uint64_t kk = k0, blo = 0, bhi = 0;
+        int ii = i0 - 1, kodd = (int)(k0 & 1);
+        std::vector<uint64_t> pred;
+        while (ii >= 1 && ea.nsteps < FF_EARLY_MAXS) {
+            const uint64_t b1 = P(kk, FB_F), b2 = P(kk - 1, FB_F), bsq = P(kk, FB_F2);
+            int lg, bw;
+            uint64_t conv;
+            const uint64_t len = (bsq + 63) / 64 + 1;
+            if (!early_shape(b1, &lg, &bw, &conv) || len > FF_EARLY_LIMBS) break;
+            const int bit = (int)((n >> ii) & 1);
+            FFEarlyStep &e = ea.st[ea.nsteps++];
+            e.lg = lg, e.bw = bw;
+            e.conv1 = (uint32_t)conv;
+            e.conv2 = b2 ? (uint32_t)(2 * ((b2 + bw - 1) / bw) - 1) : 0;
+            e.nw1 = (uint32_t)((b1 + 31) / 32), e.nw2 = (uint32_t)((b2 + 31) / 32);
+            e.len = (uint32_t)len;
+            e.cb0 = kodd ? -2 : 2;
+            e.bit = bit;
+            blo = P(2 * kk - 1 + bit, FB_F), bhi = P(2 * kk + bit, FB_F);
+            pred.push_back(blo), pred.push_back(bhi);
+            kk = 2 * kk + bit, kodd = bit, ii--;
+        }
+        if (predict && ea.nsteps > 0) {
+            for (int lg = 4; lg <= FF_EARLY_LG; lg++)
+                for (int q = 0; q < 3; q++) ea.sc[lg][q] = ff_inv_scale(1u << lg, q);
+            const int slot0 = slot_new();
+            for (size_t q = 1; q < pred.size(); q++) slot_new();
+            for (size_t q = 0; q < pred.size(); q++) checks.push_back({slot0 + (int)q, pred[q]});
+            fkm1 = dnum_alloc(blo ? (blo + 63) / 64 : 1);
+            fk = dnum_alloc((bhi + 63) / 64);
+            fkm1.bits = blo, fkm1.n = blo ? (blo + 63) / 64 : 1;
+            fk.bits = bhi, fk.n = (bhi + 63) / 64;
+            ea.nlo = (uint32_t)fkm1.n, ea.nhi = (uint32_t)fk.n;
+            ea.lo = fkm1.d, ea.hi = fk.d;
+            ea.keys = g_slots + slot0;
+            FF_CHECK(ff_launch_fib_early(&ea, g_st));
+            k = kk, k_odd = kodd;
+            step = (top - 1) - ii;
+            i = ii;
+        }
+    }
+    if (!fk.d) fkm1 = dnum_u64(0), fk = dnum_u64(1);
+    for (; i >= 1; i--, step++) {
         if (predict && small_fusable(fk)) {
             const int bit = (int)((n >> i) & 1);
             const uint64_t bsq = P(k, FB_F2);

@@ -125,6 +125,7 @@ void ff_prof_end(int kind, double bytes, hipStream_t st);
 int ff_tables_init(void);
 const FFTables *ff_tables_dev(void);
 void ff_prime_consts(FFPrime out[3]);
+uint32_t ff_inv_scale(uint32_t L, int i);  // R^2 / L mod p_i: the CRT-ready inverse scale
 int ff_plan_build(FFPlan *pl, uint32_t L, int bw, int lg, int r3, uint64_t conv, int S_log, int tile_words);
 hipError_t ff_launch_forward(const FFPlan *pl, const FFXformSet *xs, int nx, double operand_bytes, hipStream_t st);
 hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int nc, int ngeneral, double operand_bytes,
@@ -151,6 +152,29 @@ struct FFSmallStep {
     unsigned long long *keyK, *keyC;
 };
 hipError_t ff_launch_small_step(const FFSmallStep *a, hipStream_t st);
+
+// The first doubling steps of F(n), chained in one workgroup (ff_early.hip).
+#define FF_EARLY_LG 10       // largest transform: 2^10 words
+#define FF_EARLY_LIMBS 576   // limbs of a number in the chain
+#define FF_EARLY_MAXS 16     // steps per launch
+struct FFEarlyStep {
+    int lg, bw;              // transform 2^lg words, coefficient width
+    uint32_t conv1, conv2;   // coefficients of F(k)^2 and F(k-1)^2
+    uint32_t nw1, nw2;       // 32-bit words of F(k) and F(k-1)
+    uint32_t len;            // limbs of the step's results (n(F(k)^2) + 1)
+    int cb0;                 // 2 (-1)^k
+    int bit;                 // next bit of n
+};
+struct FFEarlyArgs {
+    uint64_t f[2][2];        // F(k0 - 1), F(k0)
+    int nsteps;
+    uint32_t nlo, nhi;       // limbs of the final pair to write
+    FFEarlyStep st[FF_EARLY_MAXS];
+    uint32_t sc[FF_EARLY_LG + 1][3];  // CRT scale of a 2^lg transform, per prime
+    uint64_t *lo, *hi;       // the final pair F(k - 1), F(k)
+    unsigned long long *keys;  // 2 per step: length keys of the step's new pair
+};
+hipError_t ff_launch_fib_early(const FFEarlyArgs *a, hipStream_t st);
 
 // Inner blocks alone, in place (non-small plans): forward or inverse levels.
 hipError_t ff_launch_inner_part_list(bool inv, const FFInnerItem *items, const uint32_t *start, int n,

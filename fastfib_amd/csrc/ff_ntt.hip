@@ -33,9 +33,6 @@ static const uint32_t kG[3] = {5, 31, 13};
 static FFTables *g_tables = nullptr;
 static FFPrime g_prime[3];
 
-struct FFPrimes {
-    FFPrime p[3];
-};
 static FFPrimes g_primes_arg;
 
 static uint32_t h_pow(uint32_t a, uint64_t e, uint32_t p) {
@@ -389,6 +386,9 @@ static uint32_t h_scale(uint32_t L, int i) {
     const uint64_t R2 = R * R % p;
     return (uint32_t)(R2 * h_pow(L % p, p - 2, p) % p);
 }
+
+uint32_t ff_inv_scale(uint32_t L, int i) { return h_scale(L, i); }
+FFPrimes ff_primes(void) { return g_primes_arg; }
 
 // Fills one outer pass for blocks of Lb words with R rows.
 static void fill_pass(FFPass *ps, uint32_t L, uint32_t Lb, uint32_t R, int top, int bw, int tile_words) {

@@ -102,15 +102,26 @@ __device__ __forceinline__ void ff_dit_group(uint32_t *v, int low, int b_lo, con
     }
 }
 
+// The three primes' constants, as a kernel argument.
+struct FFPrimes {
+    FFPrime p[3];
+};
+FFPrimes ff_primes(void);                // ff_ntt.hip
+
 struct FFTile {
     uint32_t *s;  // LDS base
     int ld;       // row stride in words
     int C;        // columns (power of 2)
     int logC;
     int rho;      // rows per sub-transform = 2^rho
-    int nsub;     // stacked sub-transforms (1, or 3 under a radix-3 layer)
+    int nsub;     // stacked sub-transforms (1, or 3 under a radix-3 layer, or several operands)
     int pad;      // 1: one spare word after every 16 rows (single-column tiles)
+    int tid0 = 0; // the threads [tid0, tid0 + nthr) work on the tile;
+    int nthr = 0; // nthr = 0: the whole workgroup
 };
+
+__device__ __forceinline__ int ff_tile_tid(const FFTile &t) { return (int)threadIdx.x - t.tid0; }
+__device__ __forceinline__ int ff_tile_nthr(const FFTile &t) { return t.nthr ? t.nthr : (int)blockDim.x; }
 
 // LDS index of row position pos (= sub * 2^rho + row), column c.
 __device__ __forceinline__ int ff_pos_addr(const FFTile &t, int pos, int c) {
@@ -169,7 +180,7 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
     const int G = 1 << NL;
     const int per_sub = (1 << (t.rho - NL)) << t.logC;
     const int ngroups = per_sub * t.nsub;
-    for (int gi = threadIdx.x; gi < ngroups; gi += blockDim.x) {
+    for (int gi = ff_tile_tid(t); gi < ngroups; gi += ff_tile_nthr(t)) {
         const int sub = gi / per_sub;
         const int g = gi - sub * per_sub;
         const int c = g & (t.C - 1);
@@ -252,7 +263,7 @@ template <class Ld, class St, class Gt = FFNoTw>
 __device__ void ff_pass_through(const FFTile &t, bool from_g, bool to_g, Ld ld, St st, const Gt &gt = Gt(),
                                 uint32_t p = 0, uint32_t pinv = 0) {
     const int n = t.nsub << t.logC;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = ff_tile_tid(t); i < n; i += ff_tile_nthr(t)) {
         const int pos = i >> t.logC, c = i & (t.C - 1);
         uint32_t v = from_g ? ld(pos, c) : t.s[ff_pos_addr(t, pos, c)];
         if constexpr (Gt::on) {

@@ -295,9 +295,10 @@ __global__ __launch_bounds__(SS_T) void k_small_step(FFSmallStep a, FFGarner G) 
         __threadfence();
     }
     uint64_t s1[SS_CH], s2[SS_CH], K[SS_CH], C[SS_CH];
-    ss_crt(S, a.y1, a.L1, a.bw1, a.conv1, a.len, G, s1);
-    ss_crt(S, a.y2, a.L2, a.bw2, a.conv2, a.len, G, s2);
-    ss_linear(S, s1, s2, a.cb0, a.len, K, C, a.bit);
+    SSRun R{0, 0};
+    ss_crt(S, R, a.y1, a.L1, a.bw1, a.conv1, a.len, G, s1);
+    ss_crt(S, R, a.y2, a.L2, a.bw2, a.conv2, a.len, G, s2);
+    ss_linear(S, R, s1, s2, a.cb0, a.len, K, C, a.bit);
     for (int c = 0; c < SS_CH; c++) {
         const uint64_t i = (uint64_t)c * SS_T + t;
         if (i >= a.len) break;

@@ -62,53 +62,60 @@ __device__ __forceinline__ u128 limb_sum(uint64_t i, int bw, uint64_t j0, const 
 
 #define SS_CH 3  // chunks of SS_T limbs: len <= 3 SS_T
 
-// Coefficients touching SS_T + 1 limbs at bw >= 32, with margin.
+// Coefficients touching SS_T + 1 limbs at bw >= 32, with margin. The wave
+// summaries are double-buffered by the lookahead's parity (SSRun.par), so
+// consecutive lookaheads need one barrier each.
 template <int SS_T>
 struct SSShared {
     static constexpr int MAXC = ((SS_T + 1) * 64 + 93) / 32 + 64;
     uint64_t xlo[MAXC];
     uint32_t xhi[MAXC];
     uint64_t ex[SS_T];
-    uint32_t wg[SS_T / 64], wp[SS_T / 64], wc[SS_T / 64];
-    uint32_t carry;
+    uint32_t wg[2][SS_T / 64], wp[2][SS_T / 64];
     uint64_t hi_last;
 };
 
-// Carry into this thread's limb for the chunk: u (limb sum), g (generates),
-// p (propagates), cin (carry into the chunk); returns u + carry and updates
-// S.carry to the chunk's carry-out. Every thread of the block calls it.
+// Per-thread state of a sequence of lookaheads (the same in every thread):
+// the carry into the next chunk and the summary buffer to write next.
+struct SSRun {
+    uint32_t cin;
+    int par;
+};
+
+// Carry into this thread's limb for a chunk: u (limb sum), g (generates),
+// p (propagates), R.cin (carry into the chunk); returns u + carry and sets
+// R.cin to the chunk's carry-out. Inside a wavefront the carries come from
+// one 64-bit addition of the ballots (A = g, B = g | p: C = (A + B + c) ^ A
+// ^ B); every wavefront then composes the SS_T / 64 wave summaries the same
+// way, so one barrier suffices. Every thread of the block calls it.
 template <int SS_T>
-__device__ __forceinline__ uint64_t ss_lookahead(SSShared<SS_T> &S, uint64_t u, bool g, bool p) {
+__device__ __forceinline__ uint64_t ss_lookahead(SSShared<SS_T> &S, SSRun &R, uint64_t u, bool g, bool p) {
+    constexpr int NW = SS_T / 64;
+    static_assert(NW < 64, "wave summaries fit one ballot");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t G = __ballot(g), B = G | __ballot(p);
     if (lane == 0) {
-        S.wg[w] = (G + B) < G;
-        S.wp[w] = (B == ~0ull && G == 0);
+        S.wg[R.par][w] = (G + B) < G;        // carry out with no carry in
+        S.wp[R.par][w] = (B == ~0ull && G == 0);  // every lane propagates
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t c = S.carry;
-        for (int k = 0; k < SS_T / 64; k++) {
-            S.wc[k] = c;
-            c = S.wg[k] | (S.wp[k] & c);
-        }
-        S.carry = c;
-    }
-    __syncthreads();
-    const uint64_t C = (G + B + (uint64_t)S.wc[w]) ^ G ^ B;
-    const uint64_t r = u + ((C >> lane) & 1);
-    __syncthreads();  // S.wc / S.carry are rewritten by the next call
-    return r;
+    const bool wgl = lane < NW && S.wg[R.par][lane];
+    const bool wpl = lane < NW && S.wp[R.par][lane];
+    const uint64_t WG = __ballot(wgl), WB = WG | __ballot(wpl);
+    const uint64_t WC = (WG + WB + (uint64_t)R.cin) ^ WG ^ WB;  // bit k: carry into wave k; bit NW: out
+    R.cin = (uint32_t)((WC >> NW) & 1);
+    R.par ^= 1;
+    const uint64_t C = (G + B + ((WC >> w) & 1)) ^ G ^ B;
+    return u + ((C >> lane) & 1);
 }
 
 // Limbs [0, len) of the product whose residues are y ([3][L]) into out[c]
-// (limb c * 1024 + threadIdx.x).
+// (limb c * SS_T + threadIdx.x).
 template <int SS_T>
-__device__ void ss_crt(SSShared<SS_T> &S, const uint32_t *y, uint32_t L, int bw, uint32_t conv, uint32_t len,
-                       const FFGarner &G, uint64_t out[SS_CH]) {
+__device__ void ss_crt(SSShared<SS_T> &S, SSRun &R, const uint32_t *y, uint32_t L, int bw, uint32_t conv,
+                       uint32_t len, const FFGarner &G, uint64_t out[SS_CH]) {
     const int t = threadIdx.x;
-    if (threadIdx.x == 0) S.carry = 0, S.hi_last = 0;
-    __syncthreads();
+    R.cin = 0;
     for (int c = 0; c < SS_CH; c++) {
         out[c] = 0;
         const uint64_t i0 = (uint64_t)c * SS_T;
@@ -128,23 +135,24 @@ __device__ void ss_crt(SSShared<SS_T> &S, const uint32_t *y, uint32_t L, int bw,
         const u128 T = live ? limb_sum(i, bw, jbase, S.xlo, S.xhi, jbase, jend) : 0;
         S.ex[t] = (uint64_t)(T >> 64);
         __syncthreads();
-        const uint64_t hprev = t ? S.ex[t - 1] : S.hi_last;
+        // hi_last: the previous chunk's last high word (written below, read
+        // here one chunk later, with two barriers between)
+        const uint64_t hprev = t ? S.ex[t - 1] : (c ? S.hi_last : 0);
         const u128 u = (u128)(uint64_t)T + hprev;
         const bool g = live && (u >> 64) != 0;
         const bool p = !live || (!g && (uint64_t)u == ~0ull);
         __syncthreads();
         if (t == SS_T - 1) S.hi_last = S.ex[t];
-        const uint64_t v = ss_lookahead(S, (uint64_t)u, g, p);
+        const uint64_t v = ss_lookahead(S, R, (uint64_t)u, g, p);
         out[c] = live ? v : 0;
     }
 }
 
 // out = a + b' + cin0 over len limbs in registers (b' = b or ~b).
 template <int SS_T>
-__device__ void ss_add(SSShared<SS_T> &S, const uint64_t a[SS_CH], const uint64_t b[SS_CH], bool neg, uint32_t cin0,
-                       uint32_t len, uint64_t out[SS_CH]) {
-    if (threadIdx.x == 0) S.carry = cin0;
-    __syncthreads();
+__device__ void ss_add(SSShared<SS_T> &S, SSRun &R, const uint64_t a[SS_CH], const uint64_t b[SS_CH], bool neg,
+                       uint32_t cin0, uint32_t len, uint64_t out[SS_CH]) {
+    R.cin = cin0;
     for (int c = 0; c < SS_CH; c++) {
         const uint64_t i = (uint64_t)c * SS_T + threadIdx.x;
         if ((uint64_t)c * SS_T >= len) {
@@ -156,7 +164,7 @@ __device__ void ss_add(SSShared<SS_T> &S, const uint64_t a[SS_CH], const uint64_
         const uint64_t u = a[c] + bv;
         const bool g = live && u < a[c];
         const bool p = !live || u == ~0ull;
-        const uint64_t v = ss_lookahead(S, u, g, p);
+        const uint64_t v = ss_lookahead(S, R, u, g, p);
         out[c] = live ? v : 0;
     }
 }
@@ -165,20 +173,21 @@ __device__ void ss_add(SSShared<SS_T> &S, const uint64_t a[SS_CH], const uint64_
 // (fib.c:36-62): A = s1 + s2 = F(2k-1), B = 4 s1 - s2 + cb0 = F(2k+1),
 // C = B - A = F(2k), cb0 = 2 (-1)^k; K = bit ? B : A.
 template <int SS_T>
-__device__ void ss_linear(SSShared<SS_T> &S, const uint64_t s1[SS_CH], const uint64_t s2[SS_CH], int cb0,
+__device__ void ss_linear(SSShared<SS_T> &S, SSRun &R, const uint64_t s1[SS_CH], const uint64_t s2[SS_CH], int cb0,
                           uint32_t len, uint64_t K[SS_CH], uint64_t C[SS_CH], int bit) {
     const int t = threadIdx.x;
     uint64_t A[SS_CH], D[SS_CH], B[SS_CH];
-    ss_add(S, s1, s2, false, 0, len, A);
+    ss_add(S, R, s1, s2, false, 0, len, A);
     // 4 s1 (limb i takes the top two bits of limb i - 1), then + cb0
     for (int c = 0; c < SS_CH; c++) {
+        D[c] = 0;
+        if ((uint64_t)c * SS_T >= len) continue;  // uniform
         S.ex[t] = s1[c];
         __syncthreads();
         const uint64_t prev = t ? S.ex[t - 1] : (c ? S.hi_last : 0);
         __syncthreads();
         if (t == SS_T - 1) S.hi_last = s1[c];
         D[c] = (s1[c] << 2) | (prev >> 62);
-        __syncthreads();
     }
     {
         // cb0 = +2: add 2; cb0 = -2: add 2^(64 len) - 2 (limb 0 = ~1, the rest all ones)
@@ -187,10 +196,10 @@ __device__ void ss_linear(SSShared<SS_T> &S, const uint64_t s1[SS_CH], const uin
             const uint64_t i = (uint64_t)c * SS_T + t;
             k[c] = cb0 >= 0 ? (i == 0 ? (uint64_t)cb0 : 0) : (i == 0 ? ~(uint64_t)(-cb0 - 1) : ~0ull);
         }
-        ss_add(S, D, k, false, 0, len, D);
+        ss_add(S, R, D, k, false, 0, len, D);
     }
-    ss_add(S, D, s2, true, 1, len, B);
-    ss_add(S, B, A, true, 1, len, C);
+    ss_add(S, R, D, s2, true, 1, len, B);
+    ss_add(S, R, B, A, true, 1, len, C);
     for (int c = 0; c < SS_CH; c++) K[c] = bit ? B[c] : A[c];
 }
 

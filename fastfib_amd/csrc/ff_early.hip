@@ -11,8 +11,8 @@
 //   - the CRT of both squares and the linear step of the fused small step
 //     (fib.c:36-62), every limb in a thread;
 //   - the new pair (F(k'-1), F(k')) replaces the old one in LDS.
-// The host starts the chain at F(k0 - 1), F(k0) for the top seven bits of
-// n (k0 < 128, exact in 128 bits) and passes each step's sizes, predicted
+// The host starts the chain at F(k0 - 1), F(k0) for the top ten bits of n
+// (k0 < 1024, at most 12 limbs) and passes each step's sizes, predicted
 // from the closed form; every step books its new pair's length keys, which
 // the host checks against the predictions at the end of F(n) like those of
 // the other steps. The transforms are power-of-2 lengths only: a step
@@ -37,8 +37,8 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
     __shared__ EarlyShared E;
     const int t = threadIdx.x;
     for (int i = t; i < FF_EARLY_LIMBS; i += EARLY_T) {
-        E.num[0][i] = i < 2 ? a.f[0][i] : 0;
-        E.num[1][i] = i < 2 ? a.f[1][i] : 0;
+        E.num[0][i] = i < FF_EARLY_F0 ? a.f[0][i] : 0;
+        E.num[1][i] = i < FF_EARLY_F0 ? a.f[1][i] : 0;
     }
     if (t < 2 * a.nsteps) atomicExch(a.keys + t, 0ull);
     __threadfence();
@@ -47,6 +47,7 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
     const int tid0 = 320 * pi, nthr = pi < 2 ? 320 : 384;
     const FFPrime P = PR.p[pi];
     const uint32_t p = P.p, pinv = P.pinv;
+    SSRun R{0, 0};
     for (int s = 0; s < a.nsteps; s++) {
         const FFEarlyStep st = a.st[s];
         const int lg = st.lg;
@@ -69,9 +70,9 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
             E.y[i >> lg][pi * L + (i & (L - 1))] = ff_mont(tl.s[ff_pos_addr(tl, i, 0)], sc, p, pinv);
         __syncthreads();
         uint64_t s1[SS_CH], s2[SS_CH], K[SS_CH], C[SS_CH];
-        ss_crt(E.ss, E.y[0], L, st.bw, st.conv1, st.len, G, s1);
-        ss_crt(E.ss, E.y[1], L, st.bw, st.conv2, st.len, G, s2);
-        ss_linear(E.ss, s1, s2, st.cb0, st.len, K, C, st.bit);
+        ss_crt(E.ss, R, E.y[0], L, st.bw, st.conv1, st.len, G, s1);
+        ss_crt(E.ss, R, E.y[1], L, st.bw, st.conv2, st.len, G, s2);
+        ss_linear(E.ss, R, s1, s2, st.cb0, st.len, K, C, st.bit);
         // bit 0: (F(2k-1), F(2k)) = (A, C); bit 1: (F(2k), F(2k+1)) = (C, B)
         const uint64_t lo = st.bit ? C[0] : K[0], hi = st.bit ? K[0] : C[0];
         if (t < (int)st.len) {
@@ -96,9 +97,9 @@ hipError_t ff_launch_fib_early(const FFEarlyArgs *a, hipStream_t st) {
             e.conv1 > (1u << e.lg) || e.conv2 > e.conv1 || e.nw1 > 2 * FF_EARLY_LIMBS || e.nw2 > e.nw1)
             return hipErrorInvalidValue;
     }
-    ff_prof_begin(FK_STEP, st);
+    ff_prof_begin(FK_EARLY, st);
     hipLaunchKernelGGL(k_fib_early, dim3(1), dim3(EARLY_T), 0, st, *a, ff_tables_dev(), ff_primes(),
                        ff_garner_consts());
-    ff_prof_end(FK_STEP, 16.0 * a->st[a->nsteps - 1].len, st);
+    ff_prof_end(FK_EARLY, 16.0 * a->st[a->nsteps - 1].len, st);
     return hipGetLastError();
 }

@@ -533,6 +533,64 @@ static bool early_shape(uint64_t bits, int *lg, int *bw, uint64_t *conv) {
     return false;
 }
 
+// F(k - 1), F(k) exactly on the host for k < 1024 (at most 12 limbs), by the
+// doubling of the device steps: s1 = F(k)^2, s2 = F(k-1)^2,
+// F(2k-1) = s1 + s2, F(2k+1) = 4 s1 - s2 + 2 (-1)^k, F(2k) = F(2k+1) - F(2k-1).
+typedef std::vector<uint64_t> HNum;
+
+static void hn_trim(HNum &a) {
+    while (a.size() > 1 && a.back() == 0) a.pop_back();
+}
+
+static HNum hn_sqr(const HNum &a) {
+    const size_t n = a.size();
+    HNum r(2 * n, 0);
+    for (size_t i = 0; i < n; i++) {
+        u128 c = 0;
+        for (size_t j = 0; j < n; j++) {
+            const u128 t = (u128)a[i] * a[j] + r[i + j] + c;
+            r[i + j] = (uint64_t)t;
+            c = t >> 64;
+        }
+        r[i + n] = (uint64_t)c;
+    }
+    hn_trim(r);
+    return r;
+}
+
+// a + m b + c (m = 1, or -1 with a >= b; c small, the result nonnegative)
+static HNum hn_axpy(const HNum &a, const HNum &b, int m, int64_t c) {
+    const size_t n = (a.size() > b.size() ? a.size() : b.size()) + 1;
+    HNum r(n, 0);
+    __int128 carry = c;
+    for (size_t i = 0; i < n; i++) {
+        __int128 t = carry + (i < a.size() ? (__int128)a[i] : 0);
+        if (i < b.size()) t += m > 0 ? (__int128)b[i] : -(__int128)b[i];
+        r[i] = (uint64_t)t;
+        carry = t >> 64;  // arithmetic: -1 on a borrow
+    }
+    hn_trim(r);
+    return r;
+}
+
+static void hn_fib_pair(uint64_t k, HNum *fkm1, HNum *fk) {
+    HNum lo{0}, hi{1};  // F(0), F(1)
+    const int top = 63 - __builtin_clzll(k);
+    uint64_t kk = 1;
+    for (int i = top - 1; i >= 0; i--) {
+        const int bit = (int)((k >> i) & 1);
+        const HNum s1 = hn_sqr(hi), s2 = hn_sqr(lo);
+        const HNum A = hn_axpy(s1, s2, 1, 0);
+        const HNum s1x4 = hn_axpy(hn_axpy(s1, s1, 1, 0), hn_axpy(s1, s1, 1, 0), 1, 0);
+        const HNum B = hn_axpy(s1x4, s2, -1, (kk & 1) ? -2 : 2);
+        const HNum C = hn_axpy(B, A, -1, 0);
+        if (bit) lo = C, hi = B;
+        else lo = A, hi = C;
+        kk = 2 * kk + bit;
+    }
+    *fkm1 = lo, *fk = hi;
+}
+
 static bool dfib(uint64_t n, bool predict, DNum *res) {
     slots_reset(8 * 64 + 16);
     if (n == 0) {
@@ -603,21 +661,18 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         picked[std::make_tuple(n, g_dispatch, g_dispatch_gen)] = ds;
     }
     int i = top - 1, step = 0;
-    if (predict && !g_force_len && top >= 7) {
-        // The top seven bits on the host (F(k0) < 2^88 for k0 < 128), then
-        // the steps while F(k)^2 is a transform of at most 2^10 words in
-        // one workgroup, the pair never leaving LDS.
-        const int i0 = top - 6;
+    if (predict && !g_force_len && top >= 10) {
+        // The top ten bits on the host (F(k0) for k0 < 1024: at most 12
+        // limbs), then the steps while F(k)^2 is a transform of at most
+        // 2^10 words in one workgroup, the pair never leaving LDS.
+        const int i0 = top - 9;
         const uint64_t k0 = n >> i0;
-        u128 fa = 0, fb = 1;  // F(0), F(1)
-        for (uint64_t j = 1; j < k0; j++) {
-            const u128 t = fa + fb;
-            fa = fb, fb = t;
-        }
+        HNum fa, fb;
+        hn_fib_pair(k0, &fa, &fb);
         FFEarlyArgs ea;
         memset(&ea, 0, sizeof ea);
-        ea.f[0][0] = (uint64_t)fa, ea.f[0][1] = (uint64_t)(fa >> 64);
-        ea.f[1][0] = (uint64_t)fb, ea.f[1][1] = (uint64_t)(fb >> 64);
+        for (size_t q = 0; q < fa.size() && q < FF_EARLY_F0; q++) ea.f[0][q] = fa[q];
+        for (size_t q = 0; q < fb.size() && q < FF_EARLY_F0; q++) ea.f[1][q] = fb[q];
         uint64_t kk = k0, blo = 0, bhi = 0;
         int ii = i0 - 1, kodd = (int)(k0 & 1);
         std::vector<uint64_t> pred;

@@ -112,7 +112,7 @@ struct FFInnerItem {
 // Kernel kinds for the live per-launch timer (ff_prof_*).
 enum FFKernelKind {
     FK_FWD_TOP, FK_FWD, FK_INNER, FK_SMALL, FK_INV, FK_INV_TOP,
-    FK_CRT_PACK, FK_CRT_SCAN, FK_CRT_APPLY, FK_STEP, FK_CARRY, FK_NORM, FK_LINEAR, FK_DEC_LEAF, FK_COUNT
+    FK_CRT_PACK, FK_CRT_SCAN, FK_CRT_APPLY, FK_STEP, FK_CARRY, FK_NORM, FK_LINEAR, FK_DEC_LEAF, FK_EARLY, FK_COUNT
 };
 
 extern "C" {
@@ -157,6 +157,7 @@ hipError_t ff_launch_small_step(const FFSmallStep *a, hipStream_t st);
 #define FF_EARLY_LG 10       // largest transform: 2^10 words
 #define FF_EARLY_LIMBS 576   // limbs of a number in the chain
 #define FF_EARLY_MAXS 16     // steps per launch
+#define FF_EARLY_F0 12       // limbs of the starting pair (F(k0) for k0 < 1024)
 struct FFEarlyStep {
     int lg, bw;              // transform 2^lg words, coefficient width
     uint32_t conv1, conv2;   // coefficients of F(k)^2 and F(k-1)^2
@@ -166,7 +167,7 @@ struct FFEarlyStep {
     int bit;                 // next bit of n
 };
 struct FFEarlyArgs {
-    uint64_t f[2][2];        // F(k0 - 1), F(k0)
+    uint64_t f[2][FF_EARLY_F0];  // F(k0 - 1), F(k0)
     int nsteps;
     uint32_t nlo, nhi;       // limbs of the final pair to write
     FFEarlyStep st[FF_EARLY_MAXS];

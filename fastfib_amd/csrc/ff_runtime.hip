@@ -172,7 +172,7 @@ static void prof_collect(void) {
 
 static const char *kKindNames[FK_COUNT] = {"ntt_fwd_top", "ntt_fwd", "ntt_inner", "ntt_small", "ntt_inv",
                                            "ntt_inv_top", "crt_pack", "crt_scan", "crt_apply", "fib_step",
-                                           "carry", "normlen", "bigops", "dec_leaf"};
+                                           "carry", "normlen", "bigops", "dec_leaf", "fib_early"};
 
 extern "C" int ff_prof_enable(int on) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);

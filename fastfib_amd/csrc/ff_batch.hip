@@ -33,9 +33,9 @@ struct BLaunch {
     int kind;  // 0 outer list, 1 inner list, 2 crt list, 3 zero fill
     bool inv, top, r3, small;
     const void *items;
-    const uint32_t *start, *astart;
+    const uint32_t *start;
     int n;
-    uint32_t blocks, ablocks;
+    uint32_t blocks;
     size_t lds;
     double bytes, conv_total, out_total;
     uint64_t *fill;
@@ -116,6 +116,7 @@ static void build_lists(ff_batch *B) {
     // schedule) share one transform of it
     std::map<std::tuple<const uint64_t *, uint64_t, uint64_t, unsigned, const FFPlan *>, uint32_t *> shared_b;
     std::vector<FFCrtJob> jobs(n);
+    ff_lb_begin();
     for (size_t i : order) {
         const ProdShape &s = B->shape[i];
         const int il = ff_sched_inner_log(B->sched[i]), tw = ff_sched_tile_words(B->sched[i]);
@@ -134,7 +135,6 @@ static void build_lists(ff_batch *B) {
         const bool sq = !p.b || p.b == p.a;
         const size_t words = 3 * (size_t)s.L;
         const uint64_t nout = p.na + (sq ? p.na : p.nb) + 1;
-        const uint64_t nblk = (nout + 255) / 256;
         uint32_t *sb = nullptr;
         if (!sq) {
             auto bk = std::make_tuple(p.b, (uint64_t)p.nb, (uint64_t)p.bits_b, s.bw, plan[i]);
@@ -142,16 +142,15 @@ static void build_lists(ff_batch *B) {
             if (f != shared_b.end()) sb = f->second;
         }
         const bool ownb = !sq && !sb;
-        const size_t o_fb = al(words * 4), o_bgp = o_fb + (ownb ? al(words * 4) : 0);
-        const size_t o_bcin = o_bgp + al(nblk * 4), o_mode = o_bcin + al(nblk);
-        char *w = (char *)ff_dalloc(o_mode + 256);
+        const size_t o_fb = al(words * 4);
+        char *w = (char *)ff_dalloc(o_fb + (ownb ? al(words * 4) : 0) + 256);
         B->work[i] = w;
         fa[i] = (uint32_t *)w;
         fb[i] = sq ? nullptr : (ownb ? (uint32_t *)(w + o_fb) : sb);
         own_b[i] = ownb;
         if (ownb) shared_b[std::make_tuple(p.b, (uint64_t)p.nb, (uint64_t)p.bits_b, s.bw, plan[i])] = fb[i];
-        jobs[i] = FFCrtJob{fa[i], p.out, (uint32_t *)(w + o_bgp), (uint8_t *)(w + o_bcin), (uint32_t *)(w + o_mode),
-                           B->slots + i, nout, s.conv, (uint32_t)s.L, (int)s.bw};
+        jobs[i] = FFCrtJob{fa[i], p.out, ff_lb_take(ff_crt_lb_words(nout)), B->slots + i, nout, s.conv,
+                           (uint32_t)s.L, (int)s.bw};
     }
 
     // host images of every list, then one device buffer
@@ -164,7 +163,7 @@ static void build_lists(ff_batch *B) {
     };
     struct Pending {
         BLaunch L;
-        size_t items_off, start_off, astart_off;
+        size_t items_off, start_off;
     };
     std::vector<Pending> pend;
 
@@ -296,26 +295,22 @@ static void build_lists(ff_batch *B) {
     }
     {
         std::vector<FFCrtJob> js;
-        std::vector<uint32_t> start, astart;
-        uint32_t blocks = 0, ablocks = 0;
+        std::vector<uint32_t> start;
+        uint32_t blocks = 0;
         double conv = 0, out = 0;
         for (size_t i : order) {
             js.push_back(jobs[i]);
             start.push_back(blocks);
-            astart.push_back(ablocks);
-            const uint64_t nblk = (jobs[i].nout + 255) / 256;
-            blocks += (uint32_t)nblk;
-            ablocks += (uint32_t)((nblk + 255) / 256);
+            blocks += (uint32_t)ff_crt_blocks(jobs[i].nout, jobs[i].bw);
             conv += (double)jobs[i].conv;
             out += (double)jobs[i].nout;
         }
         if (!js.empty()) {
             Pending pd{};
-            pd.L.kind = 2, pd.L.n = (int)js.size(), pd.L.blocks = blocks, pd.L.ablocks = ablocks;
+            pd.L.kind = 2, pd.L.n = (int)js.size(), pd.L.blocks = blocks;
             pd.L.conv_total = conv, pd.L.out_total = out;
             pd.items_off = put(js.data(), js.size() * sizeof(FFCrtJob));
             pd.start_off = put(start.data(), start.size() * 4);
-            pd.astart_off = put(astart.data(), astart.size() * 4);
             pend.push_back(pd);
         }
     }
@@ -325,7 +320,6 @@ static void build_lists(ff_batch *B) {
         BLaunch L = pd.L;
         L.items = B->desc + pd.items_off;
         L.start = (const uint32_t *)(B->desc + pd.start_off);
-        L.astart = L.kind == 2 ? (const uint32_t *)(B->desc + pd.astart_off) : nullptr;
         B->launches.push_back(L);
     }
     for (size_t i = 0; i < n; i++) {
@@ -349,8 +343,8 @@ static void enqueue_grouped(ff_batch *B) {
                                           L.bytes, g_st));
             break;
         case 2:
-            FF_CHECK(ff_launch_crt_list((const FFCrtJob *)L.items, L.start, L.astart, L.n, L.blocks, L.ablocks,
-                                        L.conv_total, L.out_total, g_st));
+            FF_CHECK(ff_launch_crt_list((const FFCrtJob *)L.items, L.start, L.n, L.blocks, L.conv_total, L.out_total,
+                                        g_st));
             break;
         default:
             FF_CHECK(ff_launch_fill(L.fill, 1, 0, g_st));

@@ -11,23 +11,22 @@
 // coefficients touch a limb, so T_i < 5 * 2^64. With u_i = lo(T_i) +
 // hi(T_{i-1}) <= 2^64 + 3 the remaining carries are 0/1: limb i =
 // u_i + c_i mod 2^64 with c_{i+1} = g_i | (p_i & c_i), g_i = u_i >= 2^64,
-// p_i = (u_i == 2^64 - 1). Inside a wavefront the carries come from one
-// 64-bit addition of the ballots (A = g, B = g | p: C = (A + B) ^ A ^ B),
-// across the block's waves from their carry-outs, across blocks from each
-// block's generate/propagate pair:
-//   pack   Garner into LDS, T_i, u_i, in-block carries; limbs written for a
-//          zero carry into the block; block (G, P) recorded;
-//   scan   only if some block propagates (P = 1: all its 256 limbs are all
-//          ones, which needs adversarial operands), the exact carry into
-//          every block by a lookahead scan; otherwise carry_in(w) = G(w-1);
-//   apply  adds the block's carry (ripple bounded by the block) and books
-//          the top nonzero limb.
-// This replaces the reference's 8 sequential chunks plus carry fix
-// (crt.c:326-336) with a parallel carry-lookahead over all limbs.
+// p_i = (u_i == 2^64 - 1). One launch resolves them: inside a wavefront
+// from one 64-bit addition of the ballots, across the workgroup's waves
+// from their summaries, across workgroups by decoupled look-back
+// (ff_lookback.h), and the job's last workgroup books the top limb. This
+// replaces the reference's 8 sequential chunks plus carry fix
+// (crt.c:326-336).
+//
+// The same machinery fuses a whole doubling step after its two squarings
+// (k_step_fused): the CRT of both squares and the linear combination of
+// fib.c:36-62 in one launch.
 #include <stdint.h>
 
 #include "ff_crt_dev.h"
 #include "ff_internal.h"
+#include "ff_lookback.h"
+#include "ff_runtime.h"
 
 static FFGarner g_garner;
 static int g_garner_ready = 0;
@@ -62,80 +61,102 @@ static void garner_init(void) {
 
 
 #define CRT_LIMBS 256  // limbs per block, one per thread
-// Coefficients a block can touch: limbs [i0 - 1, i0 + 256) span 257 * 64 bits
-// and a coefficient 93 more, so at most (257 * 64 + 93) / bw + 2 of them;
-// 2070 at the narrowest width, 8 (mul_split.c:151-152).
+// Coefficients a block can touch: limbs [i0 - 2, i0 + 256) span 258 * 64 bits
+// and a coefficient 93 more, so at most (258 * 64 + 93) / bw + 2 of them;
+// 2077 at the narrowest width, 8 (mul_split.c:151-152).
 #define CRT_MAXC 2112
 
 
-// One block per CRT_LIMBS output limbs (block bx of job J).
-__device__ __forceinline__ void crt_pack_body(const FFCrtJob &J, uint32_t bx, const FFGarner &G) {
-    const uint64_t nblk = (J.nout + CRT_LIMBS - 1) / CRT_LIMBS;
-    if (bx >= nblk) return;
-    const int bw = J.bw;
-    const uint64_t i0 = (uint64_t)bx * CRT_LIMBS;
-    const uint64_t i1 = i0 + CRT_LIMBS < J.nout ? i0 + CRT_LIMBS : J.nout;
-    // coefficients touching limbs [i0 - 1, i1): limb i0 - 1 gives the block's first hi
-    const uint64_t lim = J.conv < J.L ? J.conv : J.L;
-    const uint64_t jbase = first_coef(i0 ? i0 - 1 : 0, bw);
+// Limb sums of one product for the workgroup's limbs: thread t holds limbs
+// i0 + t LPT + q (q < LPT), u[q] = lo(T_i) + hi(T_{i-1}), and 2^64 - 1 at
+// and past nout (a carry passes through); with prev, thread 0 also gets u at
+// limb i0 - 1. xlo / xhi / hx: the workgroup's LDS.
+template <int LPT>
+__device__ __forceinline__ void crt_sums(const uint32_t *y, uint32_t L, int bw, uint64_t conv, uint64_t nout,
+                                         uint64_t i0, const FFGarner &G, uint64_t *xlo, uint32_t *xhi, uint64_t *hx,
+                                         bool prev, u128 *u, u128 *uprev) {
+    const int t = threadIdx.x;
+    const uint64_t i1 = i0 + CRT_LIMBS * LPT < nout ? i0 + CRT_LIMBS * LPT : nout;
+    // coefficients touching limbs [i0 - 2, i1)
+    const uint64_t lim = conv < L ? conv : L;
+    const uint64_t jbase = first_coef(i0 >= 2 ? i0 - 2 : 0, bw);
     uint64_t jend = (64 * i1 + bw - 1) / bw;
     if (jend > lim) jend = lim;
-    __shared__ uint64_t xlo[CRT_MAXC];
-    __shared__ uint32_t xhi[CRT_MAXC];
-    __shared__ uint64_t hi_s[CRT_LIMBS];
-    __shared__ uint64_t wave_c[CRT_LIMBS / 64];
-    const uint32_t *__restrict__ y1 = J.y;
-    const uint32_t *__restrict__ y2 = J.y + J.L;
-    const uint32_t *__restrict__ y3 = J.y + 2 * (size_t)J.L;
-    for (uint64_t j = jbase + threadIdx.x; j < jend; j += CRT_LIMBS) {
+    const uint32_t *__restrict__ y1 = y;
+    const uint32_t *__restrict__ y2 = y + L;
+    const uint32_t *__restrict__ y3 = y + 2 * (size_t)L;
+    for (uint64_t j = jbase + t; j < jend; j += CRT_LIMBS) {
         const u128 X = ff_garner(y1[j], y2[j], y3[j], G);
         xlo[j - jbase] = (uint64_t)X;
         xhi[j - jbase] = (uint32_t)(X >> 64);
     }
     __syncthreads();
-    const uint64_t i = i0 + threadIdx.x;
-    const bool live = i < i1;
-    u128 T = live ? limb_sum(i, bw, jbase, xlo, xhi, jbase, jend) : 0;
-    hi_s[threadIdx.x] = (uint64_t)(T >> 64);
+    const uint64_t ib = i0 + (uint64_t)t * LPT;
+    u128 T[LPT];
+#pragma unroll
+    for (int q = 0; q < LPT; q++) T[q] = ib + q < i1 ? limb_sum(ib + q, bw, jbase, xlo, xhi, jbase, jend) : 0;
+    hx[t] = (uint64_t)(T[LPT - 1] >> 64);
     uint64_t hprev = 0;
-    if (threadIdx.x == 0 && i0 > 0) hprev = (uint64_t)(limb_sum(i0 - 1, bw, jbase, xlo, xhi, jbase, jend) >> 64);
-    __syncthreads();
-    if (threadIdx.x > 0) hprev = hi_s[threadIdx.x - 1];
-    const u128 u = (u128)(uint64_t)T + hprev;
-    const uint64_t ulo = (uint64_t)u;
-    const bool g = live && (u >> 64) != 0;
-    const bool pp = live && !g && ulo == ~0ull;
-    // carries inside each wavefront: lane k's (generate, propagate) are bit k
-    // of A = g and B = g | p, so A + B + c_in carries exactly where the limbs
-    // do, and C = (A + B + c_in) ^ A ^ B has bit k = carry into lane k
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t A = __ballot(g), B = A | __ballot(pp);
-    const uint64_t wg = (uint64_t)(((u128)A + B) >> 64);  // carry out of the wave, zero carry in
-    const uint64_t wp = (~B) == 0;                          // every lane propagates (then wg = 0)
-    if (lane == 0) wave_c[wave] = wg | (wp << 1);
-    __syncthreads();
-    uint64_t cin = 0;  // carry into this wave from the earlier waves of the block
-    for (int w = 0; w < wave; w++) {
-        const uint64_t e = wave_c[w];
-        cin = (e & 1) | ((e >> 1) & cin);
-    }
-    const uint64_t C = (A + B + cin) ^ A ^ B;
-    const uint64_t cbit = (C >> lane) & 1;
-    if (live) J.out[i] = ulo + cbit;
-    // the block's generate/propagate pair
-    if (threadIdx.x == 0) {
-        uint64_t Gb = 0, Pb = 1;
-        for (int w = 0; w < CRT_LIMBS / 64; w++) {
-            const uint64_t e = wave_c[w];
-            Gb = (e & 1) | ((e >> 1) & Gb);
-            Pb = Pb & (e >> 1);
+    if (t == 0 && i0 > 0) {
+        const u128 Tm1 = limb_sum(i0 - 1, bw, jbase, xlo, xhi, jbase, jend);
+        hprev = (uint64_t)(Tm1 >> 64);
+        if (prev) {
+            const uint64_t hm2 = i0 >= 2 ? (uint64_t)(limb_sum(i0 - 2, bw, jbase, xlo, xhi, jbase, jend) >> 64) : 0;
+            *uprev = (u128)(uint64_t)Tm1 + hm2;
         }
-        J.bgp[bx] = (uint32_t)(Gb | (Pb << 1));
     }
+    __syncthreads();
+    if (t > 0) hprev = hx[t - 1];
+#pragma unroll
+    for (int q = 0; q < LPT; q++) {
+        u[q] = ib + q < nout ? (u128)(uint64_t)T[q] + hprev : (u128)~0ull;
+        hprev = (uint64_t)(T[q] >> 64);
+    }
+    __syncthreads();  // the LDS is reused by the next call
 }
 
-__global__ __launch_bounds__(CRT_LIMBS) void k_crt_pack(FFCrtSet js, FFGarner G) {
-    crt_pack_body(js.j[blockIdx.y], blockIdx.x, G);
+// One workgroup per CRT_LIMBS * LPT output limbs of job J (bx: its grid
+// index, only bounding the count; the order is the ticket's).
+struct CrtShared {
+    uint64_t xlo[CRT_MAXC];
+    uint32_t xhi[CRT_MAXC];
+    uint64_t hx[CRT_LIMBS];
+    LbShared S;
+};
+
+template <int LPT>
+__device__ __forceinline__ void crt_body(const FFCrtJob &J, uint32_t bx, const FFGarner &G, uint32_t *fault,
+                                         CrtShared &sh) {
+    const uint64_t nblk = (J.nout + CRT_LIMBS * LPT - 1) / (CRT_LIMBS * LPT);
+    if (bx >= nblk) return;
+    uint64_t *xlo = sh.xlo, *hx = sh.hx;
+    uint32_t *xhi = sh.xhi;
+    LbShared &S = sh.S;
+    const uint32_t b = lb_ticket(S, J.lb);
+    const uint64_t i0 = (uint64_t)b * CRT_LIMBS * LPT, ib = i0 + (uint64_t)threadIdx.x * LPT;
+    u128 u[LPT], unused;
+    crt_sums<LPT>(J.y, J.L, J.bw, J.conv, J.nout, i0, G, xlo, xhi, hx, false, u, &unused);
+    bool g, p;
+    lb_group_gp<LPT>(u, &g, &p);
+    const uint32_t c = lb_round<1>(S, J.lb + 2, b, &g, &p, 0, fault);
+    uint64_t v[LPT];
+    lb_group_apply<LPT>(u, c, v);
+#pragma unroll
+    for (int q = 0; q < LPT; q++)
+        if (ib + q < J.nout) J.out[ib + q] = v[q];
+    unsigned long long *const keys[1] = {J.lenkey};
+    lb_keys<1, LPT>(J.lb + 2 + nblk, b, (uint32_t)nblk, i0, J.nout, v, keys, fault);
+    if (lb_last(S, J.lb, (uint32_t)nblk)) lb_clear(J.lb, 3 + nblk);
+}
+
+// Limbs per thread of a job: four where the coefficients are at least 32
+// bits wide (those touching 1026 limbs fit CRT_MAXC), else one.
+static inline int crt_lpt(int bw) { return bw >= 32 ? 4 : 1; }
+
+template <int LPT>
+__global__ __launch_bounds__(CRT_LIMBS) void k_crt(FFCrtSet js, FFGarner G, uint32_t *fault) {
+    __shared__ CrtShared sh;
+    crt_body<LPT>(js.j[blockIdx.y], blockIdx.x, G, fault, sh);
 }
 
 __device__ __forceinline__ int crt_find(const uint32_t *start, int n, uint32_t b) {
@@ -148,132 +169,165 @@ __device__ __forceinline__ int crt_find(const uint32_t *start, int n, uint32_t b
     return lo;
 }
 
-__global__ __launch_bounds__(CRT_LIMBS) void k_crt_pack_list(const FFCrtJob *jobs, const uint32_t *start, int n,
-                                                             FFGarner G) {
+// Work-list form: job k owns blocks [start[k], start[k+1]) of the flat
+// grid; a job's limbs per thread follow from its width.
+__global__ __launch_bounds__(CRT_LIMBS) void k_crt_list(const FFCrtJob *jobs, const uint32_t *start, int n,
+                                                        FFGarner G, uint32_t *fault) {
+    __shared__ CrtShared sh;
     const int k = crt_find(start, n, blockIdx.x);
-    crt_pack_body(jobs[k], blockIdx.x - start[k], G);
+    if (jobs[k].bw >= 32) crt_body<4>(jobs[k], blockIdx.x - start[k], G, fault, sh);
+    else crt_body<1>(jobs[k], blockIdx.x - start[k], G, fault, sh);
 }
 
-// Exact carry into every block, only when some block propagates.
-#define SCAN_THREADS 1024
-
-__device__ __forceinline__ void crt_scan_body(const FFCrtJob &J) {
-    const uint64_t nblk = (J.nout + CRT_LIMBS - 1) / CRT_LIMBS;
-    int mine = 0;
-    for (uint64_t b = threadIdx.x; b < nblk; b += SCAN_THREADS) mine |= (J.bgp[b] >> 1) & 1;
-    const int any = __syncthreads_or(mine);
-    if (threadIdx.x == 0) {
-        *J.mode = (uint32_t)any;
-        *J.lenkey = 0;  // k_crt_apply books the top limb by atomicMax
-    }
-    if (!any) return;  // carry into block w is G(w-1)
-    const uint64_t per = (nblk + SCAN_THREADS - 1) / SCAN_THREADS;
-    const uint64_t b0 = threadIdx.x * per, b1 = b0 + per < nblk ? b0 + per : nblk;
-    uint32_t Gs = 0, Ps = 1;
-    for (uint64_t b = b0; b < b1; b++) {
-        const uint32_t e = J.bgp[b];
-        Gs = (e & 1) | ((e >> 1) & Gs);
-        Ps = Ps & (e >> 1);
-    }
-    __shared__ uint32_t sg[SCAN_THREADS], sp[SCAN_THREADS];
-    sg[threadIdx.x] = Gs;
-    sp[threadIdx.x] = Ps;
-    __syncthreads();
-    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-        const uint32_t g2 = sg[threadIdx.x], p2 = sp[threadIdx.x];
-        uint32_t g1 = 0, p1 = 1;
-        if ((int)threadIdx.x >= off) g1 = sg[threadIdx.x - off], p1 = sp[threadIdx.x - off];
-        __syncthreads();
-        sg[threadIdx.x] = g2 | (p2 & g1);
-        sp[threadIdx.x] = p2 & p1;
-        __syncthreads();
-    }
-    uint32_t c = threadIdx.x ? sg[threadIdx.x - 1] : 0;
-    for (uint64_t b = b0; b < b1; b++) {
-        J.bcin[b] = (uint8_t)c;
-        const uint32_t e = J.bgp[b];
-        c = (e & 1) | ((e >> 1) & c);
-    }
+uint64_t ff_crt_blocks(uint64_t nout, int bw) {
+    const uint64_t per = (uint64_t)CRT_LIMBS * crt_lpt(bw);
+    return (nout + per - 1) / per;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_crt_scan(FFCrtSet js) { crt_scan_body(js.j[blockIdx.x]); }
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_crt_scan_list(const FFCrtJob *jobs) {
-    crt_scan_body(jobs[blockIdx.x]);
-}
-
-// Top nonzero limb of d[i0..i1) booked into the product's length key.
-__device__ __forceinline__ void book_top(const FFCrtJob &J, uint64_t i0, uint64_t i1) {
-    for (uint64_t i = i1; i-- > i0;) {
-        const uint64_t v = J.out[i];
-        if (v) {
-            const unsigned long long key = ((unsigned long long)(i + 1) << 8) | (unsigned)(64 - __clzll((long long)v));
-            atomicMax(J.lenkey, key);
-            return;
-        }
-    }
-}
-
-// Thread b of a job: adds block b's carry-in (ripple bounded by the block)
-// and, for blocks holding limbs from nout - 3 up (where the top nonzero limb
-// of an na x nb product lies), books the top limb.
-__device__ __forceinline__ void crt_apply_body(const FFCrtJob &J, uint64_t b) {
-    const uint64_t nblk = (J.nout + CRT_LIMBS - 1) / CRT_LIMBS;
-    if (b >= nblk) return;
-    const uint32_t c = *J.mode ? J.bcin[b] : (b ? J.bgp[b - 1] & 1 : 0);
-    const uint64_t i0 = b * CRT_LIMBS, i1 = i0 + CRT_LIMBS < J.nout ? i0 + CRT_LIMBS : J.nout;
-    if (c) {
-        for (uint64_t i = i0; i < i1; i++) {
-            if (++J.out[i] != 0) break;
-        }
-    }
-    const uint64_t lo_top = J.nout >= 3 ? J.nout - 3 : 0;
-    if (i1 > lo_top) book_top(J, i0 > lo_top ? i0 : lo_top, i1);
-}
-
-__global__ __launch_bounds__(256) void k_crt_apply(FFCrtSet js) {
-    crt_apply_body(js.j[blockIdx.y], (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-__global__ __launch_bounds__(256) void k_crt_apply_list(const FFCrtJob *jobs, const uint32_t *astart, int n) {
-    const int k = crt_find(astart, n, blockIdx.x);
-    crt_apply_body(jobs[k], (uint64_t)(blockIdx.x - astart[k]) * blockDim.x + threadIdx.x);
-}
-
-// Algorithmic bytes: pack reads 3 residues per coefficient and writes each
-// output limb once; scan and apply touch one word per block of 256 limbs.
+// Algorithmic bytes: 3 residues read per coefficient, each output limb
+// written once.
 extern "C" hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_nout, double conv_total,
                                     double out_limbs_total, hipStream_t st) {
     garner_init();
-    const uint64_t max_blk = (max_nout + CRT_LIMBS - 1) / CRT_LIMBS;
-    dim3 g1((unsigned)max_blk, njobs);
+    // one launch per limbs-per-thread class (the jobs of one group share a width)
+    int lpt = crt_lpt(js->j[0].bw);
+    for (int k = 1; k < njobs; k++)
+        if (crt_lpt(js->j[k].bw) != lpt) lpt = 1;
+    const FFCrtSet &set = *js;
+    const uint64_t max_blk = (max_nout + (uint64_t)CRT_LIMBS * lpt - 1) / ((uint64_t)CRT_LIMBS * lpt);
     ff_prof_begin(FK_CRT_PACK, st);
-    hipLaunchKernelGGL(k_crt_pack, g1, dim3(CRT_LIMBS), 0, st, *js, g_garner);
+    if (lpt == 4)
+        hipLaunchKernelGGL(k_crt<4>, dim3((unsigned)max_blk, njobs), dim3(CRT_LIMBS), 0, st, set, g_garner,
+                           ff_lb_fault());
+    else
+        hipLaunchKernelGGL(k_crt<1>, dim3((unsigned)max_blk, njobs), dim3(CRT_LIMBS), 0, st, set, g_garner,
+                           ff_lb_fault());
     ff_prof_end(FK_CRT_PACK, 12.0 * conv_total + 8.0 * out_limbs_total, st);
-    ff_prof_begin(FK_CRT_SCAN, st);
-    hipLaunchKernelGGL(k_crt_scan, dim3(njobs), dim3(SCAN_THREADS), 0, st, *js);
-    ff_prof_end(FK_CRT_SCAN, 0.0, st);
-    dim3 g3((unsigned)((max_blk + 255) / 256), njobs);
-    ff_prof_begin(FK_CRT_APPLY, st);
-    hipLaunchKernelGGL(k_crt_apply, g3, dim3(256), 0, st, *js);
-    ff_prof_end(FK_CRT_APPLY, 4.0 * out_limbs_total / CRT_LIMBS, st);
     return hipGetLastError();
 }
 
-extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *start, const uint32_t *astart, int n,
-                                         uint32_t blocks, uint32_t ablocks, double conv_total,
-                                         double out_limbs_total, hipStream_t st) {
+extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *start, int n, uint32_t blocks,
+                                         double conv_total, double out_limbs_total, hipStream_t st) {
     if (!n) return hipSuccess;
     garner_init();
     ff_prof_begin(FK_CRT_PACK, st);
-    hipLaunchKernelGGL(k_crt_pack_list, dim3(blocks), dim3(CRT_LIMBS), 0, st, jobs, start, n, g_garner);
+    hipLaunchKernelGGL(k_crt_list, dim3(blocks), dim3(CRT_LIMBS), 0, st, jobs, start, n, g_garner, ff_lb_fault());
     ff_prof_end(FK_CRT_PACK, 12.0 * conv_total + 8.0 * out_limbs_total, st);
-    ff_prof_begin(FK_CRT_SCAN, st);
-    hipLaunchKernelGGL(k_crt_scan_list, dim3(n), dim3(SCAN_THREADS), 0, st, jobs);
-    ff_prof_end(FK_CRT_SCAN, 0.0, st);
-    ff_prof_begin(FK_CRT_APPLY, st);
-    hipLaunchKernelGGL(k_crt_apply_list, dim3(ablocks), dim3(256), 0, st, jobs, astart, n);
-    ff_prof_end(FK_CRT_APPLY, 4.0 * out_limbs_total / CRT_LIMBS, st);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ fused large step
+//
+// A doubling step from the residues of s1 = F(k)^2 and s2 = F(k-1)^2 in one
+// launch (fib.c:36-62): per workgroup of 1024 limbs (four per thread), the
+// CRT of both squares, then A = s1 + s2, E = 4 s1 + cb0, B = E - s2 =
+// F(2k+1), C = B - A = F(2k), every one a binary carry chain resolved by
+// look-back in four rounds: {s1, s2}, {A, E}, {B}, {C}. Limb i of 4 s1 takes
+// the top two bits of limb i - 1 of s1; for the workgroup's first limb they
+// come from its own sums (limb i0 - 1 is summed too), unless the carry into
+// that limb could reach bit 62, in which case from the word the predecessor
+// publishes with its last limb's top bits. cb0 = -2 adds 2^(64 len) - 2 (the
+// carry out of the top is dropped). The squares may instead be given as
+// limbs (split products), skipping the first round.
+#define STEP_LPT 4
+#define STEP_LIMBS (CRT_LIMBS * STEP_LPT)
+
+__global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner G, uint32_t *fault) {
+    const uint64_t nblk = (J.len + STEP_LIMBS - 1) / STEP_LIMBS;
+    if (blockIdx.x >= nblk) return;
+    __shared__ uint64_t xlo[CRT_MAXC];
+    __shared__ uint32_t xhi[CRT_MAXC];
+    __shared__ uint64_t hx[CRT_LIMBS];
+    __shared__ LbShared S;
+    const int t = threadIdx.x;
+    const uint32_t b = lb_ticket(S, J.lb);
+    uint32_t *st = J.lb + 2;                // four rounds of nblk words
+    uint32_t *lastw = J.lb + 2 + 4 * nblk;  // per workgroup: 4 | top two bits of its last s1 limb
+    const uint64_t i0 = (uint64_t)b * STEP_LIMBS, ib = i0 + (uint64_t)t * STEP_LPT;
+    bool live[STEP_LPT];
+#pragma unroll
+    for (int q = 0; q < STEP_LPT; q++) live[q] = ib + q < J.len;
+    bool g[2], p[2];
+    uint32_t c;
+    uint64_t s1[STEP_LPT], s2[STEP_LPT], prev_top;  // prev_top: top two bits of s1 at limb ib - 1
+    if (J.s1l) {
+#pragma unroll
+        for (int q = 0; q < STEP_LPT; q++) {
+            s1[q] = live[q] && ib + q < J.n1 ? J.s1l[ib + q] : 0;
+            s2[q] = live[q] && ib + q < J.n2 ? J.s2l[ib + q] : 0;
+        }
+        prev_top = ib >= 1 && ib - 1 < J.n1 ? J.s1l[ib - 1] >> 62 : 0;
+    } else {
+        u128 u1[STEP_LPT], u2[STEP_LPT], up = 0, unused;
+        crt_sums<STEP_LPT>(J.y1, J.L1, J.bw1, J.conv1, J.len, i0, G, xlo, xhi, hx, true, u1, &up);
+        crt_sums<STEP_LPT>(J.y2, J.L2, J.bw2, J.conv2, J.len, i0, G, xlo, xhi, hx, false, u2, &unused);
+        lb_group_gp<STEP_LPT>(u1, &g[0], &p[0]);
+        lb_group_gp<STEP_LPT>(u2, &g[1], &p[1]);
+        c = lb_round<2>(S, st, b, g, p, 0, fault);
+        lb_group_apply<STEP_LPT>(u1, c & 1, s1);
+        lb_group_apply<STEP_LPT>(u2, (c >> 1) & 1, s2);
+        if (t == CRT_LIMBS - 1) lb_store(lastw + b, 4u | (uint32_t)(s1[STEP_LPT - 1] >> 62));
+        hx[t] = s1[STEP_LPT - 1];
+        uint64_t top0 = 0;
+        if (t == 0 && b > 0) {
+            const uint64_t lo = (uint64_t)up;
+            const uint64_t low62 = (1ull << 62) - 1;
+            top0 = (lo & low62) != low62 ? lo >> 62 : (uint64_t)(lb_wait(lastw + b - 1, fault) & 3);
+        }
+        __syncthreads();
+        prev_top = t ? hx[t - 1] >> 62 : top0;
+    }
+    u128 ua[STEP_LPT], ue[STEP_LPT];
+#pragma unroll
+    for (int q = 0; q < STEP_LPT; q++) {
+        const uint64_t i = ib + q;
+        const uint64_t d = (s1[q] << 2) | (q ? s1[q - 1] >> 62 : prev_top);
+        const uint64_t kv = J.cb0 >= 0 ? (i == 0 ? (uint64_t)J.cb0 : 0) : (i == 0 ? ~(uint64_t)(-J.cb0 - 1) : ~0ull);
+        ua[q] = live[q] ? (u128)s1[q] + s2[q] : (u128)~0ull;
+        ue[q] = live[q] ? (u128)d + kv : (u128)~0ull;
+    }
+    lb_group_gp<STEP_LPT>(ua, &g[0], &p[0]);
+    lb_group_gp<STEP_LPT>(ue, &g[1], &p[1]);
+    c = lb_round<2>(S, st + nblk, b, g, p, 0, fault);
+    uint64_t A[STEP_LPT], E[STEP_LPT], B[STEP_LPT], C[STEP_LPT];
+    lb_group_apply<STEP_LPT>(ua, c & 1, A);
+    lb_group_apply<STEP_LPT>(ue, (c >> 1) & 1, E);
+    u128 ub[STEP_LPT];  // B = E + ~s2 + 1
+#pragma unroll
+    for (int q = 0; q < STEP_LPT; q++) ub[q] = live[q] ? (u128)E[q] + ~s2[q] : (u128)~0ull;
+    lb_group_gp<STEP_LPT>(ub, &g[0], &p[0]);
+    c = lb_round<1>(S, st + 2 * nblk, b, g, p, 1, fault);
+    lb_group_apply<STEP_LPT>(ub, c, B);
+    u128 uc[STEP_LPT];  // C = B + ~A + 1
+#pragma unroll
+    for (int q = 0; q < STEP_LPT; q++) uc[q] = live[q] ? (u128)B[q] + ~A[q] : (u128)~0ull;
+    lb_group_gp<STEP_LPT>(uc, &g[0], &p[0]);
+    c = lb_round<1>(S, st + 3 * nblk, b, g, p, 1, fault);
+    lb_group_apply<STEP_LPT>(uc, c, C);
+    uint64_t out[2 * STEP_LPT];
+#pragma unroll
+    for (int q = 0; q < STEP_LPT; q++) {
+        out[q] = live[q] ? (J.bit ? B[q] : A[q]) : 0;
+        out[STEP_LPT + q] = live[q] ? C[q] : 0;
+        if (live[q]) {
+            J.K[ib + q] = out[q];
+            J.C[ib + q] = out[STEP_LPT + q];
+        }
+    }
+    unsigned long long *const keys[2] = {J.keyK, J.keyC};
+    lb_keys<2, STEP_LPT>(J.lb + 2 + 5 * nblk, b, (uint32_t)nblk, i0, J.len, out, keys, fault);
+    if (lb_last(S, J.lb, (uint32_t)nblk)) lb_clear(J.lb, 3 + 5 * nblk);
+}
+
+size_t ff_step_lb_words(uint64_t len) { return 3 + 5 * ((len + STEP_LIMBS - 1) / STEP_LIMBS); }
+
+extern "C" hipError_t ff_launch_step_fused(const FFStepJob *J, double conv_total, hipStream_t st) {
+    garner_init();
+    if (!J->s1l && (J->bw1 < 32 || J->bw2 < 32)) return hipErrorInvalidValue;  // four limbs per thread need bw >= 32
+    const uint64_t nblk = (J->len + STEP_LIMBS - 1) / STEP_LIMBS;
+    ff_prof_begin(FK_STEP_CRT, st);
+    hipLaunchKernelGGL(k_step_fused, dim3((unsigned)nblk), dim3(CRT_LIMBS), 0, st, *J, g_garner, ff_lb_fault());
+    ff_prof_end(FK_STEP_CRT, 12.0 * conv_total + 16.0 * J->len, st);
     return hipGetLastError();
 }
 

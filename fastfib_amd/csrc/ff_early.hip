@@ -15,7 +15,8 @@
 // (k0 < 1024, at most 12 limbs) and passes each step's sizes, predicted
 // from the closed form; every step books its new pair's length keys, which
 // the host checks against the predictions at the end of F(n) like those of
-// the other steps. The transforms are power-of-2 lengths only: a step
+// the other steps. The root tables' first 2^10 entries are copied into LDS
+// once. The transforms are power-of-2 lengths only: a step
 // never needs the 3 * 2^k lengths to fit in LDS, and the product does not
 // depend on the length.
 #include "ff_crt_dev.h"
@@ -26,7 +27,14 @@
 #define EARLY_L (1 << FF_EARLY_LG)
 #define EARLY_TILE (2 * EARLY_L + (2 * EARLY_L >> 4) + 16)
 
+// The rounds read the root tables (their first 2^10 entries) from LDS: one
+// workgroup cannot hide the latency of table reads from HBM.
+struct FFLdsTables : FFNoTw {
+    static constexpr bool lds_tables = true;
+};
+
 struct EarlyShared {
+    uint32_t tw[3][4][EARLY_L];       // per prime: rt, rtp, irt, irtp
     uint64_t num[2][FF_EARLY_LIMBS];  // F(k - 1), F(k)
     uint32_t tile[3][EARLY_TILE];     // per prime: F(k), F(k - 1) stacked
     uint32_t y[2][3 * EARLY_L];       // residues of F(k)^2, F(k-1)^2: [prime][coefficient]
@@ -39,6 +47,13 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
     for (int i = t; i < FF_EARLY_LIMBS; i += EARLY_T) {
         E.num[0][i] = i < FF_EARLY_F0 ? a.f[0][i] : 0;
         E.num[1][i] = i < FF_EARLY_F0 ? a.f[1][i] : 0;
+    }
+    for (int i = t; i < 3 * EARLY_L; i += EARLY_T) {
+        const int q = i / EARLY_L, j = i - q * EARLY_L;
+        E.tw[q][0][j] = T->rt[q][j];
+        E.tw[q][1][j] = T->rtp[q][j];
+        E.tw[q][2][j] = T->irt[q][j];
+        E.tw[q][3][j] = T->irtp[q][j];
     }
     if (t < 2 * a.nsteps) atomicExch(a.keys + t, 0ull);
     __threadfence();
@@ -63,7 +78,8 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
         __syncthreads();
         auto ld = [&](int pos, int c) -> uint32_t { return tl.s[ff_pos_addr(tl, pos, c)]; };
         auto sto = [&](int pos, int c, uint32_t v) { tl.s[ff_pos_addr(tl, pos, c)] = v; };
-        ff_conv_io(tl, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], nullptr, p, pinv, false, false, ld, sto);
+        ff_conv_io(tl, E.tw[pi][0], E.tw[pi][1], E.tw[pi][2], E.tw[pi][3], nullptr, p, pinv, false, false, ld, sto,
+                   FFLdsTables{});
         __syncthreads();
         const uint32_t sc = a.sc[lg][pi];
         for (int i = t - tid0; i < 2 * L; i += nthr)

@@ -3,7 +3,10 @@
 #pragma once
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/fastfib_amd.h"
+#include "ff_internal.h"
 
 // A product's transform: width bw, length L = 2^lg or 3 * 2^lg (r3), conv
 // coefficients; split = 1 when no single transform fits (L > 2^25).
@@ -46,7 +49,10 @@ void ff_mul_split(const MulItem &it);
 
 // Runs up to FF_MAXB products sharing one shape and schedule, one launch
 // per pass (transform arrays come from the memory pool, stream-ordered).
-void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched &sc);
+// With hand: no CRT; hand[i] gets product i's residues, whose buffers are
+// appended to keep for the caller to free after it has queued their use.
+void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched &sc, FFCrtJob *hand = nullptr,
+                  std::vector<void *> *keep = nullptr);
 
 // Effective schedule knobs (0 in sc: the global options).
 int ff_sched_inner_log(const FFSched &sc);

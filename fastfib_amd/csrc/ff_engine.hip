@@ -128,10 +128,18 @@ static int slot_new(void) {
     return g_slot_used++;
 }
 
+// Reads n length keys back (waiting for the stream), and with them the
+// look-back time-out word, which must still be clear.
 static void slots_read(int s0, int n, uint64_t *out) {
-    uint64_t *h = ff_pinned_results((size_t)n);
+    uint64_t *h = ff_pinned_results((size_t)n + 1);
+    h[n] = 0;
     FF_CHECK(hipMemcpyAsync(h, g_slots + s0, (size_t)n * 8, hipMemcpyDeviceToHost, g_st));
+    FF_CHECK(hipMemcpyAsync(h + n, ff_lb_fault(), 4, hipMemcpyDeviceToHost, g_st));
     FF_CHECK(hipStreamSynchronize(g_st));
+    if ((uint32_t)h[n]) {
+        fprintf(stderr, "fastfib_amd: a carry look-back wait timed out\n");
+        exit(1);
+    }
     memcpy(out, h, (size_t)n * 8);
 }
 
@@ -183,7 +191,11 @@ int ff_sched_inner_log(const FFSched &sc) { return sc.inner_log ? sc.inner_log :
 int ff_sched_tile_words(const FFSched &sc) { return sc.tile_words ? sc.tile_words : g_tile_words; }
 
 // Runs up to FF_MAXB products that share one schedule, one launch per pass.
-void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched &sc) {
+// With hand, the CRT is not launched: hand[i] receives product i's residues
+// ([3][L] words, CRT-ready) and the buffers holding them are appended to
+// keep, for the caller to free after its own use.
+void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched &sc, FFCrtJob *hand,
+                  std::vector<void *> *keep) {
     FFPlan pl;
     if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, ff_sched_inner_log(sc),
                       ff_sched_tile_words(sc)) != 0) {
@@ -199,14 +211,11 @@ void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched 
     uint64_t max_nout = 0;
     double opbytes = 0, conv_total = 0, out_limbs = 0;
     int ngeneral = 0;
+    if (!hand) ff_lb_begin();
     for (int i = 0; i < cnt; i++) {
         const uint64_t nout = it[i].na + (it[i].b ? it[i].nb : it[i].na) + 1;
-        const uint64_t nblk = (nout + FF_CRT_LIMBS - 1) / FF_CRT_LIMBS;
         const size_t o_fb = align256(words * 4);
-        const size_t o_bgp = o_fb + (it[i].b ? align256(words * 4) : 0);
-        const size_t o_bcin = o_bgp + align256(nblk * 4);
-        const size_t o_mode = o_bcin + align256(nblk);
-        char *base = (char *)ff_dalloc(o_mode + 256);
+        char *base = (char *)ff_dalloc(o_fb + (it[i].b ? align256(words * 4) : 0) + 256);
         tmp.push_back(base);
         uint32_t *fa = (uint32_t *)base;
         uint32_t *fb = it[i].b ? (uint32_t *)(base + o_fb) : nullptr;
@@ -220,16 +229,8 @@ void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched 
         cv.c[i] = FFConv{asrc, bsrc, (uint32_t)(2 * it[i].na), (uint32_t)(2 * it[i].nb), fa, fb};
         inv.x[i] = FFXform{nullptr, 0, fa};
         max_nout = nout > max_nout ? nout : max_nout;
-        crt.j[i] = FFCrtJob{fa,
-                            it[i].out,
-                            (uint32_t *)(base + o_bgp),
-                            (uint8_t *)(base + o_bcin),
-                            (uint32_t *)(base + o_mode),
-                            it[i].lenkey,
-                            nout,
-                            it[i].conv,
-                            (uint32_t)s.L,
-                            (int)s.bw};
+        crt.j[i] = FFCrtJob{fa,       it[i].out, hand ? nullptr : ff_lb_take(ff_crt_lb_words(nout)),
+                            it[i].lenkey, nout,   it[i].conv, (uint32_t)s.L, (int)s.bw};
         opbytes += 8.0 * (it[i].na + (it[i].b ? it[i].nb : 0));
         ngeneral += it[i].b ? 1 : 0;
         conv_total += (double)it[i].conv;
@@ -241,6 +242,11 @@ void ff_mul_group(const MulItem *it, int cnt, const ProdShape &s, const FFSched 
     }
     FF_CHECK(ff_launch_conv(&pl, &cv, cnt, ngeneral, opbytes, g_st));
     FF_CHECK(ff_launch_inverse(&pl, &inv, cnt, g_st));
+    if (hand) {
+        for (int i = 0; i < cnt; i++) hand[i] = crt.j[i];
+        keep->insert(keep->end(), tmp.begin(), tmp.end());
+        return;
+    }
     FF_CHECK(ff_launch_crt(&crt, cnt, max_nout, conv_total, out_limbs, g_st));
     for (void *p : tmp) ff_dfree(p);
 }
@@ -329,28 +335,22 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
 
 // ------------------------------------------------------------------ fast doubling
 
-// One linear step (fib.c:78-109): from s1 = F(k)^2, s2 = F(k-1)^2 makes
-// (lo, hi) = bit ? (F(2k), F(2k+1)) : (F(2k-1), F(2k)).
-static void fib_step(const DNum &s1, const DNum &s2, int k_odd, int bit, DNum *lo, DNum *hi, const uint64_t *pred) {
-    const uint64_t len = s1.n + 1;  // s2 <= s1, and each result is < 4 s1 + 2
+// One linear step (fib.c:78-109) in one launch from s1 = F(k)^2 and
+// s2 = F(k-1)^2, given in J as residues or limbs: (lo, hi) = bit ?
+// (F(2k), F(2k+1)) : (F(2k-1), F(2k)) on len limbs.
+static void step_launch(FFStepJob &J, uint64_t len, int k_odd, int bit, DNum *lo, DNum *hi, const uint64_t *pred,
+                        double conv_total) {
     DNum K = dnum_alloc(len), C = dnum_alloc(len);
     K.slot = slot_new();
     C.slot = slot_new();
-    const uint64_t nch = (len + FF_CH - 1) / FF_CH;
-    const uint64_t nwg = (nch + FF_LIN_THREADS - 1) / FF_LIN_THREADS;
-    char *scr = (char *)ff_dalloc(align256(2 * nch * sizeof(FFCarrySum)) + align256(2 * nch) + align256(nwg * 4) + 256);
-    FFCarrySum *sums = (FFCarrySum *)scr;
-    int8_t *cin = (int8_t *)(scr + align256(2 * nch * sizeof(FFCarrySum)));
-    uint32_t *wgflag = (uint32_t *)((char *)cin + align256(2 * nch));
-    uint32_t *mode = (uint32_t *)((char *)wgflag + align256(nwg * 4));
-    FFStepArgs a{s1.d, s2.d, s1.n, s2.n, len, bit ? nullptr : K.d, bit ? K.d : nullptr, C.d, k_odd ? -2 : 2, sums,
-                 wgflag};
-    FF_CHECK(ff_launch_step(&a, g_st));
-    const uint64_t lo_top = len >= 3 ? len - 3 : 0;
-    FFCarryStreams cs{{K.d, C.d}, len, nch, 2, sums, cin, wgflag, mode, {g_slots + K.slot, g_slots + C.slot},
-                      {lo_top, lo_top}};
-    FF_CHECK(ff_launch_carry(&cs, g_st));
-    ff_dfree(scr);
+    J.len = len;
+    J.cb0 = k_odd ? -2 : 2;
+    J.bit = bit;
+    J.K = K.d, J.C = C.d;
+    J.keyK = g_slots + K.slot, J.keyC = g_slots + C.slot;
+    ff_lb_begin();
+    J.lb = ff_lb_take(ff_step_lb_words(len));
+    FF_CHECK(ff_launch_step_fused(&J, conv_total, g_st));
     if (pred) {
         // pred[0], pred[1]: bit lengths of (lo, hi)
         const uint64_t bk = bit ? pred[1] : pred[0], bc = bit ? pred[0] : pred[1];
@@ -364,6 +364,62 @@ static void fib_step(const DNum &s1, const DNum &s2, int k_odd, int bit, DNum *l
     }
     if (bit) *lo = C, *hi = K;
     else *lo = K, *hi = C;
+}
+
+// A doubling step from (F(k-1), F(k)): the two squarings' transforms (one
+// launch per pass when their shapes and schedules agree), then the CRT of
+// both squares fused with the linear step (k_step_fused). Squares past the
+// longest transform (split products) or of zero go through mul_many and the
+// step reads their limbs. ps: predicted bits of the squares; pl: of the new
+// pair; given: the squares' schedules.
+static void square_step(const DNum &fk, const DNum &fkm1, int k_odd, int bit, DNum *lo, DNum *hi,
+                        const uint64_t *ps, const uint64_t *pl, const FFSched *given) {
+    ProdShape sh[2];
+    sh[0] = product_shape(fk.bits, fk.bits);
+    sh[1] = fkm1.bits ? product_shape(fkm1.bits, fkm1.bits) : sh[0];
+    if (sh[0].split || sh[1].split || fk.bits == 0 || fkm1.bits == 0) {
+        DNum sq[2];
+        const DNum *A[2] = {&fk, &fkm1}, *B[2] = {nullptr, nullptr};
+        mul_many(A, B, sq, 2, ps, given);
+        FFStepJob J{};
+        J.s1l = sq[0].d, J.s2l = sq[1].d;
+        J.n1 = sq[0].n, J.n2 = sq[1].n;
+        step_launch(J, sq[0].n + 1, k_odd, bit, lo, hi, pl, 0.0);
+        dnum_free(sq[0]);
+        dnum_free(sq[1]);
+        return;
+    }
+    FFSched sc[2] = {FFSched{0, 0, 0}, FFSched{0, 0, 0}};
+    const DNum *x[2] = {&fk, &fkm1};
+    for (int j = 0; j < 2; j++) {
+        if (given) {
+            sc[j] = given[j];
+        } else if (g_dispatch && !g_force_len) {
+            const uint64_t bb = x[j]->bits;
+            const uint8_t sq1 = 1;
+            ff_pick_scheds(&sh[j], &bb, &bb, &sq1, 1, 2, g_dispatch, &sc[j]);
+        }
+        sh[j] = ff_shape_with(sh[j], sc[j]);
+    }
+    MulItem it[2];
+    for (int j = 0; j < 2; j++)
+        it[j] = MulItem{x[j]->d, x[j]->n, x[j]->bits, nullptr, x[j]->n, x[j]->bits, nullptr, sh[j].conv, nullptr};
+    FFCrtJob hand[2];
+    std::vector<void *> keep;
+    if (sh[1].L == sh[0].L && sh[1].bw == sh[0].bw && !memcmp(&sc[0], &sc[1], sizeof(FFSched))) {
+        ff_mul_group(it, 2, sh[0], sc[0], hand, &keep);
+    } else {
+        ff_mul_group(&it[0], 1, sh[0], sc[0], &hand[0], &keep);
+        ff_mul_group(&it[1], 1, sh[1], sc[1], &hand[1], &keep);
+    }
+    FFStepJob J{};
+    J.y1 = hand[0].y, J.L1 = hand[0].L, J.bw1 = hand[0].bw, J.conv1 = hand[0].conv;
+    J.y2 = hand[1].y, J.L2 = hand[1].L, J.bw2 = hand[1].bw, J.conv2 = hand[1].conv;
+    // limbs of s1 plus one (with predicted lengths); else a bound that
+    // keeps the results' top limbs among the last three
+    const uint64_t len = ps ? (ps[0] + 63) / 64 + 1 : 2 * fk.n + 1;
+    step_launch(J, len, k_odd, bit, lo, hi, pl, (double)(sh[0].conv + sh[1].conv));
+    for (void *p : keep) ff_dfree(p);
 }
 
 // A step whose squarings are one-block transforms (L <= 4096), with
@@ -437,7 +493,7 @@ static void small_step(const DNum &fk, const DNum &fkm1, int k_odd, int bit, DNu
     else *lo = K, *hi = C;
 }
 
-// X = 2u + v and, if Y, Y = 2u - v (fib.c:145-161).
+// X = 2u + v and, if Y, Y = 2u - v (fib.c:145-161), in one launch.
 static void last_factors(const DNum &u, const DNum &v, DNum *X, DNum *Y, const uint64_t *pred) {
     const uint64_t len = (u.n > v.n ? u.n : v.n) + 1;
     *X = dnum_alloc(len);
@@ -446,20 +502,10 @@ static void last_factors(const DNum &u, const DNum &v, DNum *X, DNum *Y, const u
         *Y = dnum_alloc(len);
         Y->slot = slot_new();
     }
-    const uint64_t nch = (len + FF_CH - 1) / FF_CH;
-    const uint64_t nwg = (nch + FF_LIN_THREADS - 1) / FF_LIN_THREADS;
-    char *scr = (char *)ff_dalloc(align256(2 * nch * sizeof(FFCarrySum)) + align256(2 * nch) + align256(nwg * 4) + 256);
-    FFCarrySum *sums = (FFCarrySum *)scr;
-    int8_t *cin = (int8_t *)(scr + align256(2 * nch * sizeof(FFCarrySum)));
-    uint32_t *wgflag = (uint32_t *)((char *)cin + align256(2 * nch));
-    uint32_t *mode = (uint32_t *)((char *)wgflag + align256(nwg * 4));
-    FFLastArgs a{u.d, v.d, u.n, v.n, len, X->d, Y ? Y->d : nullptr, sums, wgflag};
+    ff_lb_begin();
+    FFLastArgs a{u.d,        v.d, u.n, v.n, len, X->d, Y ? Y->d : nullptr, g_slots + X->slot,
+                 Y ? g_slots + Y->slot : nullptr, ff_lb_take(ff_last_lb_words(len))};
     FF_CHECK(ff_launch_last(&a, g_st));
-    const uint64_t lo_top = len >= 3 ? len - 3 : 0;
-    FFCarryStreams cs{{X->d, Y ? Y->d : nullptr}, len, nch, Y ? 2 : 1, sums, cin, wgflag, mode,
-                      {g_slots + X->slot, Y ? g_slots + Y->slot : nullptr}, {lo_top, lo_top}};
-    FF_CHECK(ff_launch_carry(&cs, g_st));
-    ff_dfree(scr);
     if (pred) {
         X->bits = pred[0], X->n = (pred[0] + 63) / 64;
         if (Y) Y->bits = pred[1], Y->n = (pred[1] + 63) / 64;
@@ -731,27 +777,22 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
             k_odd = bit;
             continue;
         }
-        DNum sq[2];
-        const DNum *A[2] = {&fk, &fkm1}, *B[2] = {nullptr, nullptr};
-        uint64_t ps[2] = {0, 0};
-        if (predict) ps[0] = P(k, FB_F2), ps[1] = P(k - 1, FB_F2);
-        mul_many(A, B, sq, 2, predict ? ps : nullptr, ds.empty() ? nullptr : &ds[2 * step]);
+        const int bit = (int)((n >> i) & 1);
+        uint64_t ps[2] = {0, 0}, pl[2] = {0, 0};
         if (predict) {
-            checks.push_back({sq[0].slot, ps[0]});
-            if (k > 1) checks.push_back({sq[1].slot, ps[1]});
+            ps[0] = P(k, FB_F2), ps[1] = P(k - 1, FB_F2);
+            pl[0] = P(2 * k - 1 + bit, FB_F), pl[1] = P(2 * k + bit, FB_F);
+        }
+        DNum nlo, nhi;
+        square_step(fk, fkm1, k_odd, bit, &nlo, &nhi, predict ? ps : nullptr, predict ? pl : nullptr,
+                    ds.empty() ? nullptr : &ds[2 * step]);
+        if (predict) {
+            checks.push_back({nlo.slot, pl[0]});
+            checks.push_back({nhi.slot, pl[1]});
         }
         dnum_free(fk);
         dnum_free(fkm1);
-        const int bit = (int)((n >> i) & 1);
-        uint64_t pl[2] = {0, 0};
-        if (predict) pl[0] = P(2 * k - 1 + bit, FB_F), pl[1] = P(2 * k + bit, FB_F);
-        fib_step(sq[0], sq[1], k_odd, bit, &fkm1, &fk, predict ? pl : nullptr);
-        if (predict) {
-            checks.push_back({fkm1.slot, pl[0]});
-            checks.push_back({fk.slot, pl[1]});
-        }
-        dnum_free(sq[0]);
-        dnum_free(sq[1]);
+        fkm1 = nlo, fk = nhi;
         k = 2 * k + bit;
         k_odd = bit;
     }
@@ -777,8 +818,7 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         dnum_free(x);
         dnum_free(y);
         const uint64_t len = result.n + 1 <= result.cap ? result.n + 1 : result.cap;
-        FF_CHECK(ff_launch_add_small(result.d, len, k_odd ? -2 : 2, g_slots + result.slot,
-                                     len >= 3 ? len - 3 : 0, g_st));
+        FF_CHECK(ff_launch_add_small(result.d, len, k_odd ? -2 : 2, g_slots + result.slot, g_st));
         if (predict) {
             checks.push_back({result.slot, pr});
         } else {
@@ -913,7 +953,10 @@ extern "C" void ff_fib_step_linear(const Big *s1, const Big *s2, int k_odd, int 
     ff_ensure_init();
     slots_reset(16);
     DNum S1 = upload(s1->d, s1->n), S2 = upload(s2->d, s2->n), L, H;
-    fib_step(S1, S2, k_odd, bit, &L, &H, nullptr);
+    FFStepJob J{};
+    J.s1l = S1.d, J.s2l = S2.d;
+    J.n1 = S1.n, J.n2 = S2.n;
+    step_launch(J, S1.n + 1, k_odd, bit, &L, &H, nullptr, 0.0);
     *lo = download(L);
     *hi = download(H);
     dnum_free(S1), dnum_free(S2), dnum_free(L), dnum_free(H);

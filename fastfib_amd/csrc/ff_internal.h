@@ -84,10 +84,8 @@ struct FFPlan {
 struct FFCrtJob {
     const uint32_t *y;       // residues [3][L]
     uint64_t *out;           // nout limbs
-    uint32_t *bgp;           // per block of limbs: generate | propagate << 1
-    uint8_t *bcin;           // per block: exact carry in (written only when some block propagates)
-    uint32_t *mode;          // written by the scan: 1 = exact carries in bcin, 0 = generate of the block below
-    unsigned long long *lenkey;  // (top nonzero limb + 1) << 8 | its bit length, by atomicMax
+    uint32_t *lb;            // look-back workspace: 2 + one word per block of 256 limbs, zeroed
+    unsigned long long *lenkey;  // (top nonzero limb + 1) << 8 | its bit length
     uint64_t nout;
     uint64_t conv;
     uint32_t L;
@@ -112,7 +110,7 @@ struct FFInnerItem {
 // Kernel kinds for the live per-launch timer (ff_prof_*).
 enum FFKernelKind {
     FK_FWD_TOP, FK_FWD, FK_INNER, FK_SMALL, FK_INV, FK_INV_TOP,
-    FK_CRT_PACK, FK_CRT_SCAN, FK_CRT_APPLY, FK_STEP, FK_CARRY, FK_NORM, FK_LINEAR, FK_DEC_LEAF, FK_EARLY, FK_COUNT
+    FK_CRT_PACK, FK_CRT_SCAN, FK_CRT_APPLY, FK_STEP, FK_CARRY, FK_NORM, FK_LINEAR, FK_DEC_LEAF, FK_EARLY, FK_STEP_CRT, FK_COUNT
 };
 
 extern "C" {
@@ -186,11 +184,32 @@ size_t ff_inner_lds_bytes(const FFInner *in);
 // ff_crt.hip
 hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_nout, double conv_total,
                          double out_limbs_total, hipStream_t st);
-// start[k]: first pack block of job k (blocks of 256 limbs); astart[k]:
-// first apply block (256 limb-blocks per apply block).
-hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *start, const uint32_t *astart, int n,
-                              uint32_t blocks, uint32_t ablocks, double conv_total, double out_limbs_total,
-                              hipStream_t st);
+// start[k]: first block of job k (blocks of 256 limbs), blocks in all.
+hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *start, int n, uint32_t blocks,
+                              double conv_total, double out_limbs_total, hipStream_t st);
+// Look-back words of a job of nout limbs; its workgroups in a work list.
+static inline size_t ff_crt_lb_words(uint64_t nout) { return 3 + (nout + 255) / 256; }
+uint64_t ff_crt_blocks(uint64_t nout, int bw);
+
+// A doubling step after its two squarings (k_step_fused): from the residues
+// (or the limbs) of s1 = F(k)^2 and s2 = F(k-1)^2, K = bit ? F(2k+1) :
+// F(2k-1) and C = F(2k) over len limbs, with their length keys.
+struct FFStepJob {
+    const uint32_t *y1, *y2;  // residues [3][L] of s1 and s2
+    uint32_t L1, L2;
+    int bw1, bw2;
+    uint64_t conv1, conv2;
+    const uint64_t *s1l, *s2l;  // or, when s1l is set, s1 and s2 as limbs (n1, n2 of them)
+    uint64_t n1, n2;
+    uint64_t len;             // limbs of K and C: n(s1) + 1
+    int cb0;                  // 2 (-1)^k
+    int bit;
+    uint64_t *K, *C;
+    unsigned long long *keyK, *keyC;
+    uint32_t *lb;             // look-back workspace, ff_step_lb_words(len) zeroed words
+};
+size_t ff_step_lb_words(uint64_t len);
+hipError_t ff_launch_step_fused(const FFStepJob *J, double conv_total, hipStream_t st);
 }
 
 #define FF_CHECK(expr)                                                                      \

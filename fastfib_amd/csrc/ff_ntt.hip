@@ -126,6 +126,7 @@ extern "C" int ff_tables_init(void) {
 template <bool INV>
 struct FFOuterTw {
     static constexpr bool on = true;
+    static constexpr bool lds_tables = false;
     const FFTables *T;
     int pi, rho, r3;
     bool top;

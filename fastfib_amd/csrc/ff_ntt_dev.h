@@ -20,25 +20,28 @@
 // are products with the 16th roots w_16^m = rt[8 + m] (companions at
 // rtp[8 + m]), the same for every group of the prime: one gathered load
 // per level instead of 2^lb, at the price of cheap multiplies.
-template <int LB>
+// LDS: the tables were copied into LDS (rt points there).
+template <int LB, bool LDS>
 __device__ __forceinline__ void ff_group_twiddles(uint32_t tw[1 << LB], uint32_t twp[1 << LB], int h, int low,
                                                   const uint32_t *rt, const uint32_t *rtp, uint32_t p,
                                                   uint32_t pinv) {
     tw[0] = rt[h + low];
     twp[0] = rtp[h + low];
-    // the 16th roots sit at the same address for every lane: read through
-    // the constant address space, so they arrive by scalar loads
+    // the 16th roots sit at the same address for every lane: from HBM they
+    // are read through the constant address space, so they arrive by scalar
+    // loads
     typedef const __attribute__((address_space(4))) uint32_t *cptr;
-    const cptr crt = (cptr)rt, crtp = (cptr)rtp;
 #pragma unroll
     for (int k = 1; k < (1 << LB); k++) {
         const int m = k << (3 - LB);  // w_(2^(LB+1))^k = w_16^(k 2^(3-LB))
-        tw[k] = ff_mont_w(tw[0], crt[8 + m], crtp[8 + m], p);
+        const uint32_t r = LDS ? rt[8 + m] : ((cptr)rt)[8 + m];
+        const uint32_t rp = LDS ? rtp[8 + m] : ((cptr)rtp)[8 + m];
+        tw[k] = ff_mont_w(tw[0], r, rp, p);
         twp[k] = tw[k] * pinv;
     }
 }
 
-template <int NL>
+template <int NL, bool LDS = false>
 __device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, const uint32_t *rt, const uint32_t *rtp,
                                              uint32_t p, uint32_t pinv) {
 #pragma unroll
@@ -57,10 +60,10 @@ __device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, con
             continue;
         }
         uint32_t tw[8], twp[8];
-        if (lb == 0) ff_group_twiddles<0>(tw, twp, h, low, rt, rtp, p, pinv);
-        else if (lb == 1) ff_group_twiddles<1>(tw, twp, h, low, rt, rtp, p, pinv);
-        else if (lb == 2) ff_group_twiddles<2>(tw, twp, h, low, rt, rtp, p, pinv);
-        else ff_group_twiddles<3>(tw, twp, h, low, rt, rtp, p, pinv);
+        if (lb == 0) ff_group_twiddles<0, LDS>(tw, twp, h, low, rt, rtp, p, pinv);
+        else if (lb == 1) ff_group_twiddles<1, LDS>(tw, twp, h, low, rt, rtp, p, pinv);
+        else if (lb == 2) ff_group_twiddles<2, LDS>(tw, twp, h, low, rt, rtp, p, pinv);
+        else ff_group_twiddles<3, LDS>(tw, twp, h, low, rt, rtp, p, pinv);
 #pragma unroll
         for (int j = 0; j < (1 << NL); j++) {
             if (j & (1 << lb)) continue;
@@ -73,7 +76,7 @@ __device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, con
     }
 }
 
-template <int NL>
+template <int NL, bool LDS = false>
 __device__ __forceinline__ void ff_dit_group(uint32_t *v, int low, int b_lo, const uint32_t *irt, const uint32_t *irtp,
                                              uint32_t p, uint32_t pinv) {
 #pragma unroll
@@ -82,10 +85,10 @@ __device__ __forceinline__ void ff_dit_group(uint32_t *v, int low, int b_lo, con
         const int h = 1 << b;
         uint32_t tw[8], twp[8];
         if (b != 0) {
-            if (lb == 0) ff_group_twiddles<0>(tw, twp, h, low, irt, irtp, p, pinv);
-            else if (lb == 1) ff_group_twiddles<1>(tw, twp, h, low, irt, irtp, p, pinv);
-            else if (lb == 2) ff_group_twiddles<2>(tw, twp, h, low, irt, irtp, p, pinv);
-            else ff_group_twiddles<3>(tw, twp, h, low, irt, irtp, p, pinv);
+            if (lb == 0) ff_group_twiddles<0, LDS>(tw, twp, h, low, irt, irtp, p, pinv);
+            else if (lb == 1) ff_group_twiddles<1, LDS>(tw, twp, h, low, irt, irtp, p, pinv);
+            else if (lb == 2) ff_group_twiddles<2, LDS>(tw, twp, h, low, irt, irtp, p, pinv);
+            else ff_group_twiddles<3, LDS>(tw, twp, h, low, irt, irtp, p, pinv);
         }
 #pragma unroll
         for (int j = 0; j < (1 << NL); j++) {
@@ -148,6 +151,7 @@ enum { FF_R_DIF = 0, FF_R_DIT = 1, FF_R_MID = 2 };
 // group instead of one per element. FFNoTw: no twiddles (inner blocks).
 struct FFNoTw {
     static constexpr bool on = false;
+    static constexpr bool lds_tables = false;  // the root tables read from HBM
     template <int NL>
     __device__ void group(int, int, uint32_t *) const {}
     __device__ uint32_t one(int, int) const { return 0; }
@@ -202,17 +206,17 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
             for (int j = 0; j < G; j++) v[j] = ff_mont(v[j], tw[j], p, pinv);
         }
         if (KIND == FF_R_DIF) {
-            ff_dif_group<NL>(v, low, b_lo, rt, rtp, p, pinv);
+            ff_dif_group<NL, Gt::lds_tables>(v, low, b_lo, rt, rtp, p, pinv);
         } else if (KIND == FF_R_DIT) {
-            ff_dit_group<NL>(v, low, b_lo, irt, irtp, p, pinv);
+            ff_dit_group<NL, Gt::lds_tables>(v, low, b_lo, irt, irtp, p, pinv);
         } else {
-            ff_dif_group<NL>(v, low, b_lo, rt, rtp, p, pinv);
+            ff_dif_group<NL, Gt::lds_tables>(v, low, b_lo, rt, rtp, p, pinv);
 #pragma unroll
             for (int j = 0; j < G; j++) {
                 const int pos = pos0 + (base | (j << b_lo));
                 v[j] = ff_mont(v[j], B ? B[ff_pos_addr(t, pos, c)] : v[j], p, pinv);
             }
-            ff_dit_group<NL>(v, low, b_lo, irt, irtp, p, pinv);
+            ff_dit_group<NL, Gt::lds_tables>(v, low, b_lo, irt, irtp, p, pinv);
         }
         if constexpr (Gt::on && KIND == FF_R_DIF && TO_G) {
             uint32_t tw[G];
@@ -314,25 +318,25 @@ __device__ void ff_inverse_io(const FFTile &t, const uint32_t *irt, const uint32
 // Whole convolution of one LDS-resident block: forward of A (first round
 // through ld when from_g), pointwise with B (LDS, already transformed) or a
 // square, inverse, the last round through st when to_g.
-template <class Ld, class St>
+template <class Ld, class St, class Gt = FFNoTw>
 __device__ void ff_conv_io(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, const uint32_t *irt,
                            const uint32_t *irtp, const uint32_t *B, uint32_t p, uint32_t pinv, bool from_g, bool to_g,
-                           Ld ld, St st) {
+                           Ld ld, St st, const Gt &gt = Gt()) {
     const int R = ff_nrounds(t.rho);
     int b = t.rho;
     for (int r = 0; r < R - 1; r++) {
         b -= 4;
         if (r) __syncthreads();
-        ff_round_any<FF_R_DIF>(4, from_g && r == 0, false, t, b, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, st);
+        ff_round_any<FF_R_DIF>(4, from_g && r == 0, false, t, b, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, st, gt);
     }
     const int nl0 = ff_last_nl(t.rho);
     if (R > 1) __syncthreads();
-    ff_round_any<FF_R_MID>(nl0, from_g && R == 1, to_g && R == 1, t, 0, rt, rtp, irt, irtp, B, p, pinv, ld, st);
+    ff_round_any<FF_R_MID>(nl0, from_g && R == 1, to_g && R == 1, t, 0, rt, rtp, irt, irtp, B, p, pinv, ld, st, gt);
     b = nl0;
     for (int r = 1; r < R; r++) {
         __syncthreads();
         ff_round_any<FF_R_DIT>(4, false, to_g && r == R - 1, t, b, nullptr, nullptr, irt, irtp, nullptr, p, pinv, ld,
-                               st);
+                               st, gt);
         b += 4;
     }
 }

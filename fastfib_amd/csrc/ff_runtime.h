@@ -13,8 +13,8 @@ extern int g_inner_log;
 extern int g_tile_words;
 extern uint64_t g_force_len;
 extern int g_max_lg;    // longest single transform 2^max_lg; longer products split
-extern int g_dispatch;
-extern int g_dispatch_gen;  // bumped whenever the table or the network changes  // schedule source of the drop-in products and F(n): 0, FF_BATCH_TUNED or FF_BATCH_LEARNED
+extern int g_dispatch;      // schedule source of the drop-in products and F(n): 0, FF_BATCH_TUNED or FF_BATCH_LEARNED
+extern int g_dispatch_gen;  // bumped whenever the table or the network changes
 
 void ff_set_err(const char *fmt, ...);
 void ff_ensure_init(void);
@@ -32,3 +32,11 @@ void ff_prof_pause(int pause);
 
 // Pinned host buffer for reading small results back.
 uint64_t *ff_pinned_results(size_t words);
+
+// Workspace of the single-pass carry kernels: zeroed regions (left zeroed
+// by the kernels), handed out from the start again by ff_lb_begin; the
+// time-out word; ff_lb_check exits if a look-back wait ever timed out.
+void ff_lb_begin(void);
+uint32_t *ff_lb_take(size_t words);
+uint32_t *ff_lb_fault(void);
+void ff_lb_check(void);

@@ -97,6 +97,62 @@ void ff_dfree(void *p) {
     g_cached += it->second;
 }
 
+// ------------------------------------------------------------------ look-back workspace
+// Zeroed device words for the single-pass carry kernels (ff_lookback.h):
+// every job's region is left zeroed by its last workgroup, so regions are
+// handed out from a bump pointer that restarts with each launch set. A
+// larger need allocates a new zeroed buffer; the old ones stay allocated
+// (captured graphs may still point into them). One separate word collects
+// look-back time-outs.
+
+static uint32_t *g_lb = nullptr, *g_lb_fault = nullptr;
+static size_t g_lb_cap = 0, g_lb_off = 0;
+static std::vector<uint32_t *> g_lb_old;
+
+void ff_lb_begin(void) { g_lb_off = 0; }
+
+uint32_t *ff_lb_take(size_t words) {
+    words = (words + 15) / 16 * 16;
+    if (g_lb_off + words > g_lb_cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        FF_CHECK(hipStreamIsCapturing(g_st, &cs));
+        if (cs != hipStreamCaptureStatusNone) {
+            fprintf(stderr, "fastfib_amd: look-back workspace grown during graph capture\n");
+            exit(1);
+        }
+        size_t cap = g_lb_cap ? 2 * g_lb_cap : (1u << 20);
+        while (cap < words) cap *= 2;
+        uint32_t *p = nullptr;
+        FF_CHECK(hipMalloc(&p, cap * 4));
+        FF_CHECK(hipMemsetAsync(p, 0, cap * 4, g_st));
+        if (g_lb) g_lb_old.push_back(g_lb);
+        g_lb = p, g_lb_cap = cap, g_lb_off = 0;
+    }
+    uint32_t *r = g_lb + g_lb_off;
+    g_lb_off += words;
+    return r;
+}
+
+uint32_t *ff_lb_fault(void) {
+    if (!g_lb_fault) {
+        FF_CHECK(hipMalloc(&g_lb_fault, 256));
+        FF_CHECK(hipMemsetAsync(g_lb_fault, 0, 256, g_st));
+    }
+    return g_lb_fault;
+}
+
+// Fails loudly if a look-back wait ever ran out (waits for the stream).
+void ff_lb_check(void) {
+    if (!g_lb_fault) return;
+    uint32_t f = 0;
+    FF_CHECK(hipMemcpyAsync(&f, g_lb_fault, 4, hipMemcpyDeviceToHost, g_st));
+    FF_CHECK(hipStreamSynchronize(g_st));
+    if (f) {
+        fprintf(stderr, "fastfib_amd: a carry look-back wait timed out\n");
+        exit(1);
+    }
+}
+
 static uint64_t *g_res_h = nullptr;
 static size_t g_res_words = 0;
 
@@ -172,7 +228,7 @@ static void prof_collect(void) {
 
 static const char *kKindNames[FK_COUNT] = {"ntt_fwd_top", "ntt_fwd", "ntt_inner", "ntt_small", "ntt_inv",
                                            "ntt_inv_top", "crt_pack", "crt_scan", "crt_apply", "fib_step",
-                                           "carry", "normlen", "bigops", "dec_leaf", "fib_early"};
+                                           "carry", "normlen", "bigops", "dec_leaf", "fib_early", "step_crt"};
 
 extern "C" int ff_prof_enable(int on) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -226,6 +282,10 @@ extern "C" int ff_init(int device) {
         ff_set_err("table upload failed");
         return FF_ERR_DEVICE;
     }
+    // the carry look-back workspace (1M words) and its time-out word
+    (void)ff_lb_take(1u << 20);
+    ff_lb_begin();
+    (void)ff_lb_fault();
     g_inited = 1;
     return FF_OK;
 }

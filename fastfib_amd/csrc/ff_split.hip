@@ -273,19 +273,9 @@ void ff_mul_split(const MulItem &it) {
 
     // the CRT over the whole product
     const uint64_t nout = it.na + (sq ? it.na : it.nb) + 1;
-    const uint64_t nblk = (nout + 255) / 256;
-    char *w = (char *)alloc(((nblk * 4 + 255) / 256 * 256) + ((nblk + 255) / 256 * 256) + 256);
     FFCrtSet cs{};
-    cs.j[0] = FFCrtJob{Y,
-                       it.out,
-                       (uint32_t *)w,
-                       (uint8_t *)(w + (nblk * 4 + 255) / 256 * 256),
-                       (uint32_t *)(w + (nblk * 4 + 255) / 256 * 256 + (nblk + 255) / 256 * 256),
-                       it.lenkey,
-                       nout,
-                       conv,
-                       (uint32_t)Lbig,
-                       (int)bw};
+    ff_lb_begin();
+    cs.j[0] = FFCrtJob{Y, it.out, ff_lb_take(ff_crt_lb_words(nout)), it.lenkey, nout, conv, (uint32_t)Lbig, (int)bw};
     FF_CHECK(ff_launch_crt(&cs, 1, nout, (double)conv, (double)nout, g_st));
     for (void *p : lists) ff_dfree(p);
     for (void *p : mem) ff_dfree(p);

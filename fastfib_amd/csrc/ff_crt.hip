@@ -67,60 +67,101 @@ static void garner_init(void) {
 #define CRT_MAXC 2112
 
 
-// Limb sums of one product for the workgroup's limbs: thread t holds limbs
-// i0 + t LPT + q (q < LPT), u[q] = lo(T_i) + hi(T_{i-1}), and 2^64 - 1 at
-// and past nout (a carry passes through); with prev, thread 0 also gets u at
-// limb i0 - 1. xlo / xhi / hx: the workgroup's LDS.
-template <int LPT>
-__device__ __forceinline__ void crt_sums(const uint32_t *y, uint32_t L, int bw, uint64_t conv, uint64_t nout,
-                                         uint64_t i0, const FFGarner &G, uint64_t *xlo, uint32_t *xhi, uint64_t *hx,
-                                         bool prev, u128 *u, u128 *uprev) {
+// One product's residues for the limb sums.
+struct CrtSrc {
+    const uint32_t *y;  // [3][L]
+    uint32_t L;
+    int bw;
+    uint64_t conv;
+};
+
+// Limb sums of NP products for the workgroup's limbs: thread t holds limbs
+// i0 + t LPT + q (q < LPT), u[P][q] = lo(T_i) + hi(T_{i-1}) of product P,
+// and 2^64 - 1 at and past nout (a carry passes through); with prev,
+// thread 0 also gets product 0's u at limb i0 - 1. The products' Garner
+// values are formed in one loop (their loads overlap). xlo / xhi: NP x MAXC
+// LDS words, hx: 256.
+template <int LPT, int NP, int MAXC>
+__device__ __forceinline__ void crt_sums(const CrtSrc *src, uint64_t nout, uint64_t i0, const FFGarner &G,
+                                         uint64_t *xlo, uint32_t *xhi, uint64_t *hx, bool prev, u128 (*u)[LPT],
+                                         u128 *uprev) {
     const int t = threadIdx.x;
     const uint64_t i1 = i0 + CRT_LIMBS * LPT < nout ? i0 + CRT_LIMBS * LPT : nout;
-    // coefficients touching limbs [i0 - 2, i1)
-    const uint64_t lim = conv < L ? conv : L;
-    const uint64_t jbase = first_coef(i0 >= 2 ? i0 - 2 : 0, bw);
-    uint64_t jend = (64 * i1 + bw - 1) / bw;
-    if (jend > lim) jend = lim;
-    const uint32_t *__restrict__ y1 = y;
-    const uint32_t *__restrict__ y2 = y + L;
-    const uint32_t *__restrict__ y3 = y + 2 * (size_t)L;
-    for (uint64_t j = jbase + t; j < jend; j += CRT_LIMBS) {
-        const u128 X = ff_garner(y1[j], y2[j], y3[j], G);
-        xlo[j - jbase] = (uint64_t)X;
-        xhi[j - jbase] = (uint32_t)(X >> 64);
-    }
-    __syncthreads();
-    const uint64_t ib = i0 + (uint64_t)t * LPT;
-    u128 T[LPT];
+    uint64_t jbase[NP], jend[NP];
 #pragma unroll
-    for (int q = 0; q < LPT; q++) T[q] = ib + q < i1 ? limb_sum(ib + q, bw, jbase, xlo, xhi, jbase, jend) : 0;
-    hx[t] = (uint64_t)(T[LPT - 1] >> 64);
-    uint64_t hprev = 0;
-    if (t == 0 && i0 > 0) {
-        const u128 Tm1 = limb_sum(i0 - 1, bw, jbase, xlo, xhi, jbase, jend);
-        hprev = (uint64_t)(Tm1 >> 64);
-        if (prev) {
-            const uint64_t hm2 = i0 >= 2 ? (uint64_t)(limb_sum(i0 - 2, bw, jbase, xlo, xhi, jbase, jend) >> 64) : 0;
-            *uprev = (u128)(uint64_t)Tm1 + hm2;
+    for (int P = 0; P < NP; P++) {
+        // coefficients touching limbs [i0 - 2, i1)
+        const uint64_t lim = src[P].conv < src[P].L ? src[P].conv : src[P].L;
+        jbase[P] = first_coef(i0 >= 2 ? i0 - 2 : 0, src[P].bw);
+        jend[P] = (64 * i1 + src[P].bw - 1) / src[P].bw;
+        if (jend[P] > lim) jend[P] = lim;
+        if (jend[P] < jbase[P]) jend[P] = jbase[P];
+    }
+    const uint64_t nj = NP == 1 ? jend[0] - jbase[0]
+                                : (jend[0] - jbase[0] > jend[NP - 1] - jbase[NP - 1] ? jend[0] - jbase[0]
+                                                                                   : jend[NP - 1] - jbase[NP - 1]);
+    for (uint64_t k = t; k < nj; k += CRT_LIMBS) {
+#pragma unroll
+        for (int P = 0; P < NP; P++) {
+            const uint64_t j = jbase[P] + k;
+            if (j < jend[P]) {
+                const uint32_t *y = src[P].y;
+                const size_t L = src[P].L;
+                const u128 X = ff_garner(y[j], y[L + j], y[2 * L + j], G);
+                xlo[P * MAXC + k] = (uint64_t)X;
+                xhi[P * MAXC + k] = (uint32_t)(X >> 64);
+            }
         }
     }
     __syncthreads();
-    if (t > 0) hprev = hx[t - 1];
+    const uint64_t ib = i0 + (uint64_t)t * LPT;
 #pragma unroll
-    for (int q = 0; q < LPT; q++) {
-        u[q] = ib + q < nout ? (u128)(uint64_t)T[q] + hprev : (u128)~0ull;
-        hprev = (uint64_t)(T[q] >> 64);
+    for (int P = 0; P < NP; P++) {
+        const int bw = src[P].bw;
+        const uint64_t *xl = xlo + P * MAXC;
+        const uint32_t *xh = xhi + P * MAXC;
+        u128 T[LPT];
+#pragma unroll
+        for (int q = 0; q < LPT; q++) T[q] = ib + q < i1 ? limb_sum(ib + q, bw, jbase[P], xl, xh, jbase[P], jend[P]) : 0;
+        hx[P * CRT_LIMBS + t] = (uint64_t)(T[LPT - 1] >> 64);
+        uint64_t hprev = 0;
+        if (t == 0 && i0 > 0) {
+            const u128 Tm1 = limb_sum(i0 - 1, bw, jbase[P], xl, xh, jbase[P], jend[P]);
+            hprev = (uint64_t)(Tm1 >> 64);
+            if (prev && P == 0) {
+                const uint64_t hm2 =
+                    i0 >= 2 ? (uint64_t)(limb_sum(i0 - 2, bw, jbase[P], xl, xh, jbase[P], jend[P]) >> 64) : 0;
+                *uprev = (u128)(uint64_t)Tm1 + hm2;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < LPT; q++) {
+            const u128 lo = (u128)(uint64_t)T[q];
+            if (q) u[P][q] = lo + (uint64_t)(T[q - 1] >> 64);
+            else u[P][0] = lo;
+        }
+        if (t == 0) hx[NP * CRT_LIMBS + P] = hprev;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int P = 0; P < NP; P++) {
+        const uint64_t hp = t ? hx[P * CRT_LIMBS + t - 1] : hx[NP * CRT_LIMBS + P];
+        u[P][0] += hp;
+#pragma unroll
+        for (int q = 0; q < LPT; q++)
+            if (ib + q >= nout) u[P][q] = (u128)~0ull;
     }
     __syncthreads();  // the LDS is reused by the next call
 }
 
-// One workgroup per CRT_LIMBS * LPT output limbs of job J (bx: its grid
-// index, only bounding the count; the order is the ticket's).
+// Garner slots for the coefficients touching 256 LPT + 2 limbs at widths of
+// at least 32 bits (the fused step's, and the CRT's at four limbs per thread).
+#define CRT_MAXC_W32(LPT) (((CRT_LIMBS * (LPT) + 2) * 64 + 93) / 32 + 3)
+
 struct CrtShared {
     uint64_t xlo[CRT_MAXC];
     uint32_t xhi[CRT_MAXC];
-    uint64_t hx[CRT_LIMBS];
+    uint64_t hx[CRT_LIMBS + 1];
     LbShared S;
 };
 
@@ -134,13 +175,14 @@ __device__ __forceinline__ void crt_body(const FFCrtJob &J, uint32_t bx, const F
     LbShared &S = sh.S;
     const uint32_t b = lb_ticket(S, J.lb);
     const uint64_t i0 = (uint64_t)b * CRT_LIMBS * LPT, ib = i0 + (uint64_t)threadIdx.x * LPT;
-    u128 u[LPT], unused;
-    crt_sums<LPT>(J.y, J.L, J.bw, J.conv, J.nout, i0, G, xlo, xhi, hx, false, u, &unused);
+    u128 u[1][LPT], unused;
+    const CrtSrc src{J.y, J.L, J.bw, J.conv};
+    crt_sums<LPT, 1, CRT_MAXC>(&src, J.nout, i0, G, xlo, xhi, hx, false, u, &unused);
     bool g, p;
-    lb_group_gp<LPT>(u, &g, &p);
+    lb_group_gp<LPT>(u[0], &g, &p);
     const uint32_t c = lb_round<1>(S, J.lb + 2, b, &g, &p, 0, fault);
     uint64_t v[LPT];
-    lb_group_apply<LPT>(u, c, v);
+    lb_group_apply<LPT>(u[0], c, v);
 #pragma unroll
     for (int q = 0; q < LPT; q++)
         if (ib + q < J.nout) J.out[ib + q] = v[q];
@@ -149,9 +191,12 @@ __device__ __forceinline__ void crt_body(const FFCrtJob &J, uint32_t bx, const F
     if (lb_last(S, J.lb, (uint32_t)nblk)) lb_clear(J.lb, 3 + nblk);
 }
 
-// Limbs per thread of a job: four where the coefficients are at least 32
-// bits wide (those touching 1026 limbs fit CRT_MAXC), else one.
-static inline int crt_lpt(int bw) { return bw >= 32 ? 4 : 1; }
+// Limbs per thread of a job: one (shortest latency per workgroup) while a
+// workgroup per 256 limbs fits one wave of residency (~1280 workgroups);
+// above, four, where the coefficients are at least 32 bits wide (those
+// touching 1026 limbs fit CRT_MAXC).
+#define CRT_ONE_WAVE_LIMBS (1280u * CRT_LIMBS)
+__host__ __device__ static inline int crt_lpt(int bw, uint64_t nout) { return bw >= 32 && nout > CRT_ONE_WAVE_LIMBS ? 4 : 1; }
 
 template <int LPT>
 __global__ __launch_bounds__(CRT_LIMBS) void k_crt(FFCrtSet js, FFGarner G, uint32_t *fault) {
@@ -175,12 +220,12 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_crt_list(const FFCrtJob *jobs, co
                                                         FFGarner G, uint32_t *fault) {
     __shared__ CrtShared sh;
     const int k = crt_find(start, n, blockIdx.x);
-    if (jobs[k].bw >= 32) crt_body<4>(jobs[k], blockIdx.x - start[k], G, fault, sh);
+    if (crt_lpt(jobs[k].bw, jobs[k].nout) == 4) crt_body<4>(jobs[k], blockIdx.x - start[k], G, fault, sh);
     else crt_body<1>(jobs[k], blockIdx.x - start[k], G, fault, sh);
 }
 
 uint64_t ff_crt_blocks(uint64_t nout, int bw) {
-    const uint64_t per = (uint64_t)CRT_LIMBS * crt_lpt(bw);
+    const uint64_t per = (uint64_t)CRT_LIMBS * crt_lpt(bw, nout);
     return (nout + per - 1) / per;
 }
 
@@ -189,10 +234,10 @@ uint64_t ff_crt_blocks(uint64_t nout, int bw) {
 extern "C" hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_nout, double conv_total,
                                     double out_limbs_total, hipStream_t st) {
     garner_init();
-    // one launch per limbs-per-thread class (the jobs of one group share a width)
-    int lpt = crt_lpt(js->j[0].bw);
+    // one launch: four limbs per thread only when every job takes four
+    int lpt = crt_lpt(js->j[0].bw, js->j[0].nout);
     for (int k = 1; k < njobs; k++)
-        if (crt_lpt(js->j[k].bw) != lpt) lpt = 1;
+        if (crt_lpt(js->j[k].bw, js->j[k].nout) != lpt) lpt = 1;
     const FFCrtSet &set = *js;
     const uint64_t max_blk = (max_nout + (uint64_t)CRT_LIMBS * lpt - 1) / ((uint64_t)CRT_LIMBS * lpt);
     ff_prof_begin(FK_CRT_PACK, st);
@@ -219,8 +264,8 @@ extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *s
 // ------------------------------------------------------------------ fused large step
 //
 // A doubling step from the residues of s1 = F(k)^2 and s2 = F(k-1)^2 in one
-// launch (fib.c:36-62): per workgroup of 1024 limbs (four per thread), the
-// CRT of both squares, then A = s1 + s2, E = 4 s1 + cb0, B = E - s2 =
+// launch (fib.c:36-62): per workgroup of 256 or 1024 limbs (one or four per
+// thread), the CRT of both squares (their Garner values formed in one loop), then A = s1 + s2, E = 4 s1 + cb0, B = E - s2 =
 // F(2k+1), C = B - A = F(2k), every one a binary carry chain resolved by
 // look-back in four rounds: {s1, s2}, {A, E}, {B}, {C}. Limb i of 4 s1 takes
 // the top two bits of limb i - 1 of s1; for the workgroup's first limb they
@@ -229,45 +274,45 @@ extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *s
 // publishes with its last limb's top bits. cb0 = -2 adds 2^(64 len) - 2 (the
 // carry out of the top is dropped). The squares may instead be given as
 // limbs (split products), skipping the first round.
-#define STEP_LPT 4
-#define STEP_LIMBS (CRT_LIMBS * STEP_LPT)
-
+template <int LPT>
 __global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner G, uint32_t *fault) {
-    const uint64_t nblk = (J.len + STEP_LIMBS - 1) / STEP_LIMBS;
+    constexpr uint64_t PER = (uint64_t)CRT_LIMBS * LPT;
+    constexpr int MAXC = CRT_MAXC_W32(LPT);
+    const uint64_t nblk = (J.len + PER - 1) / PER;
     if (blockIdx.x >= nblk) return;
-    __shared__ uint64_t xlo[CRT_MAXC];
-    __shared__ uint32_t xhi[CRT_MAXC];
-    __shared__ uint64_t hx[CRT_LIMBS];
+    __shared__ uint64_t xlo[2 * MAXC];
+    __shared__ uint32_t xhi[2 * MAXC];
+    __shared__ uint64_t hx[2 * CRT_LIMBS + 2];
     __shared__ LbShared S;
     const int t = threadIdx.x;
     const uint32_t b = lb_ticket(S, J.lb);
     uint32_t *st = J.lb + 2;                // four rounds of nblk words
     uint32_t *lastw = J.lb + 2 + 4 * nblk;  // per workgroup: 4 | top two bits of its last s1 limb
-    const uint64_t i0 = (uint64_t)b * STEP_LIMBS, ib = i0 + (uint64_t)t * STEP_LPT;
-    bool live[STEP_LPT];
+    const uint64_t i0 = (uint64_t)b * PER, ib = i0 + (uint64_t)t * LPT;
+    bool live[LPT];
 #pragma unroll
-    for (int q = 0; q < STEP_LPT; q++) live[q] = ib + q < J.len;
+    for (int q = 0; q < LPT; q++) live[q] = ib + q < J.len;
     bool g[2], p[2];
     uint32_t c;
-    uint64_t s1[STEP_LPT], s2[STEP_LPT], prev_top;  // prev_top: top two bits of s1 at limb ib - 1
+    uint64_t s1[LPT], s2[LPT], prev_top;  // prev_top: top two bits of s1 at limb ib - 1
     if (J.s1l) {
 #pragma unroll
-        for (int q = 0; q < STEP_LPT; q++) {
+        for (int q = 0; q < LPT; q++) {
             s1[q] = live[q] && ib + q < J.n1 ? J.s1l[ib + q] : 0;
             s2[q] = live[q] && ib + q < J.n2 ? J.s2l[ib + q] : 0;
         }
         prev_top = ib >= 1 && ib - 1 < J.n1 ? J.s1l[ib - 1] >> 62 : 0;
     } else {
-        u128 u1[STEP_LPT], u2[STEP_LPT], up = 0, unused;
-        crt_sums<STEP_LPT>(J.y1, J.L1, J.bw1, J.conv1, J.len, i0, G, xlo, xhi, hx, true, u1, &up);
-        crt_sums<STEP_LPT>(J.y2, J.L2, J.bw2, J.conv2, J.len, i0, G, xlo, xhi, hx, false, u2, &unused);
-        lb_group_gp<STEP_LPT>(u1, &g[0], &p[0]);
-        lb_group_gp<STEP_LPT>(u2, &g[1], &p[1]);
+        u128 u[2][LPT], up = 0;
+        const CrtSrc src[2] = {{J.y1, J.L1, J.bw1, J.conv1}, {J.y2, J.L2, J.bw2, J.conv2}};
+        crt_sums<LPT, 2, MAXC>(src, J.len, i0, G, xlo, xhi, hx, true, u, &up);
+        lb_group_gp<LPT>(u[0], &g[0], &p[0]);
+        lb_group_gp<LPT>(u[1], &g[1], &p[1]);
         c = lb_round<2>(S, st, b, g, p, 0, fault);
-        lb_group_apply<STEP_LPT>(u1, c & 1, s1);
-        lb_group_apply<STEP_LPT>(u2, (c >> 1) & 1, s2);
-        if (t == CRT_LIMBS - 1) lb_store(lastw + b, 4u | (uint32_t)(s1[STEP_LPT - 1] >> 62));
-        hx[t] = s1[STEP_LPT - 1];
+        lb_group_apply<LPT>(u[0], c & 1, s1);
+        lb_group_apply<LPT>(u[1], (c >> 1) & 1, s2);
+        if (t == CRT_LIMBS - 1) lb_store(lastw + b, 4u | (uint32_t)(s1[LPT - 1] >> 62));
+        hx[t] = s1[LPT - 1];
         uint64_t top0 = 0;
         if (t == 0 && b > 0) {
             const uint64_t lo = (uint64_t)up;
@@ -277,56 +322,67 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner 
         __syncthreads();
         prev_top = t ? hx[t - 1] >> 62 : top0;
     }
-    u128 ua[STEP_LPT], ue[STEP_LPT];
+    u128 ua[LPT], ue[LPT];
 #pragma unroll
-    for (int q = 0; q < STEP_LPT; q++) {
+    for (int q = 0; q < LPT; q++) {
         const uint64_t i = ib + q;
         const uint64_t d = (s1[q] << 2) | (q ? s1[q - 1] >> 62 : prev_top);
         const uint64_t kv = J.cb0 >= 0 ? (i == 0 ? (uint64_t)J.cb0 : 0) : (i == 0 ? ~(uint64_t)(-J.cb0 - 1) : ~0ull);
         ua[q] = live[q] ? (u128)s1[q] + s2[q] : (u128)~0ull;
         ue[q] = live[q] ? (u128)d + kv : (u128)~0ull;
     }
-    lb_group_gp<STEP_LPT>(ua, &g[0], &p[0]);
-    lb_group_gp<STEP_LPT>(ue, &g[1], &p[1]);
+    lb_group_gp<LPT>(ua, &g[0], &p[0]);
+    lb_group_gp<LPT>(ue, &g[1], &p[1]);
     c = lb_round<2>(S, st + nblk, b, g, p, 0, fault);
-    uint64_t A[STEP_LPT], E[STEP_LPT], B[STEP_LPT], C[STEP_LPT];
-    lb_group_apply<STEP_LPT>(ua, c & 1, A);
-    lb_group_apply<STEP_LPT>(ue, (c >> 1) & 1, E);
-    u128 ub[STEP_LPT];  // B = E + ~s2 + 1
+    uint64_t A[LPT], E[LPT], B[LPT], C[LPT];
+    lb_group_apply<LPT>(ua, c & 1, A);
+    lb_group_apply<LPT>(ue, (c >> 1) & 1, E);
+    u128 ub[LPT];  // B = E + ~s2 + 1
 #pragma unroll
-    for (int q = 0; q < STEP_LPT; q++) ub[q] = live[q] ? (u128)E[q] + ~s2[q] : (u128)~0ull;
-    lb_group_gp<STEP_LPT>(ub, &g[0], &p[0]);
+    for (int q = 0; q < LPT; q++) ub[q] = live[q] ? (u128)E[q] + ~s2[q] : (u128)~0ull;
+    lb_group_gp<LPT>(ub, &g[0], &p[0]);
     c = lb_round<1>(S, st + 2 * nblk, b, g, p, 1, fault);
-    lb_group_apply<STEP_LPT>(ub, c, B);
-    u128 uc[STEP_LPT];  // C = B + ~A + 1
+    lb_group_apply<LPT>(ub, c, B);
+    u128 uc[LPT];  // C = B + ~A + 1
 #pragma unroll
-    for (int q = 0; q < STEP_LPT; q++) uc[q] = live[q] ? (u128)B[q] + ~A[q] : (u128)~0ull;
-    lb_group_gp<STEP_LPT>(uc, &g[0], &p[0]);
+    for (int q = 0; q < LPT; q++) uc[q] = live[q] ? (u128)B[q] + ~A[q] : (u128)~0ull;
+    lb_group_gp<LPT>(uc, &g[0], &p[0]);
     c = lb_round<1>(S, st + 3 * nblk, b, g, p, 1, fault);
-    lb_group_apply<STEP_LPT>(uc, c, C);
-    uint64_t out[2 * STEP_LPT];
+    lb_group_apply<LPT>(uc, c, C);
+    uint64_t out[2 * LPT];
 #pragma unroll
-    for (int q = 0; q < STEP_LPT; q++) {
+    for (int q = 0; q < LPT; q++) {
         out[q] = live[q] ? (J.bit ? B[q] : A[q]) : 0;
-        out[STEP_LPT + q] = live[q] ? C[q] : 0;
+        out[LPT + q] = live[q] ? C[q] : 0;
         if (live[q]) {
             J.K[ib + q] = out[q];
-            J.C[ib + q] = out[STEP_LPT + q];
+            J.C[ib + q] = out[LPT + q];
         }
     }
     unsigned long long *const keys[2] = {J.keyK, J.keyC};
-    lb_keys<2, STEP_LPT>(J.lb + 2 + 5 * nblk, b, (uint32_t)nblk, i0, J.len, out, keys, fault);
+    lb_keys<2, LPT>(J.lb + 2 + 5 * nblk, b, (uint32_t)nblk, i0, J.len, out, keys, fault);
     if (lb_last(S, J.lb, (uint32_t)nblk)) lb_clear(J.lb, 3 + 5 * nblk);
 }
 
-size_t ff_step_lb_words(uint64_t len) { return 3 + 5 * ((len + STEP_LIMBS - 1) / STEP_LIMBS); }
+// Limbs per thread of a step: as for the CRT (one while a workgroup per
+// 256 limbs fits one wave of residency).
+static inline int step_lpt(uint64_t len) { return len > CRT_ONE_WAVE_LIMBS ? 4 : 1; }
+
+size_t ff_step_lb_words(uint64_t len) {
+    const uint64_t per = (uint64_t)CRT_LIMBS * step_lpt(len);
+    return 3 + 5 * ((len + per - 1) / per);
+}
 
 extern "C" hipError_t ff_launch_step_fused(const FFStepJob *J, double conv_total, hipStream_t st) {
     garner_init();
-    if (!J->s1l && (J->bw1 < 32 || J->bw2 < 32)) return hipErrorInvalidValue;  // four limbs per thread need bw >= 32
-    const uint64_t nblk = (J->len + STEP_LIMBS - 1) / STEP_LIMBS;
+    if (!J->s1l && (J->bw1 < 32 || J->bw2 < 32)) return hipErrorInvalidValue;  // the LDS holds widths >= 32
+    const int lpt = step_lpt(J->len);
+    const uint64_t nblk = (J->len + (uint64_t)CRT_LIMBS * lpt - 1) / ((uint64_t)CRT_LIMBS * lpt);
     ff_prof_begin(FK_STEP_CRT, st);
-    hipLaunchKernelGGL(k_step_fused, dim3((unsigned)nblk), dim3(CRT_LIMBS), 0, st, *J, g_garner, ff_lb_fault());
+    if (lpt == 4)
+        hipLaunchKernelGGL(k_step_fused<4>, dim3((unsigned)nblk), dim3(CRT_LIMBS), 0, st, *J, g_garner, ff_lb_fault());
+    else
+        hipLaunchKernelGGL(k_step_fused<1>, dim3((unsigned)nblk), dim3(CRT_LIMBS), 0, st, *J, g_garner, ff_lb_fault());
     ff_prof_end(FK_STEP_CRT, 12.0 * conv_total + 16.0 * J->len, st);
     return hipGetLastError();
 }

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
         auto ld = [&](int pos, int c) -> uint32_t { return tl.s[ff_pos_addr(tl, pos, c)]; };
         auto sto = [&](int pos, int c, uint32_t v) { tl.s[ff_pos_addr(tl, pos, c)] = v; };
         ff_conv_io(tl, E.tw[pi][0], E.tw[pi][1], E.tw[pi][2], E.tw[pi][3], nullptr, p, pinv, false, false, ld, sto,
-                   FFLdsTables{});
+                   FFLdsTables{}, 2);
         __syncthreads();
         const uint32_t sc = a.sc[lg][pi];
         for (int i = t - tid0; i < 2 * L; i += nthr)

@@ -319,25 +319,29 @@ __device__ void ff_inverse_io(const FFTile &t, const uint32_t *irt, const uint32
 // through ld when from_g), pointwise with B (LDS, already transformed) or a
 // square, inverse, the last round through st when to_g.
 template <class Ld, class St, class Gt = FFNoTw>
+// rl: levels per round (1..4). Fewer levels per round means more, shorter
+// rounds with more threads busy: lower latency for small tiles in one
+// workgroup; four is the throughput choice.
 __device__ void ff_conv_io(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, const uint32_t *irt,
                            const uint32_t *irtp, const uint32_t *B, uint32_t p, uint32_t pinv, bool from_g, bool to_g,
-                           Ld ld, St st, const Gt &gt = Gt()) {
-    const int R = ff_nrounds(t.rho);
+                           Ld ld, St st, const Gt &gt = Gt(), int rl = 4) {
+    const int R = (t.rho + rl - 1) / rl;
     int b = t.rho;
     for (int r = 0; r < R - 1; r++) {
-        b -= 4;
+        b -= rl;
         if (r) __syncthreads();
-        ff_round_any<FF_R_DIF>(4, from_g && r == 0, false, t, b, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, st, gt);
+        ff_round_any<FF_R_DIF>(rl, from_g && r == 0, false, t, b, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, st,
+                               gt);
     }
-    const int nl0 = ff_last_nl(t.rho);
+    const int nl0 = t.rho % rl == 0 ? rl : t.rho % rl;
     if (R > 1) __syncthreads();
     ff_round_any<FF_R_MID>(nl0, from_g && R == 1, to_g && R == 1, t, 0, rt, rtp, irt, irtp, B, p, pinv, ld, st, gt);
     b = nl0;
     for (int r = 1; r < R; r++) {
         __syncthreads();
-        ff_round_any<FF_R_DIT>(4, false, to_g && r == R - 1, t, b, nullptr, nullptr, irt, irtp, nullptr, p, pinv, ld,
+        ff_round_any<FF_R_DIT>(rl, false, to_g && r == R - 1, t, b, nullptr, nullptr, irt, irtp, nullptr, p, pinv, ld,
                                st, gt);
-        b += 4;
+        b += rl;
     }
 }
 

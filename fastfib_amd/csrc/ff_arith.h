@@ -49,9 +49,14 @@ struct FFTables {
     uint32_t th[3][FF_TH_SIZE];
 };
 
+// hi - mh mod p for hi, mh < p: r = hi - mh wraps to above 2^32 - p when
+// hi < mh, where r + p wraps back below p; otherwise r < p <= r + p < 2^32
+// (p < 2^31). So the result is min(r, r + p): three instructions, no
+// compare-and-select.
 __device__ __forceinline__ uint32_t ff_mont_fix(uint32_t hi, uint32_t mh, uint32_t p) {
-    uint32_t r = hi - mh;
-    return hi < mh ? r + p : r;
+    const uint32_t r = hi - mh;
+    const uint32_t rp = r + p;
+    return rp < r ? rp : r;
 }
 
 // a * b * 2^-32 mod p, any 32-bit a, b < p.

@@ -100,14 +100,32 @@ __device__ __forceinline__ void crt_sums(const CrtSrc *src, uint64_t nout, uint6
     const uint64_t nj = NP == 1 ? jend[0] - jbase[0]
                                 : (jend[0] - jbase[0] > jend[NP - 1] - jbase[NP - 1] ? jend[0] - jbase[0]
                                                                                    : jend[NP - 1] - jbase[NP - 1]);
-    for (uint64_t k = t; k < nj; k += CRT_LIMBS) {
+    // every residue load issued before the first Garner step: one HBM
+    // latency per workgroup instead of one per loop trip
+    constexpr int ITERS = (MAXC + CRT_LIMBS - 1) / CRT_LIMBS;
+    uint32_t r[NP][ITERS][3];
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+        const uint64_t k = t + (uint64_t)it * CRT_LIMBS;
 #pragma unroll
         for (int P = 0; P < NP; P++) {
             const uint64_t j = jbase[P] + k;
-            if (j < jend[P]) {
+            if (k < nj && j < jend[P]) {
                 const uint32_t *y = src[P].y;
                 const size_t L = src[P].L;
-                const u128 X = ff_garner(y[j], y[L + j], y[2 * L + j], G);
+                r[P][it][0] = y[j];
+                r[P][it][1] = y[L + j];
+                r[P][it][2] = y[2 * L + j];
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+        const uint64_t k = t + (uint64_t)it * CRT_LIMBS;
+#pragma unroll
+        for (int P = 0; P < NP; P++) {
+            if (k < nj && jbase[P] + k < jend[P]) {
+                const u128 X = ff_garner(r[P][it][0], r[P][it][1], r[P][it][2], G);
                 xlo[P * MAXC + k] = (uint64_t)X;
                 xhi[P * MAXC + k] = (uint32_t)(X >> 64);
             }
@@ -121,8 +139,7 @@ __device__ __forceinline__ void crt_sums(const CrtSrc *src, uint64_t nout, uint6
         const uint64_t *xl = xlo + P * MAXC;
         const uint32_t *xh = xhi + P * MAXC;
         u128 T[LPT];
-#pragma unroll
-        for (int q = 0; q < LPT; q++) T[q] = ib + q < i1 ? limb_sum(ib + q, bw, jbase[P], xl, xh, jbase[P], jend[P]) : 0;
+        limb_sums<LPT>(ib, i1, bw, xl, xh, jbase[P], jend[P], T);
         hx[P * CRT_LIMBS + t] = (uint64_t)(T[LPT - 1] >> 64);
         uint64_t hprev = 0;
         if (t == 0 && i0 > 0) {

@@ -31,9 +31,36 @@ __device__ __forceinline__ u128 ff_garner(uint32_t r1, uint32_t r2, uint32_t r3,
 }
 
 // First coefficient touching limb i: coefficient j spans bits [j bw, j bw + 93).
+// (A 32-bit division below 2^32 bits: a fraction of the 64-bit one's cost.)
 __device__ __forceinline__ uint64_t first_coef(uint64_t i, int bw) {
     const uint64_t lo = 64 * i;
-    return lo >= 93 ? (lo - 93) / bw + 1 : 0;
+    if (lo < 93) return 0;
+    if (lo - 93 < (1ull << 32)) return (uint32_t)(lo - 93) / (uint32_t)bw + 1;
+    return (lo - 93) / bw + 1;
+}
+
+// T_i for the n consecutive limbs i0, i0 + 1, .. (n <= N) below i1, in one
+// walk over the coefficients: the first one touching each limb follows the
+// previous limb's by at most a few steps (one division per walk).
+template <int N>
+__device__ __forceinline__ void limb_sums(uint64_t i0, uint64_t i1, int bw, const uint64_t *xlo, const uint32_t *xhi,
+                                          uint64_t jbase, uint64_t jend, u128 *T) {
+    uint64_t j = first_coef(i0, bw);
+    if (j < jbase) j = jbase;
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        T[q] = 0;
+        const uint64_t i = i0 + q;
+        if (i >= i1) continue;
+        const uint64_t bit = 64 * i;
+        while (j < jend && j * (uint64_t)bw + 93 <= bit) j++;
+        for (uint64_t jj = j; jj < jend; jj++) {
+            const uint64_t jb = jj * (uint64_t)bw;
+            if (jb >= bit + 64) break;
+            const u128 X = ((u128)xhi[jj - jbase] << 64) | xlo[jj - jbase];
+            T[q] += jb >= bit ? (uint64_t)(X << (unsigned)(jb - bit)) : (uint64_t)(X >> (unsigned)(bit - jb));
+        }
+    }
 }
 
 // T_i = sum over the coefficients touching limb i of digit i of X_j 2^(j bw).

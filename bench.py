@@ -435,7 +435,20 @@ def main() -> int:
     ap.add_argument("--large-n", type=int, default=10 ** 9, help="F(n) leg with dispatcher schedules (0: skip)")
     ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--product-steps", type=int, default=10, help="2^26-bit product leg (0: skip)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (the reference dispatch for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes), then exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(64 << 20, device="cuda")  # 64 Mi float32 = 256 MiB
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copied 256 MiB x3", float(b[0]))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (products shard by prime or "
               "by item only in design); run with --gpus 1", file=sys.stderr)

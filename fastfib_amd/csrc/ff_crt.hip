@@ -184,13 +184,13 @@ struct CrtShared {
 
 template <int LPT>
 __device__ __forceinline__ void crt_body(const FFCrtJob &J, uint32_t bx, const FFGarner &G, uint32_t *fault,
-                                         CrtShared &sh) {
+                                         CrtShared &sh, bool resident) {
     const uint64_t nblk = (J.nout + CRT_LIMBS * LPT - 1) / (CRT_LIMBS * LPT);
     if (bx >= nblk) return;
     uint64_t *xlo = sh.xlo, *hx = sh.hx;
     uint32_t *xhi = sh.xhi;
     LbShared &S = sh.S;
-    const uint32_t b = lb_ticket(S, J.lb);
+    const uint32_t b = lb_ticket(S, J.lb, resident, bx);
     const uint64_t i0 = (uint64_t)b * CRT_LIMBS * LPT, ib = i0 + (uint64_t)threadIdx.x * LPT;
     u128 u[1][LPT], unused;
     const CrtSrc src{J.y, J.L, J.bw, J.conv};
@@ -219,9 +219,9 @@ __host__ __device__ static inline int crt_lpt(int bw, uint64_t nout) { return bw
 __host__ __device__ static inline int crt_lpt_list(int bw) { return bw >= 32 ? 4 : 1; }
 
 template <int LPT>
-__global__ __launch_bounds__(CRT_LIMBS) void k_crt(FFCrtSet js, FFGarner G, uint32_t *fault) {
+__global__ __launch_bounds__(CRT_LIMBS) void k_crt(FFCrtSet js, FFGarner G, uint32_t *fault, int resident) {
     __shared__ CrtShared sh;
-    crt_body<LPT>(js.j[blockIdx.y], blockIdx.x, G, fault, sh);
+    crt_body<LPT>(js.j[blockIdx.y], blockIdx.x, G, fault, sh, resident != 0);
 }
 
 __device__ __forceinline__ int crt_find(const uint32_t *start, int n, uint32_t b) {
@@ -240,8 +240,8 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_crt_list(const FFCrtJob *jobs, co
                                                         FFGarner G, uint32_t *fault) {
     __shared__ CrtShared sh;
     const int k = crt_find(start, n, blockIdx.x);
-    if (crt_lpt_list(jobs[k].bw) == 4) crt_body<4>(jobs[k], blockIdx.x - start[k], G, fault, sh);
-    else crt_body<1>(jobs[k], blockIdx.x - start[k], G, fault, sh);
+    if (crt_lpt_list(jobs[k].bw) == 4) crt_body<4>(jobs[k], blockIdx.x - start[k], G, fault, sh, false);
+    else crt_body<1>(jobs[k], blockIdx.x - start[k], G, fault, sh, false);
 }
 
 uint64_t ff_crt_blocks(uint64_t nout, int bw) {
@@ -262,12 +262,15 @@ extern "C" hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_
     const FFCrtSet &set = *js;
     const uint64_t max_blk = (max_nout + (uint64_t)CRT_LIMBS * lpt - 1) / ((uint64_t)CRT_LIMBS * lpt);
     ff_prof_begin(FK_CRT_PACK, st);
-    if (lpt == 4)
+    if (lpt == 4) {
+        const int res = max_blk * njobs <= ff_resident_blocks((const void *)k_crt<4>, CRT_LIMBS);
         hipLaunchKernelGGL(k_crt<4>, dim3((unsigned)max_blk, njobs), dim3(CRT_LIMBS), 0, st, set, g_garner,
-                           ff_lb_fault());
-    else
+                           ff_lb_fault(), res);
+    } else {
+        const int res = max_blk * njobs <= ff_resident_blocks((const void *)k_crt<1>, CRT_LIMBS);
         hipLaunchKernelGGL(k_crt<1>, dim3((unsigned)max_blk, njobs), dim3(CRT_LIMBS), 0, st, set, g_garner,
-                           ff_lb_fault());
+                           ff_lb_fault(), res);
+    }
     ff_prof_end(FK_CRT_PACK, 12.0 * conv_total + 8.0 * out_limbs_total, st);
     return hipGetLastError();
 }
@@ -296,7 +299,7 @@ extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *s
 // carry out of the top is dropped). The squares may instead be given as
 // limbs (split products), skipping the first round.
 template <int LPT>
-__global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner G, uint32_t *fault) {
+__global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner G, uint32_t *fault, int resident) {
     constexpr uint64_t PER = (uint64_t)CRT_LIMBS * LPT;
     constexpr int MAXC = CRT_MAXC_W32(LPT);
     const uint64_t nblk = (J.len + PER - 1) / PER;
@@ -306,7 +309,7 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner 
     __shared__ uint64_t hx[2 * CRT_LIMBS + 2];
     __shared__ LbShared S;
     const int t = threadIdx.x;
-    const uint32_t b = lb_ticket(S, J.lb);
+    const uint32_t b = lb_ticket(S, J.lb, resident != 0, blockIdx.x);
     uint32_t *st = J.lb + 2;                // four rounds of nblk words
     uint32_t *lastw = J.lb + 2 + 4 * nblk;  // per workgroup: 4 | top two bits of its last s1 limb
     const uint64_t i0 = (uint64_t)b * PER, ib = i0 + (uint64_t)t * LPT;
@@ -400,10 +403,15 @@ extern "C" hipError_t ff_launch_step_fused(const FFStepJob *J, double conv_total
     const int lpt = step_lpt(J->len);
     const uint64_t nblk = (J->len + (uint64_t)CRT_LIMBS * lpt - 1) / ((uint64_t)CRT_LIMBS * lpt);
     ff_prof_begin(FK_STEP_CRT, st);
-    if (lpt == 4)
-        hipLaunchKernelGGL(k_step_fused<4>, dim3((unsigned)nblk), dim3(CRT_LIMBS), 0, st, *J, g_garner, ff_lb_fault());
-    else
-        hipLaunchKernelGGL(k_step_fused<1>, dim3((unsigned)nblk), dim3(CRT_LIMBS), 0, st, *J, g_garner, ff_lb_fault());
+    if (lpt == 4) {
+        const int res = nblk <= ff_resident_blocks((const void *)k_step_fused<4>, CRT_LIMBS);
+        hipLaunchKernelGGL(k_step_fused<4>, dim3((unsigned)nblk), dim3(CRT_LIMBS), 0, st, *J, g_garner, ff_lb_fault(),
+                           res);
+    } else {
+        const int res = nblk <= ff_resident_blocks((const void *)k_step_fused<1>, CRT_LIMBS);
+        hipLaunchKernelGGL(k_step_fused<1>, dim3((unsigned)nblk), dim3(CRT_LIMBS), 0, st, *J, g_garner, ff_lb_fault(),
+                           res);
+    }
     ff_prof_end(FK_STEP_CRT, 12.0 * conv_total + 16.0 * J->len, st);
     return hipGetLastError();
 }

@@ -22,11 +22,11 @@ typedef unsigned __int128 u128;
 #define LAST_LPT 4
 #define LAST_LIMBS (LB_T * LAST_LPT)
 
-__global__ __launch_bounds__(LB_T) void k_last(FFLastArgs a, uint32_t *fault) {
+__global__ __launch_bounds__(LB_T) void k_last(FFLastArgs a, uint32_t *fault, int resident) {
     const uint64_t nblk = (a.len + LAST_LIMBS - 1) / LAST_LIMBS;
     if (blockIdx.x >= nblk) return;
     __shared__ LbShared S;
-    const uint32_t b = lb_ticket(S, a.lb);
+    const uint32_t b = lb_ticket(S, a.lb, resident != 0, blockIdx.x);
     const uint64_t i0 = (uint64_t)b * LAST_LIMBS, ib = i0 + (uint64_t)threadIdx.x * LAST_LPT;
     u128 sx[LAST_LPT], sy[LAST_LPT];
     uint64_t prev = ib >= 1 && ib - 1 < a.nu ? a.u[ib - 1] : 0;
@@ -68,7 +68,8 @@ size_t ff_last_lb_words(uint64_t len) { return 3 + (len + LAST_LIMBS - 1) / LAST
 extern "C" hipError_t ff_launch_last(const FFLastArgs *a, hipStream_t st) {
     const uint64_t nblk = (a->len + LAST_LIMBS - 1) / LAST_LIMBS;
     ff_prof_begin(FK_STEP, st);
-    hipLaunchKernelGGL(k_last, dim3((unsigned)nblk), dim3(LB_T), 0, st, *a, ff_lb_fault());
+    const int res = nblk <= ff_resident_blocks((const void *)k_last, LB_T);
+    hipLaunchKernelGGL(k_last, dim3((unsigned)nblk), dim3(LB_T), 0, st, *a, ff_lb_fault(), res);
     ff_prof_end(FK_STEP, 8.0 * (a->nu + a->nv + (a->Y ? 2 : 1) * a->len), st);
     return hipGetLastError();
 }

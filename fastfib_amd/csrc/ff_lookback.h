@@ -65,8 +65,12 @@ __device__ __noinline__ uint32_t lb_wait(const uint32_t *p, uint32_t *fault) {
     return 0;
 }
 
-// The workgroup's position among the job's workgroups, in start order.
-__device__ __forceinline__ uint32_t lb_ticket(LbShared &S, uint32_t *ws) {
+// The workgroup's position among the job's workgroups, in start order. A
+// grid that fits the GPU at once (every workgroup resident together) uses
+// its grid index and skips the ticket atomic, which all workgroups would
+// contend for at the same moment.
+__device__ __forceinline__ uint32_t lb_ticket(LbShared &S, uint32_t *ws, bool resident, uint32_t bx) {
+    if (resident) return bx;
     if (threadIdx.x == 0) S.word = atomicAdd(ws, 1u);
     __syncthreads();
     return S.word;

@@ -34,9 +34,10 @@ void ff_prof_pause(int pause);
 uint64_t *ff_pinned_results(size_t words);
 
 // Workspace of the single-pass carry kernels: zeroed regions (left zeroed
-// by the kernels), handed out from the start again by ff_lb_begin; the
-// time-out word; ff_lb_check exits if a look-back wait ever timed out.
+// by the kernels), handed out from the start again by ff_lb_begin; the word
+// a look-back wait that times out sets (read back with the length keys).
 void ff_lb_begin(void);
 uint32_t *ff_lb_take(size_t words);
 uint32_t *ff_lb_fault(void);
-void ff_lb_check(void);
+// Workgroups of the kernel (at this block size) the GPU holds at once.
+uint64_t ff_resident_blocks(const void *kernel, int threads);

@@ -133,24 +133,26 @@ uint32_t *ff_lb_take(size_t words) {
     return r;
 }
 
+// Workgroups of a kernel that the GPU holds at once (occupancy per CU times
+// the CUs), memoized per kernel.
+uint64_t ff_resident_blocks(const void *kernel, int threads) {
+    static std::map<const void *, uint64_t> memo;
+    auto it = memo.find(kernel);
+    if (it != memo.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    FF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    FF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g_device));
+    const uint64_t r = (uint64_t)per_cu * (uint64_t)cus;
+    memo[kernel] = r;
+    return r;
+}
+
 uint32_t *ff_lb_fault(void) {
     if (!g_lb_fault) {
         FF_CHECK(hipMalloc(&g_lb_fault, 256));
         FF_CHECK(hipMemsetAsync(g_lb_fault, 0, 256, g_st));
     }
     return g_lb_fault;
-}
-
-// Fails loudly if a look-back wait ever ran out (waits for the stream).
-void ff_lb_check(void) {
-    if (!g_lb_fault) return;
-    uint32_t f = 0;
-    FF_CHECK(hipMemcpyAsync(&f, g_lb_fault, 4, hipMemcpyDeviceToHost, g_st));
-    FF_CHECK(hipStreamSynchronize(g_st));
-    if (f) {
-        fprintf(stderr, "fastfib_amd: a carry look-back wait timed out\n");
-        exit(1);
-    }
 }
 
 static uint64_t *g_res_h = nullptr;

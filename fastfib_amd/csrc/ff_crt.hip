@@ -205,7 +205,7 @@ __device__ __forceinline__ void crt_body(const FFCrtJob &J, uint32_t bx, const F
         if (ib + q < J.nout) J.out[ib + q] = v[q];
     unsigned long long *const keys[1] = {J.lenkey};
     lb_keys<1, LPT>(J.lb + 2 + nblk, b, (uint32_t)nblk, i0, J.nout, v, keys, fault);
-    if (lb_last(S, J.lb, (uint32_t)nblk)) lb_clear(J.lb, 3 + nblk);
+    if (lb_last(S, J.lb, J.lb + 3 + nblk, b, (uint32_t)nblk)) lb_clear(J.lb, 3 + nblk + lb_groups(nblk));
 }
 
 // Limbs per thread of a job: one (shortest latency per workgroup) while a
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner 
     }
     unsigned long long *const keys[2] = {J.keyK, J.keyC};
     lb_keys<2, LPT>(J.lb + 2 + 5 * nblk, b, (uint32_t)nblk, i0, J.len, out, keys, fault);
-    if (lb_last(S, J.lb, (uint32_t)nblk)) lb_clear(J.lb, 3 + 5 * nblk);
+    if (lb_last(S, J.lb, J.lb + 3 + 5 * nblk, b, (uint32_t)nblk)) lb_clear(J.lb, 3 + 5 * nblk + lb_groups(nblk));
 }
 
 // Limbs per thread of a step: as for the CRT (one while a workgroup per
@@ -394,7 +394,8 @@ static inline int step_lpt(uint64_t len) { return len > CRT_ONE_WAVE_LIMBS ? 4 :
 
 size_t ff_step_lb_words(uint64_t len) {
     const uint64_t per = (uint64_t)CRT_LIMBS * step_lpt(len);
-    return 3 + 5 * ((len + per - 1) / per);
+    const uint64_t nblk = (len + per - 1) / per;
+    return 3 + 5 * nblk + lb_groups(nblk);
 }
 
 extern "C" hipError_t ff_launch_step_fused(const FFStepJob *J, double conv_total, hipStream_t st) {

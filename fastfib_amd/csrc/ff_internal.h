@@ -188,7 +188,7 @@ hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_nout, doubl
 hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *start, int n, uint32_t blocks,
                               double conv_total, double out_limbs_total, hipStream_t st);
 // Look-back words of a job of nout limbs; its workgroups in a work list.
-static inline size_t ff_crt_lb_words(uint64_t nout) { return 3 + (nout + 255) / 256; }
+static inline size_t ff_crt_lb_words(uint64_t nout) { return 3 + 2 * ((nout + 255) / 256); }
 uint64_t ff_crt_blocks(uint64_t nout, int bw);
 
 // A doubling step after its two squarings (k_step_fused): from the residues

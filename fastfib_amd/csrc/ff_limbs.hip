@@ -60,10 +60,13 @@ __global__ __launch_bounds__(LB_T) void k_last(FFLastArgs a, uint32_t *fault, in
     unsigned long long *const keys[2] = {a.keyX, a.keyY};
     if (a.Y) lb_keys<2, LAST_LPT>(a.lb + 2 + nblk, b, (uint32_t)nblk, i0, a.len, out, keys, fault);
     else lb_keys<1, LAST_LPT>(a.lb + 2 + nblk, b, (uint32_t)nblk, i0, a.len, out, keys, fault);
-    if (lb_last(S, a.lb, (uint32_t)nblk)) lb_clear(a.lb, 3 + nblk);
+    if (lb_last(S, a.lb, a.lb + 3 + nblk, b, (uint32_t)nblk)) lb_clear(a.lb, 3 + nblk + lb_groups(nblk));
 }
 
-size_t ff_last_lb_words(uint64_t len) { return 3 + (len + LAST_LIMBS - 1) / LAST_LIMBS; }
+size_t ff_last_lb_words(uint64_t len) {
+    const uint64_t nblk = (len + LAST_LIMBS - 1) / LAST_LIMBS;
+    return 3 + nblk + lb_groups(nblk);
+}
 
 extern "C" hipError_t ff_launch_last(const FFLastArgs *a, hipStream_t st) {
     const uint64_t nblk = (a->len + LAST_LIMBS - 1) / LAST_LIMBS;

@@ -17,9 +17,10 @@
 // arithmetic as inside one workgroup: A = g, B = g | p, C = (A + B + c) ^ A ^ B.
 //
 // Workspace of a job (uint32 words, zero on entry): [0] ticket counter,
-// [1] finished-workgroup counter, then status words, then one word through
+// [1] finished-group counter, then status words, then one word through
 // which the second-to-last workgroup passes the bit lengths of its last two
-// limbs to the last (for the length keys). The last workgroup to finish
+// limbs to the last (for the length keys), then the finished-workgroup
+// counters of groups of LB_GROUP workgroups. The last workgroup to finish
 // zeroes it again, so it can be reused by the next launch on the stream
 // (and by graph replays) without a clearing pass.
 //
@@ -154,11 +155,23 @@ __device__ __forceinline__ uint32_t lb_round(LbShared &S, uint32_t *st, uint32_t
     return c;
 }
 
+// Finished-workgroup counting is two-level: a counter per group of
+// LB_GROUP workgroups (after the status words, at gc), and ws[1] counting
+// the complete groups, so no single word takes every workgroup's atomic.
+#define LB_GROUP 32
+__host__ __device__ __forceinline__ uint64_t lb_groups(uint64_t nblk) { return (nblk + LB_GROUP - 1) / LB_GROUP; }
+
 // True in the job's last workgroup to finish (every other workgroup is past
-// its last read of the workspace).
-__device__ __forceinline__ bool lb_last(LbShared &S, uint32_t *ws, uint32_t nblk) {
+// its last read of the workspace); b: the workgroup's position.
+__device__ __forceinline__ bool lb_last(LbShared &S, uint32_t *ws, uint32_t *gc, uint32_t b, uint32_t nblk) {
     __syncthreads();
-    if (threadIdx.x == 0) S.word = (atomicAdd(ws + 1, 1u) == nblk - 1) ? 1u : 0u;
+    if (threadIdx.x == 0) {
+        const uint32_t g = b / LB_GROUP;
+        const uint32_t gsize = nblk - g * LB_GROUP < LB_GROUP ? nblk - g * LB_GROUP : LB_GROUP;
+        bool last = false;
+        if (atomicAdd(gc + g, 1u) == gsize - 1) last = atomicAdd(ws + 1, 1u) == (uint32_t)lb_groups(nblk) - 1;
+        S.word = last ? 1u : 0u;
+    }
     __syncthreads();
     return S.word != 0;
 }

@@ -78,9 +78,9 @@ struct CrtSrc {
 // Limb sums of NP products for the workgroup's limbs: thread t holds limbs
 // i0 + t LPT + q (q < LPT), u[P][q] = lo(T_i) + hi(T_{i-1}) of product P,
 // and 2^64 - 1 at and past nout (a carry passes through); with prev,
-// thread 0 also gets product 0's u at limb i0 - 1. The products' Garner
+// thread 0 also gets each product's u at limb i0 - 1 (uprev[P]). The products' Garner
 // values are formed in one loop (their loads overlap). xlo / xhi: NP x MAXC
-// LDS words, hx: 256.
+// LDS words, hx: NP x 256 + 4 NP.
 template <int LPT, int NP, int MAXC>
 __device__ __forceinline__ void crt_sums(const CrtSrc *src, uint64_t nout, uint64_t i0, const FFGarner &G,
                                          uint64_t *xlo, uint32_t *xhi, uint64_t *hx, bool prev, u128 (*u)[LPT],
@@ -140,16 +140,16 @@ __device__ __forceinline__ void crt_sums(const CrtSrc *src, uint64_t nout, uint6
         const uint32_t *xh = xhi + P * MAXC;
         u128 T[LPT];
         limb_sums<LPT>(ib, i1, bw, xl, xh, jbase[P], jend[P], T);
-        hx[P * CRT_LIMBS + t] = (uint64_t)(T[LPT - 1] >> 64);
-        uint64_t hprev = 0;
-        if (t == 0 && i0 > 0) {
-            const u128 Tm1 = limb_sum(i0 - 1, bw, jbase[P], xl, xh, jbase[P], jend[P]);
-            hprev = (uint64_t)(Tm1 >> 64);
-            if (prev && P == 0) {
-                const uint64_t hm2 =
-                    i0 >= 2 ? (uint64_t)(limb_sum(i0 - 2, bw, jbase[P], xl, xh, jbase[P], jend[P]) >> 64) : 0;
-                *uprev = (u128)(uint64_t)Tm1 + hm2;
-            }
+            hx[P * CRT_LIMBS + t] = (uint64_t)(T[LPT - 1] >> 64);
+        // the sums below the workgroup's first limb, by lanes 2P and 2P + 1
+        // of the first wavefront together: T at i0 - 1 and at i0 - 2
+        if (t >> 1 == P && i0 > (uint64_t)(t & 1)) {
+            const u128 Tb = limb_sum(i0 - 1 - (t & 1), bw, jbase[P], xl, xh, jbase[P], jend[P]);
+            hx[NP * CRT_LIMBS + 4 * P + 2 * (t & 1)] = (uint64_t)Tb;
+            hx[NP * CRT_LIMBS + 4 * P + 2 * (t & 1) + 1] = (uint64_t)(Tb >> 64);
+        } else if (t >> 1 == P) {
+            hx[NP * CRT_LIMBS + 4 * P + 2 * (t & 1)] = 0;
+            hx[NP * CRT_LIMBS + 4 * P + 2 * (t & 1) + 1] = 0;
         }
 #pragma unroll
         for (int q = 0; q < LPT; q++) {
@@ -157,12 +157,14 @@ __device__ __forceinline__ void crt_sums(const CrtSrc *src, uint64_t nout, uint6
             if (q) u[P][q] = lo + (uint64_t)(T[q - 1] >> 64);
             else u[P][0] = lo;
         }
-        if (t == 0) hx[NP * CRT_LIMBS + P] = hprev;
     }
     __syncthreads();
 #pragma unroll
     for (int P = 0; P < NP; P++) {
-        const uint64_t hp = t ? hx[P * CRT_LIMBS + t - 1] : hx[NP * CRT_LIMBS + P];
+        // hx[NP 256 + 4P ..]: lo, hi of T at i0 - 1, then at i0 - 2
+        const uint64_t *tb = hx + NP * CRT_LIMBS + 4 * P;
+        const uint64_t hp = t ? hx[P * CRT_LIMBS + t - 1] : tb[1];
+        if (prev && t == 0) uprev[P] = (u128)tb[0] + tb[3];
         u[P][0] += hp;
 #pragma unroll
         for (int q = 0; q < LPT; q++)
@@ -178,7 +180,7 @@ __device__ __forceinline__ void crt_sums(const CrtSrc *src, uint64_t nout, uint6
 struct CrtShared {
     uint64_t xlo[CRT_MAXC];
     uint32_t xhi[CRT_MAXC];
-    uint64_t hx[CRT_LIMBS + 1];
+    uint64_t hx[CRT_LIMBS + 4];
     LbShared S;
 };
 
@@ -192,9 +194,9 @@ __device__ __forceinline__ void crt_body(const FFCrtJob &J, uint32_t bx, const F
     LbShared &S = sh.S;
     const uint32_t b = lb_ticket(S, J.lb, resident, bx);
     const uint64_t i0 = (uint64_t)b * CRT_LIMBS * LPT, ib = i0 + (uint64_t)threadIdx.x * LPT;
-    u128 u[1][LPT], unused;
+    u128 u[1][LPT], unused[1];
     const CrtSrc src{J.y, J.L, J.bw, J.conv};
-    crt_sums<LPT, 1, CRT_MAXC>(&src, J.nout, i0, G, xlo, xhi, hx, false, u, &unused);
+    crt_sums<LPT, 1, CRT_MAXC>(&src, J.nout, i0, G, xlo, xhi, hx, false, u, unused);
     bool g, p;
     lb_group_gp<LPT>(u[0], &g, &p);
     const uint32_t c = lb_round<1>(S, J.lb + 2, b, &g, &p, 0, fault);
@@ -289,15 +291,17 @@ extern "C" hipError_t ff_launch_crt_list(const FFCrtJob *jobs, const uint32_t *s
 //
 // A doubling step from the residues of s1 = F(k)^2 and s2 = F(k-1)^2 in one
 // launch (fib.c:36-62): per workgroup of 256 or 1024 limbs (one or four per
-// thread), the CRT of both squares (their Garner values formed in one loop), then A = s1 + s2, E = 4 s1 + cb0, B = E - s2 =
-// F(2k+1), C = B - A = F(2k), every one a binary carry chain resolved by
-// look-back in four rounds: {s1, s2}, {A, E}, {B}, {C}. Limb i of 4 s1 takes
-// the top two bits of limb i - 1 of s1; for the workgroup's first limb they
-// come from its own sums (limb i0 - 1 is summed too), unless the carry into
-// that limb could reach bit 62, in which case from the word the predecessor
-// publishes with its last limb's top bits. cb0 = -2 adds 2^(64 len) - 2 (the
-// carry out of the top is dropped). The squares may instead be given as
-// limbs (split products), skipping the first round.
+// thread), the CRT of both squares (their Garner values formed in one loop),
+// then A = s1 + s2 = F(2k-1), E = 4 s1 + cb0, H = s1 + 2 s2,
+// B = E - s2 = F(2k+1) and C = E - H = F(2k), every one a binary carry chain
+// resolved by look-back in three rounds: {s1, s2}, {A, E, H}, {B, C}. The
+// shifted limb i of 4 s1 (2 s2) takes the top two bits (top bit) of limb
+// i - 1; for the workgroup's first limb they come from its own sums (limb
+// i0 - 1 is summed too), unless the carry into that limb could reach them,
+// in which case from the word the predecessor publishes with its last
+// limbs' top bits. cb0 = -2 adds 2^(64 len) - 2 (the carry out of the top
+// is dropped). The squares may instead be given as limbs (split products),
+// skipping the first round.
 template <int LPT>
 __global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner G, uint32_t *fault, int resident) {
     constexpr uint64_t PER = (uint64_t)CRT_LIMBS * LPT;
@@ -306,73 +310,86 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner 
     if (blockIdx.x >= nblk) return;
     __shared__ uint64_t xlo[2 * MAXC];
     __shared__ uint32_t xhi[2 * MAXC];
-    __shared__ uint64_t hx[2 * CRT_LIMBS + 2];
+    __shared__ uint64_t hx[2 * CRT_LIMBS + 8];
     __shared__ LbShared S;
     const int t = threadIdx.x;
     const uint32_t b = lb_ticket(S, J.lb, resident != 0, blockIdx.x);
-    uint32_t *st = J.lb + 2;                // four rounds of nblk words
-    uint32_t *lastw = J.lb + 2 + 4 * nblk;  // per workgroup: 4 | top two bits of its last s1 limb
+    uint32_t *st = J.lb + 2;                // three rounds of nblk words
+    uint32_t *lastw = J.lb + 2 + 3 * nblk;  // per workgroup: 8 | top bit of its last s2 limb << 2 | top two of s1's
     const uint64_t i0 = (uint64_t)b * PER, ib = i0 + (uint64_t)t * LPT;
     bool live[LPT];
 #pragma unroll
     for (int q = 0; q < LPT; q++) live[q] = ib + q < J.len;
-    bool g[2], p[2];
+    bool g[3], p[3];
     uint32_t c;
-    uint64_t s1[LPT], s2[LPT], prev_top;  // prev_top: top two bits of s1 at limb ib - 1
+    uint64_t s1[LPT], s2[LPT], prev1, prev2;  // top two bits of s1, top bit of s2, at limb ib - 1
     if (J.s1l) {
 #pragma unroll
         for (int q = 0; q < LPT; q++) {
             s1[q] = live[q] && ib + q < J.n1 ? J.s1l[ib + q] : 0;
             s2[q] = live[q] && ib + q < J.n2 ? J.s2l[ib + q] : 0;
         }
-        prev_top = ib >= 1 && ib - 1 < J.n1 ? J.s1l[ib - 1] >> 62 : 0;
+        prev1 = ib >= 1 && ib - 1 < J.n1 ? J.s1l[ib - 1] >> 62 : 0;
+        prev2 = ib >= 1 && ib - 1 < J.n2 ? J.s2l[ib - 1] >> 63 : 0;
     } else {
-        u128 u[2][LPT], up = 0;
+        u128 u[2][LPT], up[2] = {0, 0};
         const CrtSrc src[2] = {{J.y1, J.L1, J.bw1, J.conv1}, {J.y2, J.L2, J.bw2, J.conv2}};
-        crt_sums<LPT, 2, MAXC>(src, J.len, i0, G, xlo, xhi, hx, true, u, &up);
+        crt_sums<LPT, 2, MAXC>(src, J.len, i0, G, xlo, xhi, hx, true, u, up);
         lb_group_gp<LPT>(u[0], &g[0], &p[0]);
         lb_group_gp<LPT>(u[1], &g[1], &p[1]);
         c = lb_round<2>(S, st, b, g, p, 0, fault);
         lb_group_apply<LPT>(u[0], c & 1, s1);
         lb_group_apply<LPT>(u[1], (c >> 1) & 1, s2);
-        if (t == CRT_LIMBS - 1) lb_store(lastw + b, 4u | (uint32_t)(s1[LPT - 1] >> 62));
+        if (t == CRT_LIMBS - 1)
+            lb_store(lastw + b, 8u | (uint32_t)(s1[LPT - 1] >> 62) | ((uint32_t)(s2[LPT - 1] >> 63) << 2));
         hx[t] = s1[LPT - 1];
-        uint64_t top0 = 0;
+        hx[CRT_LIMBS + t] = s2[LPT - 1];
+        uint64_t p1 = 0, p2 = 0;
         if (t == 0 && b > 0) {
-            const uint64_t lo = (uint64_t)up;
-            const uint64_t low62 = (1ull << 62) - 1;
-            top0 = (lo & low62) != low62 ? lo >> 62 : (uint64_t)(lb_wait(lastw + b - 1, fault) & 3);
+            // a carry into limb i0 - 1 changes its top bits only through all-ones low bits
+            const uint64_t lo1 = (uint64_t)up[0], lo2 = (uint64_t)up[1];
+            const uint64_t low62 = (1ull << 62) - 1, low63 = (1ull << 63) - 1;
+            if ((lo1 & low62) != low62 && (lo2 & low63) != low63) {
+                p1 = lo1 >> 62, p2 = lo2 >> 63;
+            } else {
+                const uint32_t w = lb_wait(lastw + b - 1, fault);
+                p1 = w & 3, p2 = (w >> 2) & 1;
+            }
         }
         __syncthreads();
-        prev_top = t ? hx[t - 1] >> 62 : top0;
+        prev1 = t ? hx[t - 1] >> 62 : p1;
+        prev2 = t ? hx[CRT_LIMBS + t - 1] >> 63 : p2;
     }
-    u128 ua[LPT], ue[LPT];
+    u128 ua[LPT], ue[LPT], uh[LPT];
 #pragma unroll
     for (int q = 0; q < LPT; q++) {
         const uint64_t i = ib + q;
-        const uint64_t d = (s1[q] << 2) | (q ? s1[q - 1] >> 62 : prev_top);
+        const uint64_t d = (s1[q] << 2) | (q ? s1[q - 1] >> 62 : prev1);
+        const uint64_t d2 = (s2[q] << 1) | (q ? s2[q - 1] >> 63 : prev2);
         const uint64_t kv = J.cb0 >= 0 ? (i == 0 ? (uint64_t)J.cb0 : 0) : (i == 0 ? ~(uint64_t)(-J.cb0 - 1) : ~0ull);
         ua[q] = live[q] ? (u128)s1[q] + s2[q] : (u128)~0ull;
         ue[q] = live[q] ? (u128)d + kv : (u128)~0ull;
+        uh[q] = live[q] ? (u128)s1[q] + d2 : (u128)~0ull;
     }
     lb_group_gp<LPT>(ua, &g[0], &p[0]);
     lb_group_gp<LPT>(ue, &g[1], &p[1]);
-    c = lb_round<2>(S, st + nblk, b, g, p, 0, fault);
-    uint64_t A[LPT], E[LPT], B[LPT], C[LPT];
+    lb_group_gp<LPT>(uh, &g[2], &p[2]);
+    c = lb_round<3>(S, st + nblk, b, g, p, 0, fault);
+    uint64_t A[LPT], E[LPT], H[LPT], B[LPT], C[LPT];
     lb_group_apply<LPT>(ua, c & 1, A);
     lb_group_apply<LPT>(ue, (c >> 1) & 1, E);
-    u128 ub[LPT];  // B = E + ~s2 + 1
+    lb_group_apply<LPT>(uh, (c >> 2) & 1, H);
+    u128 ub[LPT], uc[LPT];  // B = E + ~s2 + 1, C = E + ~H + 1
 #pragma unroll
-    for (int q = 0; q < LPT; q++) ub[q] = live[q] ? (u128)E[q] + ~s2[q] : (u128)~0ull;
+    for (int q = 0; q < LPT; q++) {
+        ub[q] = live[q] ? (u128)E[q] + ~s2[q] : (u128)~0ull;
+        uc[q] = live[q] ? (u128)E[q] + ~H[q] : (u128)~0ull;
+    }
     lb_group_gp<LPT>(ub, &g[0], &p[0]);
-    c = lb_round<1>(S, st + 2 * nblk, b, g, p, 1, fault);
-    lb_group_apply<LPT>(ub, c, B);
-    u128 uc[LPT];  // C = B + ~A + 1
-#pragma unroll
-    for (int q = 0; q < LPT; q++) uc[q] = live[q] ? (u128)B[q] + ~A[q] : (u128)~0ull;
-    lb_group_gp<LPT>(uc, &g[0], &p[0]);
-    c = lb_round<1>(S, st + 3 * nblk, b, g, p, 1, fault);
-    lb_group_apply<LPT>(uc, c, C);
+    lb_group_gp<LPT>(uc, &g[1], &p[1]);
+    c = lb_round<2>(S, st + 2 * nblk, b, g, p, 3, fault);
+    lb_group_apply<LPT>(ub, c & 1, B);
+    lb_group_apply<LPT>(uc, (c >> 1) & 1, C);
     uint64_t out[2 * LPT];
 #pragma unroll
     for (int q = 0; q < LPT; q++) {
@@ -384,8 +401,8 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner 
         }
     }
     unsigned long long *const keys[2] = {J.keyK, J.keyC};
-    lb_keys<2, LPT>(J.lb + 2 + 5 * nblk, b, (uint32_t)nblk, i0, J.len, out, keys, fault);
-    if (lb_last(S, J.lb, J.lb + 3 + 5 * nblk, b, (uint32_t)nblk)) lb_clear(J.lb, 3 + 5 * nblk + lb_groups(nblk));
+    lb_keys<2, LPT>(J.lb + 2 + 4 * nblk, b, (uint32_t)nblk, i0, J.len, out, keys, fault);
+    if (lb_last(S, J.lb, J.lb + 3 + 4 * nblk, b, (uint32_t)nblk)) lb_clear(J.lb, 3 + 4 * nblk + lb_groups(nblk));
 }
 
 // Limbs per thread of a step: as for the CRT (one while a workgroup per
@@ -395,7 +412,7 @@ static inline int step_lpt(uint64_t len) { return len > CRT_ONE_WAVE_LIMBS ? 4 :
 size_t ff_step_lb_words(uint64_t len) {
     const uint64_t per = (uint64_t)CRT_LIMBS * step_lpt(len);
     const uint64_t nblk = (len + per - 1) / per;
-    return 3 + 5 * nblk + lb_groups(nblk);
+    return 3 + 4 * nblk + lb_groups(nblk);
 }
 
 extern "C" hipError_t ff_launch_step_fused(const FFStepJob *J, double conv_total, hipStream_t st) {

@@ -30,7 +30,7 @@
 // launch.
 //
 // Status word: bits 0-1 kind (LB_AGG, LB_INC); aggregate: G_x at bit 2 + 2x,
-// P_x at bit 3 + 2x; inclusive: carry-out of chain x at bit 2 + x.
+// P_x at bit 3 + 2x; inclusive: carry-out of chain x at bit 2 + x (x < 3).
 #pragma once
 #include <stdint.h>
 
@@ -41,7 +41,7 @@
 #define LB_SPIN_MAX (1u << 18)
 
 struct LbShared {
-    uint32_t wg[2][LB_NW], wp[2][LB_NW];
+    uint32_t wg[3][LB_NW], wp[3][LB_NW];
     uint32_t cin;   // the block's carry-ins, bit x for chain x
     uint32_t word;  // broadcast of thread 0's ticket / last flag
 };
@@ -77,7 +77,7 @@ __device__ __forceinline__ uint32_t lb_ticket(LbShared &S, uint32_t *ws, bool re
     return S.word;
 }
 
-// One round over NC (1 or 2) chains: limb (generate, propagate) per thread
+// One round over NC (1 to 3) chains: limb (generate, propagate) per thread
 // in g[x], p[x]; st: the round's status words; init: bit x = carry into
 // block 0 of chain x. Returns bit x = carry into this thread's limb.
 template <int NC>

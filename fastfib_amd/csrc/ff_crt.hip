@@ -257,10 +257,12 @@ uint64_t ff_crt_blocks(uint64_t nout, int bw) {
 extern "C" hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_nout, double conv_total,
                                     double out_limbs_total, hipStream_t st) {
     garner_init();
-    // one launch: four limbs per thread only when every job takes four
-    int lpt = crt_lpt(js->j[0].bw, js->j[0].nout);
-    for (int k = 1; k < njobs; k++)
-        if (crt_lpt(js->j[k].bw, js->j[k].nout) != lpt) lpt = 1;
+    // one launch: four limbs per thread when the jobs together pass one
+    // wave of workgroups and every width allows it
+    uint64_t total = 0;
+    int wide = 1;
+    for (int k = 0; k < njobs; k++) total += js->j[k].nout, wide &= js->j[k].bw >= 32;
+    const int lpt = crt_lpt(wide ? 32 : 0, total);
     const FFCrtSet &set = *js;
     const uint64_t max_blk = (max_nout + (uint64_t)CRT_LIMBS * lpt - 1) / ((uint64_t)CRT_LIMBS * lpt);
     ff_prof_begin(FK_CRT_PACK, st);

@@ -38,7 +38,7 @@ int ff_width_ok(uint64_t bits_a, uint64_t bits_b, unsigned w) { return width_ok(
 static uint64_t ntt_length(uint64_t conv, int *lg_out, int *r3_out) {
     uint64_t L = 16;
     int lg = 4, r3 = 0;
-    while (L < conv) L <<= 1, lg++;
+    while (L < conv && lg < 62) L <<= 1, lg++;
     if (L / 4 * 3 >= conv && L / 4 >= 16) L = L / 4 * 3, lg -= 2, r3 = 1;
     *lg_out = lg;
     *r3_out = r3;
@@ -76,9 +76,10 @@ static ProdShape product_shape(uint64_t bits_a, uint64_t bits_b) {
     for (unsigned w = 32; w <= 35; w++) {
         if (!width_ok(bits_a, bits_b, w)) break;
         const uint64_t ca = (bits_a + w - 1) / w, cb = (bits_b + w - 1) / w;
+        const uint64_t conv = ca + cb > 1 ? ca + cb - 1 : 1;  // a zero operand: one coefficient
         int lg, r3;
-        const uint64_t L = ntt_length(ca + cb - 1, &lg, &r3);
-        if (s.L == 0 || L < s.L) s.L = L, s.lg = lg, s.r3 = r3, s.bw = w, s.conv = ca + cb - 1;
+        const uint64_t L = ntt_length(conv, &lg, &r3);
+        if (s.L == 0 || L < s.L) s.L = L, s.lg = lg, s.r3 = r3, s.bw = w, s.conv = conv;
     }
     if (s.L == 0 || s.L > (1u << g_max_lg)) s.split = 1;
     if (!s.split && g_force_len) {
@@ -422,6 +423,155 @@ static void square_step(const DNum &fk, const DNum &fkm1, int k_odd, int bit, DN
     for (void *p : keep) ff_dfree(p);
 }
 
+// ------------------------------------------------------------------ pair products
+// u = cu0 a^2 + cu1 ab + cu2 b^2 and v = cv0 a^2 + cv1 ab + cv2 b^2 (b <= a)
+// from one forward transform of each of a and b: the three pointwise
+// products are weighted and summed before the inverse transforms
+// (k_inner's pair mode), and the plain CRT writes u and v. A doubling step
+// is such a pair, with no linear pass and no constant after its CRT:
+//   F(2k-1) = a^2 + b^2, F(2k) = a^2 + 2ab, F(2k+1) = 2a^2 + 2ab + b^2
+// for a = F(k), b = F(k-1) (the identities behind fib.c:36-62's step). The
+// CRT bound carries the largest weight sum s: s ca 2^(2w) < p1 p2 p3, with
+// the reference rule's widths otherwise (32..35, shortest length first).
+static ProdShape pair_shape(uint64_t bits_a, unsigned wsum) {
+    ProdShape s{};
+    const u128 M = (u128)FF_P1 * FF_P2 * FF_P3;
+    for (unsigned w = 32; w <= 35; w++) {
+        const uint64_t ca = (bits_a + w - 1) / w;
+        if ((((u128)ca * wsum) << (2 * w)) >= M) break;
+        int lg, r3;
+        const uint64_t L = ntt_length(2 * ca - 1, &lg, &r3);
+        if (s.L == 0 || L < s.L) s.L = L, s.lg = lg, s.r3 = r3, s.bw = w, s.conv = 2 * ca - 1;
+    }
+    if (s.L == 0 || s.L > (1u << g_max_lg)) s.split = 1;
+    return s;
+}
+
+// g_pair_steps bit 0: doubling steps as pairs; bit 1: the last product as a
+// pair. Quadrupling (always a pair) follows g_quad_max_lg alone.
+static bool pair_free(void) { return !g_dispatch && !g_force_len; }
+static bool pair_enabled(int bit) { return pair_free() && (g_pair_steps >> bit & 1); }
+
+// Runs a pair of the given shape when its transforms take outer passes
+// (longer than one block, not split); false, with nothing launched,
+// otherwise. deg4: the weights apply to x^2, xy, y^2 for x = a^2 + 2ab,
+// y = a^2 + b^2. u (and v when cv) get nout limbs and a length slot each;
+// pred: their bit lengths. The CRT books a length key only when the top
+// nonzero limb is among the last three (lb_keys), so nout must be tight:
+// the callers derive it from an upper bound within a few bits of the
+// outputs' true lengths.
+static bool pair_run(const DNum &a, const DNum &b, const ProdShape &s, int deg4, const uint8_t *cu,
+                     const uint8_t *cv, uint64_t nout, DNum *u, DNum *v, const uint64_t *pred) {
+    if (b.bits == 0 || b.bits > a.bits || s.split || s.L <= FF_TW) return false;
+    FFPlan pl;
+    if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, g_inner_log, g_tile_words) != 0 ||
+        pl.npass == 0)
+        return false;
+    const int two = cv != nullptr;
+    const size_t o_fb = align256(3 * (size_t)s.L * 4);
+    char *base = (char *)ff_dalloc(2 * o_fb);
+    uint32_t *fa = (uint32_t *)base, *fb = (uint32_t *)(base + o_fb);
+    const double opbytes = 8.0 * (a.n + b.n);
+    FFXformSet fwd = {};
+    fwd.x[0] = FFXform{(const uint32_t *)a.d, (uint32_t)(2 * a.n), fa};
+    fwd.x[1] = FFXform{(const uint32_t *)b.d, (uint32_t)(2 * b.n), fb};
+    FF_CHECK(ff_launch_forward(&pl, &fwd, 2, opbytes, g_st));
+    FFConvSet cs = {};
+    cs.c[0] = FFConv{(const uint32_t *)a.d, (const uint32_t *)b.d, (uint32_t)(2 * a.n), (uint32_t)(2 * b.n), fa, fb};
+    cs.c[0].pair = (two ? 2 : 1) | (deg4 ? 4 : 0);
+    for (int j = 0; j < 3; j++) cs.c[0].cu[j] = cu[j], cs.c[0].cv[j] = two ? cv[j] : 0;
+    FF_CHECK(ff_launch_conv(&pl, &cs, 1, 1 + two, opbytes, g_st));
+    FFXformSet inv = {};
+    inv.x[0] = FFXform{nullptr, 0, fa};
+    inv.x[1] = FFXform{nullptr, 0, fb};
+    FF_CHECK(ff_launch_inverse(&pl, &inv, 1 + two, g_st));
+    DNum *o[2] = {u, v};
+    FFCrtSet crt = {};
+    ff_lb_begin();
+    for (int j = 0; j <= two; j++) {
+        *o[j] = dnum_alloc(nout);
+        o[j]->slot = slot_new();
+        crt.j[j] = FFCrtJob{j ? fb : fa, o[j]->d, ff_lb_take(ff_crt_lb_words(nout)), g_slots + o[j]->slot,
+                            nout,        s.conv,  (uint32_t)s.L,                      (int)s.bw};
+    }
+    FF_CHECK(ff_launch_crt(&crt, 1 + two, nout, (double)s.conv * (1 + two), (double)nout * (1 + two), g_st));
+    ff_dfree(base);
+    if (pred) {
+        for (int j = 0; j <= two; j++) o[j]->bits = pred[j], o[j]->n = pred[j] ? (pred[j] + 63) / 64 : 1;
+    } else {
+        uint64_t k[2];
+        slots_read(u->slot, 1 + two, k);
+        for (int j = 0; j <= two; j++) key_decode(k[j], &o[j]->n, &o[j]->bits);
+    }
+    return true;
+}
+
+static unsigned wsum3(const uint8_t *w) { return (unsigned)w[0] + w[1] + w[2]; }
+
+// Output limbs of a doubling pair: with a = F(k) of bits_a bits, F(2k-1),
+// F(2k) and F(2k+1) lie in [a^2, 5 a^2], so between 2 bits_a - 1 and
+// 2 bits_a + 3 bits; one limb of room above that bound.
+static uint64_t pair_limbs(uint64_t bits_a) { return (2 * bits_a + 3 + 63) / 64 + 1; }
+// Of a quadrupling: F(4k-1) .. F(4k+3) lie between 2^(4 bits_a - 2) and
+// 2^(4 bits_a + 6).
+static uint64_t quad_limbs(uint64_t bits_a) { return (4 * bits_a + 6 + 63) / 64 + 1; }
+
+// A doubling step as one pair: (lo, hi) = bit ? (F(2k), F(2k+1)) :
+// (F(2k-1), F(2k)); pl: their bit lengths.
+static bool pair_step(const DNum &fk, const DNum &fkm1, int bit, DNum *lo, DNum *hi, const uint64_t *pl) {
+    static const uint8_t W_LO[2][3] = {{1, 0, 1}, {1, 2, 0}}, W_HI[2][3] = {{1, 2, 0}, {2, 2, 1}};
+    const unsigned ws = std::max(wsum3(W_LO[bit]), wsum3(W_HI[bit]));
+    return pair_run(fk, fkm1, pair_shape(fk.bits, ws), 0, W_LO[bit], W_HI[bit], pair_limbs(fk.bits), lo, hi, pl);
+}
+
+// ------------------------------------------------------------------ quadrupling
+// Two bits of n in one transform round trip: with a = F(k), b = F(k-1),
+// x = F(2k) = a^2 + 2ab and y = F(2k-1) = a^2 + b^2 are formed pointwise,
+// and for m = 4k + d the new pair (F(m-1), F(m)) is a combination of x^2,
+// xy, y^2 (F(4k) = x^2 + 2xy, F(4k-1) = x^2 + y^2, then F(j+1) = F(j) +
+// F(j-1)):
+//   d = 0: (x^2 + y^2,         x^2 + 2xy)
+//   d = 1: (x^2 + 2xy,         2x^2 + 2xy + y^2)
+//   d = 2: (2x^2 + 2xy + y^2,  3x^2 + 4xy + y^2)
+//   d = 3: (3x^2 + 4xy + y^2,  5x^2 + 6xy + 2y^2)
+// The outputs are degree 4 in the coefficients of a and b: x has at most
+// 2 ca - 1 coefficients below 3 ca B^2 (B = 2^w), y below 2 ca B^2, so a
+// coefficient of w0 x^2 + w1 xy + w2 y^2 stays below
+// (18 w0 + 12 w1 + 8 w2) ca^3 B^4, which must stay below p1 p2 p3: widths
+// near 10 bits. Where a step costs its launches' latency rather than its
+// arithmetic, one such step replaces two.
+static const uint8_t Q_W[4][2][3] = {{{1, 0, 1}, {1, 2, 0}},
+                                     {{1, 2, 0}, {2, 2, 1}},
+                                     {{2, 2, 1}, {3, 4, 1}},
+                                     {{3, 4, 1}, {5, 6, 2}}};
+
+static unsigned quad_bound(const uint8_t *w) { return 18u * w[0] + 12u * w[1] + 8u * w[2]; }
+
+// The shortest transform over widths 8..35 that keep K ca^3 2^(4w) below
+// p1 p2 p3, the widest width among equal lengths (fewest coefficients).
+static ProdShape quad_shape(uint64_t bits_a, unsigned K) {
+    ProdShape s{};
+    const u128 M = (u128)FF_P1 * FF_P2 * FF_P3;
+    for (unsigned w = 8; w <= 35 && 4 * w < 93; w++) {
+        const uint64_t ca = (bits_a + w - 1) / w;
+        if (ca >= (1u << 25)) continue;
+        if ((u128)ca * ca * ca * K >= (M >> (4 * w))) break;
+        int lg, r3;
+        const uint64_t L = ntt_length(4 * ca - 3, &lg, &r3);
+        if (s.L == 0 || L <= s.L) s.L = L, s.lg = lg, s.r3 = r3, s.bw = w, s.conv = 4 * ca - 3;
+    }
+    if (s.L == 0 || s.L > (1u << g_max_lg)) s.split = 1;
+    return s;
+}
+
+static bool quad_step(const DNum &fk, const DNum &fkm1, int d, DNum *lo, DNum *hi, const uint64_t *pl) {
+    if (!g_quad_max_lg || fkm1.bits == 0) return false;
+    const unsigned K = std::max(quad_bound(Q_W[d][0]), quad_bound(Q_W[d][1]));
+    const ProdShape s = quad_shape(fk.bits, K);
+    if (s.split || s.L > (1ull << g_quad_max_lg)) return false;
+    return pair_run(fk, fkm1, s, 1, Q_W[d][0], Q_W[d][1], quad_limbs(fk.bits), lo, hi, pl);
+}
+
 // A step whose squarings are one-block transforms (L <= 4096), with
 // predicted lengths: the two transforms (one launch when their shapes
 // agree), then the CRT of both squares and the linear step fused in one
@@ -762,6 +912,26 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
     }
     if (!fk.d) fkm1 = dnum_u64(0), fk = dnum_u64(1);
     for (; i >= 1; i--, step++) {
+        if (pair_free() && i >= 2) {
+            // bits i and i - 1 at once
+            const int d = (int)((n >> (i - 1)) & 3);
+            uint64_t pq[2] = {0, 0};
+            if (predict) pq[0] = P(4 * k + d - 1, FB_F), pq[1] = P(4 * k + d, FB_F);
+            DNum nlo, nhi;
+            if (quad_step(fk, fkm1, d, &nlo, &nhi, predict ? pq : nullptr)) {
+                if (predict) {
+                    checks.push_back({nlo.slot, pq[0]});
+                    checks.push_back({nhi.slot, pq[1]});
+                }
+                dnum_free(fk);
+                dnum_free(fkm1);
+                fkm1 = nlo, fk = nhi;
+                k = 4 * k + d;
+                k_odd = d & 1;
+                i--, step++;
+                continue;
+            }
+        }
         if (predict && small_fusable(fk)) {
             const int bit = (int)((n >> i) & 1);
             const uint64_t bsq = P(k, FB_F2);
@@ -784,8 +954,9 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
             pl[0] = P(2 * k - 1 + bit, FB_F), pl[1] = P(2 * k + bit, FB_F);
         }
         DNum nlo, nhi;
-        square_step(fk, fkm1, k_odd, bit, &nlo, &nhi, predict ? ps : nullptr, predict ? pl : nullptr,
-                    ds.empty() ? nullptr : &ds[2 * step]);
+        if (!(pair_enabled(0) && pair_step(fk, fkm1, bit, &nlo, &nhi, predict ? pl : nullptr)))
+            square_step(fk, fkm1, k_odd, bit, &nlo, &nhi, predict ? ps : nullptr, predict ? pl : nullptr,
+                        ds.empty() ? nullptr : &ds[2 * step]);
         if (predict) {
             checks.push_back({nlo.slot, pl[0]});
             checks.push_back({nhi.slot, pl[1]});
@@ -802,7 +973,16 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         return true;
     }
     DNum result;
-    if (n & 1) {
+    // the last step as a pair with one output: F(2k) = a^2 + 2ab,
+    // F(2k+1) = 2a^2 + 2ab + b^2
+    static const uint8_t W_LAST[2][3] = {{1, 2, 0}, {2, 2, 1}};
+    uint64_t pr_last = predict ? P(n, FB_F) : 0;
+    if (pair_enabled(1) && pair_run(fk, fkm1, pair_shape(fk.bits, wsum3(W_LAST[n & 1])), 0, W_LAST[n & 1], nullptr,
+                                   pair_limbs(fk.bits), &result, nullptr, predict ? &pr_last : nullptr)) {
+        if (predict) checks.push_back({result.slot, pr_last});
+        dnum_free(fk);
+        dnum_free(fkm1);
+    } else if (n & 1) {
         // F(2k+1) = (2F(k) + F(k-1)) (2F(k) - F(k-1)) + 2(-1)^k, the
         // factors being F(k+2) and L(k-1)
         DNum x, y;

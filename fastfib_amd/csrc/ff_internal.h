@@ -26,11 +26,17 @@ struct FFXformSet {
 // One product for the inner kernels: fa holds a's forward transform (or, for
 // one-block transforms, receives it from asrc) and gets the product's; fb /
 // bsrc = nullptr squares.
+// pair != 0 (multi-pass plans only): fa, fb hold the transforms of a and b,
+// and get u = cu0 a^2 + cu1 ab + cu2 b^2 and (pair = 2) v = cv0 a^2 +
+// cv1 ab + cv2 b^2, weights of 0, 1 or 2 combining the three pointwise
+// products before the inverse transform.
 struct FFConv {
     const uint32_t *asrc, *bsrc;
     uint32_t anw, bnw;
     uint32_t *fa;
     const uint32_t *fb;
+    int pair;
+    uint8_t cu[3], cv[3];
 };
 struct FFConvSet {
     FFConv c[FF_MAXB];

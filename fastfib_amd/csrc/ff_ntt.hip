@@ -245,6 +245,7 @@ __global__ __launch_bounds__(FF_THREADS) void k_outer_list(const FFOuterItem *it
     outer_body<INV, TOP, R3>(it.ps, it.x, pi, local - (uint32_t)pi * ntile, T, PR, lds);
 }
 
+
 // Inner blocks: forward transform of a (and b), pointwise product, inverse,
 // all in LDS. small = 1: the whole transform, read from the limbs and
 // scaled for the CRT at the end.
@@ -324,6 +325,137 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
                lds_a, lds_a_st);
     ff_radix3_layer<true>(ta, T, pi, P, in.rstep3);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) store_a((int)i, 0, lds_a((int)i, 0));
+}
+
+// Pair mode (multi-pass plans): 2 * FF_THREADS threads, the first half on
+// the block of a, the second on the block of b, each through the forward
+// levels into its LDS tile (tile A, tile B). The middle round then fuses the
+// pointwise combination with the first inverse levels: each thread of the
+// first half forms u = cu0 X + cu1 Y + cu2 Z of its group (X = a^2, Y = ab,
+// Z = b^2 pointwise), each of the second half v likewise, both from tiles A
+// and B; a barrier between those reads and the writes back keeps the two
+// halves off each other's operands. The remaining inverse rounds end in
+// HBM. With u alone the second half runs the rounds on an empty tile
+// (nsub = 0), keeping the barrier counts equal.
+//
+// Degree 4 (pair & 4): the weights apply to X = x^2, Y = xy, Z = y^2 for
+// x = a^2 + 2ab, y = a^2 + b^2 (F(2k), F(2k-1) from F(k), F(k-1)), and one
+// product by R^3 restores the single R^-1 of a plain pointwise product that
+// the inverse pass's CRT scale expects.
+
+// w x mod p for a weight w < 8.
+__device__ __forceinline__ uint32_t ff_wmul(uint32_t x, int w, uint32_t p) {
+    const uint32_t x2 = ff_add(x, x, p), x4 = ff_add(x2, x2, p);
+    uint32_t r = (w & 1) ? x : 0u;
+    if (w & 2) r = ff_add(r, x2, p);
+    if (w & 4) r = ff_add(r, x4, p);
+    return r;
+}
+
+struct FFPairPoint {
+    int w0, w1, w2;
+    int deg4;
+    uint32_t r3;  // R^3 mod p
+    __device__ uint32_t operator()(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) const {
+        if (deg4) {
+            const uint32_t aa = ff_mont(x, x, p, pinv), ab = ff_mont(x, y, p, pinv), bb = ff_mont(y, y, p, pinv);
+            x = ff_add(aa, ff_add(ab, ab, p), p);
+            y = ff_add(aa, bb, p);
+        }
+        const uint32_t xx = ff_mont(x, x, p, pinv), xy = ff_mont(x, y, p, pinv), yy = ff_mont(y, y, p, pinv);
+        uint32_t u = ff_add(ff_add(ff_wmul(xx, w0, p), ff_wmul(xy, w1, p), p), ff_wmul(yy, w2, p), p);
+        if (deg4) u = ff_mont(u, r3, p, pinv);
+        return u;
+    }
+};
+
+// The middle round on row bits [0, NL) of a one-column tile: this thread's
+// output from tiles A and B, the first NL inverse levels, then (after a
+// barrier) its own tile, or HBM through st when it is the last round. Every
+// thread of the workgroup runs the same number of iterations.
+template <int NL, class St>
+__device__ __forceinline__ void pair_mid_round(const FFTile &t, bool active, const uint32_t *A, const uint32_t *B,
+                                               const uint32_t *irt, const uint32_t *irtp, uint32_t p, uint32_t pinv,
+                                               const FFPairPoint &pt, bool to_g, St st) {
+    const int G = 1 << NL;
+    const int ngroups = 1 << (t.rho - NL);
+    const int nthr = ff_tile_nthr(t);
+    const int iters = (ngroups + nthr - 1) / nthr;
+    for (int it = 0; it < iters; it++) {
+        const int g = ff_tile_tid(t) + it * nthr;
+        const bool act = active && g < ngroups;
+        uint32_t v[G];
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                const int ad = ff_pos_addr(t, (g << NL) | j, 0);
+                v[j] = pt(A[ad], B[ad], p, pinv);
+            }
+            ff_dit_group<NL>(v, 0, 0, irt, irtp, p, pinv);
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                const int pos = (g << NL) | j;
+                if (to_g) st(pos, 0, v[j]);
+                else t.s[ff_pos_addr(t, pos, 0)] = v[j];
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void inner_pair_body(const FFInner &in, const FFConv &C, int pi, uint32_t blkx,
+                                                const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
+    const FFPrime P = PR.p[pi];
+    const uint32_t p = P.p, pinv = P.pinv;
+    const uint32_t n = in.S;
+    const int words = (int)(n + (n >> 4) + 16);
+    const int half = threadIdx.x >= FF_THREADS;
+    FFTile th{lds + half * words, 1, 1, 0, in.s, 1, 1, half * FF_THREADS, FF_THREADS};
+    const size_t off = (size_t)pi * in.L + (size_t)blkx * in.S;
+    const uint32_t *src = (half ? C.fb : C.fa) + off;
+    uint32_t *dst = (half ? const_cast<uint32_t *>(C.fb) : C.fa) + off;
+    auto load_g = [&](int pos, int c) -> uint32_t {
+        (void)c;
+        return src[pos];
+    };
+    auto lds_ld = [&](int pos, int c) -> uint32_t { return th.s[ff_pos_addr(th, pos, c)]; };
+    auto lds_st = [&](int pos, int c, uint32_t v) { th.s[ff_pos_addr(th, pos, c)] = v; };
+    auto store_g = [&](int pos, int c, uint32_t v) {
+        (void)c;
+        dst[pos] = v;
+    };
+    ff_forward_io(th, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_g, lds_st);
+    __syncthreads();
+    const int two = (C.pair & 3) == 2;
+    const FFPairPoint pt{half ? C.cv[0] : C.cu[0], half ? C.cv[1] : C.cu[1], half ? C.cv[2] : C.cu[2],
+                         (C.pair & 4) != 0, ff_mont(P.r2, P.r2, p, pinv)};
+    const bool active = !half || two;
+    const int R = ff_nrounds(in.s);
+    const int nl0 = ff_last_nl(in.s);
+    const uint32_t *A = lds, *B = lds + words;
+    switch (nl0) {
+    case 4: pair_mid_round<4>(th, active, A, B, T->irt[pi], T->irtp[pi], p, pinv, pt, R == 1, store_g); break;
+    case 3: pair_mid_round<3>(th, active, A, B, T->irt[pi], T->irtp[pi], p, pinv, pt, R == 1, store_g); break;
+    case 2: pair_mid_round<2>(th, active, A, B, T->irt[pi], T->irtp[pi], p, pinv, pt, R == 1, store_g); break;
+    default: pair_mid_round<1>(th, active, A, B, T->irt[pi], T->irtp[pi], p, pinv, pt, R == 1, store_g); break;
+    }
+    if (!active) th.nsub = 0;
+    int b = nl0;
+    for (int r = 1; r < R; r++) {
+        __syncthreads();
+        ff_round_any<FF_R_DIT>(4, false, r == R - 1, th, b, nullptr, nullptr, T->irt[pi], T->irtp[pi], nullptr, p,
+                               pinv, lds_ld, store_g);
+        b += 4;
+    }
+}
+
+__global__ __launch_bounds__(2 * FF_THREADS, 2) void k_inner_pair(FFInner in, FFConvSet cs, const FFTables *T,
+                                                                  FFPrimes PR) {
+    extern __shared__ uint32_t lds[];
+    const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
+    inner_pair_body(in, cs.c[item], pi, blockIdx.x, T, PR, lds);
 }
 
 template <bool SMALL, bool R3>
@@ -552,6 +684,7 @@ extern "C" hipError_t ff_launch_forward(const FFPlan *pl, const FFXformSet *xs, 
 extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int nc, int ngeneral,
                                      double operand_bytes, hipStream_t st) {
     const FFInner &in = pl->inner;
+    if (cs->c[0].pair && in.small) return hipErrorInvalidValue;  // pairs take multi-pass plans
     const uint32_t n = in.small ? in.L : in.S;
     const size_t words = n + (n >> 4) + 16;
     dim3 grid(in.small ? 1 : pl->L / in.S, 3 * nc);
@@ -559,7 +692,9 @@ extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int 
     const double L4 = 4.0 * pl->L;
     ff_prof_begin(kind, st);
     const size_t lds = 2 * words * sizeof(uint32_t);
-    if (!in.small)
+    if (cs->c[0].pair) {
+        hipLaunchKernelGGL(k_inner_pair, grid, dim3(2 * FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
+    } else if (!in.small)
         hipLaunchKernelGGL((k_inner<false, false>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
     else if (in.r3)
         hipLaunchKernelGGL((k_inner<true, true>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);

@@ -246,6 +246,16 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
      "max_lg"      longest single transform 2^max_lg (12..25, default 25);
                    longer products split into transformed-once pieces (the
                    reference's table bound NTT_MAX_LG, ntt_tables.c)
+     "quad_max_lg" F(n): two bits of n per step (F(4k + d) from F(k),
+                   F(k-1) through weighted pointwise products of one
+                   transform of each) while that step's transform is at
+                   most 2^quad_max_lg words (0: off, else 13..25, default
+                   18)
+     "pair_steps"  F(n), bit 0: doubling steps the same way (one transform
+                   of F(k) and of F(k-1), the new pair combined pointwise)
+                   instead of two squarings and the linear step after the
+                   CRT (fib.c:36-62); bit 1: the last product likewise.
+                   Default 0 (the squarings measure faster)
      "dispatch"    schedules of single products and F(n): 0 the reference
                    rule (default), FF_BATCH_TUNED or FF_BATCH_LEARNED
    Returns FF_ERR_ARG for an unknown name or value. */

@@ -47,6 +47,50 @@ def test_fib_1e8_fingerprint(ff):
     assert [x % p for p in FINGERPRINT_PRIMES] == fp["mods"]
 
 
+@pytest.mark.parametrize("n", [264_889, 563_764, 2_387_614])
+def test_fib_step_lengths_predicted(ff, n, capfd):
+    """Each doubling or quadrupling step books its results' lengths from
+    their top limbs, which must fall among the last three limbs the step
+    writes. For these n a quadrupling step's result is up to 4 limbs shorter
+    than four times its input, so the step's limb count has to be tight:
+    the values are exact and the predicted lengths are confirmed without the
+    read-back rerun."""
+    assert from_limbs(ff.fibonacci(n)) == fib_int(n), n
+    assert "predicted lengths disagreed" not in capfd.readouterr().err
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_fib_pair_steps(ff, mode):
+    """Doubling steps and the last product as weighted pointwise pairs
+    (option pair_steps), against exact values and the F(10^8) fingerprint,
+    with quadrupling off and on."""
+    from oracle.oracle import FINGERPRINT_PRIMES
+    ns = [200_003, 563_764, 2_387_614, 5_000_000]
+    try:
+        ff.set_option("pair_steps", mode)
+        for q in (0, 18):
+            ff.set_option("quad_max_lg", q)
+            for n in ns:
+                assert from_limbs(ff.fibonacci(n)) == fib_int(n), (mode, q, n)
+        fp = fib_fingerprint(10 ** 8)
+        x = ff.dfibonacci(10 ** 8).value()
+        assert x.bit_length() == fp["bits"] and x % (1 << 64) == fp["low64"]
+        assert [x % p for p in FINGERPRINT_PRIMES] == fp["mods"]
+    finally:
+        ff.set_option("pair_steps", 0)
+        ff.set_option("quad_max_lg", 18)
+
+
+def test_fib_quad_off(ff):
+    """Doubling only (quad_max_lg 0), the path of the reference's fib.c."""
+    try:
+        ff.set_option("quad_max_lg", 0)
+        for n in [200_003, 563_764, 5_000_000]:
+            assert from_limbs(ff.fibonacci(n)) == fib_int(n), n
+    finally:
+        ff.set_option("quad_max_lg", 18)
+
+
 def test_fib_repeatable(ff):
     """The sync-free path (lengths predicted on the host) and repeated runs
     agree with the reference values."""

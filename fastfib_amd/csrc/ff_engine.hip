@@ -464,7 +464,8 @@ static bool pair_run(const DNum &a, const DNum &b, const ProdShape &s, int deg4,
                      const uint8_t *cv, uint64_t nout, DNum *u, DNum *v, const uint64_t *pred) {
     if (b.bits == 0 || b.bits > a.bits || s.split || s.L <= FF_TW) return false;
     FFPlan pl;
-    if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, g_inner_log, g_tile_words) != 0 ||
+    const int il = deg4 && g_quad_inner_log ? g_quad_inner_log : g_inner_log;
+    if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, il, g_tile_words) != 0 ||
         pl.npass == 0)
         return false;
     const int two = cv != nullptr;

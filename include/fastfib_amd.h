@@ -256,6 +256,8 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
                    instead of two squarings and the linear step after the
                    CRT (fib.c:36-62); bit 1: the last product likewise.
                    Default 0 (the squarings measure faster)
+     "quad_inner_log" inner block (log2, 8..12) of quadrupling transforms
+                   (0, the default: "inner_log")
      "dispatch"    schedules of single products and F(n): 0 the reference
                    rule (default), FF_BATCH_TUNED or FF_BATCH_LEARNED
    Returns FF_ERR_ARG for an unknown name or value. */

@@ -65,20 +65,34 @@ def fingerprint_ok(n: int, limbs) -> bool:
     return all(x % p == fib_mod(n, p) for p in FP_PRIMES)
 
 
-def profiled_traffic(kind: str):
-    """HBM bytes per launch of a kernel kind from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy), or
-    None when the summary does not cover it."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "fib_1e8_hbm_traffic.md")
+def _profiled(file: str, tag: str, kind: str):
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", file)
     try:
         with open(path) as f:
             for line in f:
                 w = line.split()
-                if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == kind:
+                if len(w) == 3 and w[0] == tag and w[1] == kind:
                     return int(w[2])
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(kind: str):
+    """HBM bytes per launch of a kernel kind from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy), or
+    None when the summary does not cover it."""
+    return _profiled("fib_1e8_hbm_traffic.md", "traffic_bytes_per_launch", kind)
+
+
+# One wave64 VALU instruction per 4 cycles per SIMD: 256 CUs x 4 SIMDs x 2.4 GHz / 4.
+VALU_PEAK_G = 256 * 4 * 2.4 / 4
+
+
+def profiled_valu(kind: str):
+    """VALU wave-instructions per launch of a kernel kind from the committed
+    rocprofv3 SQ_INSTS_VALU summary, or None."""
+    return _profiled("fib_1e8_valu.md", "valu_insts_per_launch", kind)
 
 
 def kernel_stats(ff) -> dict:
@@ -490,6 +504,13 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom),
                 "algorithmic_bytes_per_launch": round(d["bytes"] / d["launches"]),
                 "avg_launch_us": round(per_launch_ms * 1e3, 2), "launches": d["launches"]}
+    vi = profiled_valu(dom)
+    if vi:
+        # the transform kernels' actual bound: VALU issue (profiled instructions
+        # per launch over this run's launch time)
+        rate = vi / (per_launch_ms * 1e-3) / 1e9
+        roofline["valu"] = {"insts_per_launch": vi, "achieved": round(rate, 1), "peak": VALU_PEAK_G,
+                            "unit": "G wave64 VALU instr/s", "frac": round(rate / VALU_PEAK_G, 4)}
 
     cpu = cpu_baseline(n, args.cpu_seconds)
     batch = None

@@ -739,17 +739,33 @@ static void hn_trim(HNum &a) {
     while (a.size() > 1 && a.back() == 0) a.pop_back();
 }
 
+// a^2: the products a_i a_j (i < j) once, doubled, plus the squares a_i^2.
 static HNum hn_sqr(const HNum &a) {
     const size_t n = a.size();
     HNum r(2 * n, 0);
-    for (size_t i = 0; i < n; i++) {
+    for (size_t i = 0; i + 1 < n; i++) {
         u128 c = 0;
-        for (size_t j = 0; j < n; j++) {
+        for (size_t j = i + 1; j < n; j++) {
             const u128 t = (u128)a[i] * a[j] + r[i + j] + c;
             r[i + j] = (uint64_t)t;
             c = t >> 64;
         }
         r[i + n] = (uint64_t)c;
+    }
+    uint64_t top = 0;
+    for (size_t q = 0; q < 2 * n; q++) {  // r <<= 1
+        const uint64_t v = r[q];
+        r[q] = (v << 1) | top;
+        top = v >> 63;
+    }
+    u128 c = 0;
+    for (size_t i = 0; i < n; i++) {
+        const u128 sq = (u128)a[i] * a[i];
+        u128 t = (u128)r[2 * i] + (uint64_t)sq + c;
+        r[2 * i] = (uint64_t)t;
+        t = (u128)r[2 * i + 1] + (uint64_t)(sq >> 64) + (t >> 64);
+        r[2 * i + 1] = (uint64_t)t;
+        c = t >> 64;
     }
     hn_trim(r);
     return r;
@@ -858,11 +874,11 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
         picked[std::make_tuple(n, g_dispatch, g_dispatch_gen)] = ds;
     }
     int i = top - 1, step = 0;
-    if (predict && !g_force_len && top >= 10) {
-        // The top ten bits on the host (F(k0) for k0 < 1024: at most 12
-        // limbs), then the steps while F(k)^2 is a transform of at most
-        // 2^10 words in one workgroup, the pair never leaving LDS.
-        const int i0 = top - 9;
+    if (predict && !g_force_len && top >= g_host_bits) {
+        // The top host_bits bits on the host (F(k0) for k0 < 2^host_bits:
+        // at most 89 limbs), then the steps while F(k)^2 is a transform of
+        // at most 2^10 words in one workgroup, the pair never leaving LDS.
+        const int i0 = top - (g_host_bits - 1);
         const uint64_t k0 = n >> i0;
         HNum fa, fb;
         hn_fib_pair(k0, &fa, &fb);

@@ -161,7 +161,7 @@ hipError_t ff_launch_small_step(const FFSmallStep *a, hipStream_t st);
 #define FF_EARLY_LG 10       // largest transform: 2^10 words
 #define FF_EARLY_LIMBS 576   // limbs of a number in the chain
 #define FF_EARLY_MAXS 16     // steps per launch
-#define FF_EARLY_F0 12       // limbs of the starting pair (F(k0) for k0 < 1024)
+#define FF_EARLY_F0 96       // limbs of the starting pair (F(k0) for k0 < 2^13: 89 limbs)
 struct FFEarlyStep {
     int lg, bw;              // transform 2^lg words, coefficient width
     uint32_t conv1, conv2;   // coefficients of F(k)^2 and F(k-1)^2

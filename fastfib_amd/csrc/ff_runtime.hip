@@ -28,6 +28,7 @@ int g_max_lg = 25;
 int g_pair_steps = 0;
 int g_quad_max_lg = 18;
 int g_quad_inner_log = 0;
+int g_host_bits = 13;
 int g_dispatch_gen = 0;
 
 void ff_set_err(const char *fmt, ...) {
@@ -336,6 +337,9 @@ extern "C" int ff_set_option(const char *name, long long v) {
     } else if (!strcmp(name, "quad_inner_log")) {
         if (v != 0 && (v < 8 || v > 12)) return FF_ERR_ARG;
         g_quad_inner_log = (int)v;
+    } else if (!strcmp(name, "host_bits")) {
+        if (v < 10 || v > 13) return FF_ERR_ARG;
+        g_host_bits = (int)v;
     } else if (!strcmp(name, "dispatch")) {
         if (v != 0 && v != FF_BATCH_TUNED && v != FF_BATCH_LEARNED) return FF_ERR_ARG;
         g_dispatch = (int)v;

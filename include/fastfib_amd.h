@@ -258,6 +258,10 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
                    Default 0 (the squarings measure faster)
      "quad_inner_log" inner block (log2, 8..12) of quadrupling transforms
                    (0, the default: "inner_log")
+     "host_bits"   F(n): the top bits of n (10..13, default 13) done on the
+                   host (F(k0), k0 < 2^host_bits, at most 89 limbs) before
+                   the device chain; each bit replaces one one-workgroup
+                   step (about 10 us) by microseconds of host squaring
      "dispatch"    schedules of single products and F(n): 0 the reference
                    rule (default), FF_BATCH_TUNED or FF_BATCH_LEARNED
    Returns FF_ERR_ARG for an unknown name or value. */

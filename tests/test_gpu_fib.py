@@ -81,6 +81,18 @@ def test_fib_pair_steps(ff, mode):
         ff.set_option("quad_max_lg", 18)
 
 
+def test_fib_host_bits(ff):
+    """The top bits of n on the host (10..13) before the device chain, at
+    sizes where the chain starts right after them and further on."""
+    try:
+        for hb in (10, 11, 12, 13):
+            ff.set_option("host_bits", hb)
+            for n in [8_191, 8_192, 16_385, 563_764, 5_000_000]:
+                assert from_limbs(ff.fibonacci(n)) == fib_int(n), (hb, n)
+    finally:
+        ff.set_option("host_bits", 13)
+
+
 def test_fib_quad_off(ff):
     """Doubling only (quad_max_lg 0), the path of the reference's fib.c."""
     try:

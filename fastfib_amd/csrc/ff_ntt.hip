@@ -127,6 +127,8 @@ template <bool INV>
 struct FFOuterTw {
     static constexpr bool on = true;
     static constexpr bool lds_tables = false;
+    // outer passes have no middle round
+    __device__ uint32_t mid(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) const { return ff_mont(x, y, p, pinv); }
     const FFTables *T;
     int pi, rho, r3;
     bool top;
@@ -291,8 +293,17 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
             ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_b, lds_b_st);
             __syncthreads();
         }
-        ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, true, true,
-                   load_a, store_a);
+        if (C.pair == 1) {
+            // a weighted pair with one output: the middle round forms
+            // cu0 a^2 + cu1 ab + cu2 b^2 from a's group and B
+            FFPairMid pm;
+            pm.w0 = C.cu[0], pm.w1 = C.cu[1], pm.w2 = C.cu[2];
+            ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], B, p, pinv, true, true, load_a, store_a,
+                       pm);
+        } else {
+            ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, true,
+                       true, load_a, store_a);
+        }
         return;
     }
     if (!R3) {
@@ -692,7 +703,7 @@ extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int 
     const double L4 = 4.0 * pl->L;
     ff_prof_begin(kind, st);
     const size_t lds = 2 * words * sizeof(uint32_t);
-    if (cs->c[0].pair) {
+    if (cs->c[0].pair & 6) {  // two outputs or degree 4: the 512-thread pair kernel
         hipLaunchKernelGGL(k_inner_pair, grid, dim3(2 * FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
     } else if (!in.small)
         hipLaunchKernelGGL((k_inner<false, false>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);

@@ -149,12 +149,28 @@ enum { FF_R_DIF = 0, FF_R_DIT = 1, FF_R_MID = 2 };
 // lowest row bits, so a group holds 2^NL consecutive rows, whose twiddles
 // are base * g^rev(j): the provider computes them from two table roots per
 // group instead of one per element. FFNoTw: no twiddles (inner blocks).
+// mid(x, y): the pointwise step of the middle round, x of the array being
+// transformed, y of the other spectrum (or x itself for a square).
 struct FFNoTw {
     static constexpr bool on = false;
     static constexpr bool lds_tables = false;  // the root tables read from HBM
     template <int NL>
     __device__ void group(int, int, uint32_t *) const {}
     __device__ uint32_t one(int, int) const { return 0; }
+    __device__ uint32_t mid(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) const { return ff_mont(x, y, p, pinv); }
+};
+
+// A weighted pair with one output: u = w0 x^2 + w1 xy + w2 y^2 (weights 0..2).
+struct FFPairMid : FFNoTw {
+    int w0, w1, w2;
+    __device__ static uint32_t scale(uint32_t v, int w, uint32_t p) {
+        const uint32_t v2 = ff_add(v, v, p);
+        return w == 0 ? 0u : (w == 1 ? v : v2);
+    }
+    __device__ uint32_t mid(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) const {
+        const uint32_t xx = ff_mont(x, x, p, pinv), xy = ff_mont(x, y, p, pinv), yy = ff_mont(y, y, p, pinv);
+        return ff_add(ff_add(scale(xx, w0, p), scale(xy, w1, p), p), scale(yy, w2, p), p);
+    }
 };
 
 // nl-bit reversal of j, folded at compile time in unrolled loops.
@@ -214,7 +230,7 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
 #pragma unroll
             for (int j = 0; j < G; j++) {
                 const int pos = pos0 + (base | (j << b_lo));
-                v[j] = ff_mont(v[j], B ? B[ff_pos_addr(t, pos, c)] : v[j], p, pinv);
+                v[j] = gt.mid(v[j], B ? B[ff_pos_addr(t, pos, c)] : v[j], p, pinv);
             }
             ff_dit_group<NL, Gt::lds_tables>(v, low, b_lo, irt, irtp, p, pinv);
         }

@@ -25,7 +25,7 @@ int g_tile_words = 4096;
 uint64_t g_force_len = 0;
 int g_dispatch = 0;
 int g_max_lg = 25;
-int g_pair_steps = 0;
+int g_pair_steps = 2;
 int g_quad_max_lg = 18;
 int g_quad_inner_log = 0;
 int g_host_bits = 13;

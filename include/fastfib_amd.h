@@ -254,8 +254,10 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
      "pair_steps"  F(n), bit 0: doubling steps the same way (one transform
                    of F(k) and of F(k-1), the new pair combined pointwise)
                    instead of two squarings and the linear step after the
-                   CRT (fib.c:36-62); bit 1: the last product likewise.
-                   Default 0 (the squarings measure faster)
+                   CRT (fib.c:36-62); bit 1: the last product likewise
+                   (F(2k) = a^2 + 2ab, F(2k+1) = 2a^2 + 2ab + b^2 formed in
+                   the product's middle round, no factor pass). Default 2:
+                   the doubling steps measure faster as squarings
      "quad_inner_log" inner block (log2, 8..12) of quadrupling transforms
                    (0, the default: "inner_log")
      "host_bits"   F(n): the top bits of n (10..13, default 13) done on the

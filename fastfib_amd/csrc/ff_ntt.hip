@@ -369,12 +369,15 @@ struct FFPairPoint {
     uint32_t r3;  // R^3 mod p
     __device__ uint32_t operator()(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) const {
         if (deg4) {
-            const uint32_t aa = ff_mont(x, x, p, pinv), ab = ff_mont(x, y, p, pinv), bb = ff_mont(y, y, p, pinv);
-            x = ff_add(aa, ff_add(ab, ab, p), p);
+            // x' = a (a + 2b), y' = a^2 + b^2
+            const uint32_t aa = ff_mont(x, x, p, pinv), bb = ff_mont(y, y, p, pinv);
+            const uint32_t y2 = ff_add(y, y, p);
+            x = ff_mont(x, ff_add(x, y2, p), p, pinv);
             y = ff_add(aa, bb, p);
         }
-        const uint32_t xx = ff_mont(x, x, p, pinv), xy = ff_mont(x, y, p, pinv), yy = ff_mont(y, y, p, pinv);
-        uint32_t u = ff_add(ff_add(ff_wmul(xx, w0, p), ff_wmul(xy, w1, p), p), ff_wmul(yy, w2, p), p);
+        // w0 x^2 + w1 xy + w2 y^2 = x (w0 x + w1 y) + w2 y^2
+        uint32_t u = ff_mont(x, ff_add(ff_wmul(x, w0, p), ff_wmul(y, w1, p), p), p, pinv);
+        if (w2) u = ff_add(u, ff_wmul(ff_mont(y, y, p, pinv), w2, p), p);
         if (deg4) u = ff_mont(u, r3, p, pinv);
         return u;
     }

@@ -160,16 +160,20 @@ struct FFNoTw {
     __device__ uint32_t mid(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) const { return ff_mont(x, y, p, pinv); }
 };
 
-// A weighted pair with one output: u = w0 x^2 + w1 xy + w2 y^2 (weights 0..2).
+// w v mod p for a weight w of 0, 1 or 2.
+__device__ __forceinline__ uint32_t ff_w012(uint32_t v, int w, uint32_t p) {
+    const uint32_t v2 = ff_add(v, v, p);
+    return w == 0 ? 0u : (w == 1 ? v : v2);
+}
+
+// A weighted pair with one output: u = w0 x^2 + w1 xy + w2 y^2 (weights 0..2)
+// as x (w0 x + w1 y) + w2 y^2: one product when w2 = 0 (F(2k) = a (a + 2b)),
+// two otherwise.
 struct FFPairMid : FFNoTw {
     int w0, w1, w2;
-    __device__ static uint32_t scale(uint32_t v, int w, uint32_t p) {
-        const uint32_t v2 = ff_add(v, v, p);
-        return w == 0 ? 0u : (w == 1 ? v : v2);
-    }
     __device__ uint32_t mid(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) const {
-        const uint32_t xx = ff_mont(x, x, p, pinv), xy = ff_mont(x, y, p, pinv), yy = ff_mont(y, y, p, pinv);
-        return ff_add(ff_add(scale(xx, w0, p), scale(xy, w1, p), p), scale(yy, w2, p), p);
+        const uint32_t u = ff_mont(x, ff_add(ff_w012(x, w0, p), ff_w012(y, w1, p), p), p, pinv);
+        return w2 ? ff_add(u, ff_w012(ff_mont(y, y, p, pinv), w2, p), p) : u;
     }
 };
 

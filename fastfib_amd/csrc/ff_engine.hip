@@ -929,8 +929,10 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
     }
     if (!fk.d) fkm1 = dnum_u64(0), fk = dnum_u64(1);
     for (; i >= 1; i--, step++) {
-        if (pair_free() && i >= 2) {
-            // bits i and i - 1 at once
+        if (!g_force_len && i >= 2) {
+            // bits i and i - 1 at once (own shapes and the default schedule,
+            // whatever the schedule source: the up-front picks of the two
+            // doubling steps it replaces go unused)
             const int d = (int)((n >> (i - 1)) & 3);
             uint64_t pq[2] = {0, 0};
             if (predict) pq[0] = P(4 * k + d - 1, FB_F), pq[1] = P(4 * k + d, FB_F);

@@ -425,9 +425,10 @@ static void square_step(const DNum &fk, const DNum &fkm1, int k_odd, int bit, DN
 
 // ------------------------------------------------------------------ pair products
 // u = cu0 a^2 + cu1 ab + cu2 b^2 and v = cv0 a^2 + cv1 ab + cv2 b^2 (b <= a)
-// from one forward transform of each of a and b: the three pointwise
-// products are weighted and summed before the inverse transforms
-// (k_inner's pair mode), and the plain CRT writes u and v. A doubling step
+// from one forward transform of each of a and b: the pointwise products are
+// weighted and summed before the inverse transforms (k_inner_pair for two
+// outputs or degree 4, the general product's middle round for one), and the
+// plain CRT writes u and v. A doubling step
 // is such a pair, with no linear pass and no constant after its CRT:
 //   F(2k-1) = a^2 + b^2, F(2k) = a^2 + 2ab, F(2k+1) = 2a^2 + 2ab + b^2
 // for a = F(k), b = F(k-1) (the identities behind fib.c:36-62's step). The

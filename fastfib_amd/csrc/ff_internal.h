@@ -26,10 +26,13 @@ struct FFXformSet {
 // One product for the inner kernels: fa holds a's forward transform (or, for
 // one-block transforms, receives it from asrc) and gets the product's; fb /
 // bsrc = nullptr squares.
-// pair != 0 (multi-pass plans only): fa, fb hold the transforms of a and b,
-// and get u = cu0 a^2 + cu1 ab + cu2 b^2 and (pair = 2) v = cv0 a^2 +
-// cv1 ab + cv2 b^2, weights of 0, 1 or 2 combining the three pointwise
-// products before the inverse transform.
+// pair != 0 (multi-pass plans only): fa, fb hold the transforms of a and b;
+// fa gets u = cu0 X + cu1 Y + cu2 Z and, with pair bit 2, fb gets v =
+// cv0 X + cv1 Y + cv2 Z, combined pointwise before the inverse transform.
+// X, Y, Z = a^2, ab, b^2, or with pair bit 4 (degree 4) x^2, xy, y^2 for
+// x = a^2 + 2ab, y = a^2 + b^2. pair = 1 (one output, degree 2, weights 0..2)
+// runs in the general product's middle round; the others in k_inner_pair
+// (weights below 8).
 struct FFConv {
     const uint32_t *asrc, *bsrc;
     uint32_t anw, bnw;

@@ -103,6 +103,26 @@ def test_fib_quad_off(ff):
         ff.set_option("quad_max_lg", 18)
 
 
+def test_fib_fingerprint_sweep(ff, capfd):
+    """250 seeded n, log-uniform from 2^13 to 2^27, odd and even: every mix of
+    host bits, one-workgroup steps, quadruplings, doubling steps and the
+    last product's two forms. Each result's bit length, low 64 bits and
+    residues mod four primes against the fingerprint (fast doubling on
+    residues), and every predicted length confirmed (no read-back rerun)."""
+    import math
+
+    from oracle.oracle import FINGERPRINT_PRIMES
+    rng = random.Random(20261203)
+    ns = sorted({int(math.exp(rng.uniform(math.log(2 ** 13), math.log(2 ** 27)))) for _ in range(250)})
+    for n in ns:
+        x = ff.dfibonacci(n).value()
+        fp = fib_fingerprint(n)
+        assert x.bit_length() == fp["bits"], n
+        assert x % (1 << 64) == fp["low64"], n
+        assert [x % p for p in FINGERPRINT_PRIMES] == fp["mods"], n
+    assert "predicted lengths disagreed" not in capfd.readouterr().err
+
+
 def test_fib_repeatable(ff):
     """The sync-free path (lengths predicted on the host) and repeated runs
     agree with the reference values."""

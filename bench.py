@@ -87,6 +87,8 @@ def profiled_traffic(kind: str):
 
 # One wave64 VALU instruction per 4 cycles per SIMD: 256 CUs x 4 SIMDs x 2.4 GHz / 4.
 VALU_PEAK_G = 256 * 4 * 2.4 / 4
+# 32-bit integer multiplies: one per lane per cycle, 256 CUs x 64 lanes x 2.4 GHz.
+INT32_MUL_PEAK_T = 256 * 64 * 2.4e9 / 1e12
 
 
 def profiled_valu(kind: str):
@@ -106,7 +108,9 @@ def kernel_stats(ff) -> dict:
         if L.ff_prof_read(k, ctypes.byref(name), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)) != 0:
             break
         if n.value:
-            out[name.value.decode()] = {"launches": n.value, "ms": ms.value, "bytes": b.value}
+            m = ctypes.c_double()
+            L.ff_prof_read_mults(k, ctypes.byref(m))
+            out[name.value.decode()] = {"launches": n.value, "ms": ms.value, "bytes": b.value, "mults": m.value}
         k += 1
     return out
 
@@ -504,6 +508,13 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom),
                 "algorithmic_bytes_per_launch": round(d["bytes"] / d["launches"]),
                 "avg_launch_us": round(per_launch_ms * 1e3, 2), "launches": d["launches"]}
+    if d.get("mults"):
+        # algorithmic multiplies booked by the launcher (3 per twiddled
+        # butterfly, 4 per pointwise Montgomery product) over the launch time
+        mr = d["mults"] / d["launches"] / (per_launch_ms * 1e-3) / 1e12
+        roofline["int32_mul"] = {"mults_per_launch": round(d["mults"] / d["launches"]), "achieved": round(mr, 2),
+                                 "peak": round(INT32_MUL_PEAK_T, 1), "unit": "T int32 mul/s",
+                                 "frac": round(mr / INT32_MUL_PEAK_T, 4)}
     vi = profiled_valu(dom)
     if vi:
         # the transform kernels' actual bound: VALU issue (profiled instructions

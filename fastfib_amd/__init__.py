@@ -73,6 +73,7 @@ def lib() -> ctypes.CDLL:
     L.ff_prof_enable.argtypes = [ctypes.c_int]
     L.ff_prof_read.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_long),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.ff_prof_read_mults.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 

@@ -127,6 +127,8 @@ extern "C" {
 // stream and books its algorithmic bytes under its kind.
 void ff_prof_begin(int kind, hipStream_t st);
 void ff_prof_end(int kind, double bytes, hipStream_t st);
+// the algorithmic 32-bit multiplies of the launch the next ff_prof_end closes
+void ff_prof_mults(double mults);
 
 // ff_ntt.hip
 int ff_tables_init(void);

@@ -695,6 +695,38 @@ extern "C" hipError_t ff_launch_forward(const FFPlan *pl, const FFXformSet *xs, 
 // Algorithmic bytes per prime: the inner kernel reads a's (and b's)
 // transform and writes the product's, 4L each; the one-block kernel reads
 // the operands' limbs instead.
+// Algorithmic 32-bit multiplies of one inner launch, three primes: per
+// transform of an S-point block (S/2)(log2 S - 1) twiddled butterflies (the
+// lowest level multiplies by 1) at 3 each (a Montgomery product by a root
+// with its companion), and 4 per pointwise Montgomery product. A square
+// transforms one array each way with one product per point; a general
+// product two forward, one back; a weighted pair two forward, one or two
+// back, with one or two products per point and output (x (w0 x + w1 y) +
+// w2 y^2); at degree 4, three more for x, y and one for the scale.
+static double conv_mults(const FFPlan *pl, const FFConvSet *cs, int nc) {
+    const FFInner &in = pl->inner;
+    const double S = in.small ? (double)in.L : (double)in.S, blocks = in.small ? 1.0 : (double)pl->L / in.S;
+    const double bf = S / 2 * (in.s - 1) * 3;  // per block transform
+    double m = 0;
+    for (int i = 0; i < nc; i++) {
+        const FFConv &c = cs->c[i];
+        double fwd, inv, pw;
+        if (!c.pair) {
+            const bool sq = in.small ? c.bsrc == nullptr : c.fb == nullptr;
+            fwd = sq ? 1 : 2, inv = 1, pw = 1;
+        } else {
+            const int outs = (c.pair & 2) ? 2 : 1;
+            fwd = 2, inv = outs, pw = 0;
+            for (int o = 0; o < outs; o++) {
+                const uint8_t *w = o ? c.cv : c.cu;
+                pw += 1 + (w[2] ? 1 : 0) + ((c.pair & 4) ? 4 : 0);
+            }
+        }
+        m += blocks * ((fwd + inv) * bf + S * pw * 4);
+    }
+    return 3 * m;
+}
+
 extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int nc, int ngeneral,
                                      double operand_bytes, hipStream_t st) {
     const FFInner &in = pl->inner;
@@ -714,6 +746,7 @@ extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int 
         hipLaunchKernelGGL((k_inner<true, true>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
     else
         hipLaunchKernelGGL((k_inner<true, false>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
+    ff_prof_mults(conv_mults(pl, cs, nc));
     ff_prof_end(kind, in.small ? 3.0 * (operand_bytes + nc * L4) : 3.0 * (nc + ngeneral + nc) * L4, st);
     return hipGetLastError();
 }

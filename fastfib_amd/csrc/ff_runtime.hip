@@ -178,13 +178,14 @@ uint64_t *ff_pinned_results(size_t words) {
 
 struct ProfRec {
     int kind;
-    double bytes;
+    double bytes, mults;
     hipEvent_t a, b;
 };
 static int g_prof_on = 0;
 static std::vector<ProfRec> g_prof_pending;
 static std::vector<hipEvent_t> g_prof_spare;
-static double g_prof_ms[FK_COUNT], g_prof_bytes[FK_COUNT];
+static double g_prof_ms[FK_COUNT], g_prof_bytes[FK_COUNT], g_prof_mults[FK_COUNT];
+static double g_prof_next_mults = 0;  // the open launch's algorithmic 32-bit multiplies
 static long g_prof_n[FK_COUNT];
 static hipEvent_t g_prof_open = nullptr;
 static int g_prof_paused = 0;
@@ -209,11 +210,15 @@ extern "C" void ff_prof_begin(int kind, hipStream_t st) {
     FF_CHECK(hipEventRecord(g_prof_open, st));
 }
 
+extern "C" void ff_prof_mults(double mults) { g_prof_next_mults = mults; }
+
 extern "C" void ff_prof_end(int kind, double bytes, hipStream_t st) {
+    const double mults = g_prof_next_mults;
+    g_prof_next_mults = 0;
     if (!g_prof_on || g_prof_paused || !g_prof_open) return;
     hipEvent_t e = prof_event();
     FF_CHECK(hipEventRecord(e, st));
-    g_prof_pending.push_back(ProfRec{kind, bytes, g_prof_open, e});
+    g_prof_pending.push_back(ProfRec{kind, bytes, mults, g_prof_open, e});
     g_prof_open = nullptr;
 }
 
@@ -225,6 +230,7 @@ static void prof_collect(void) {
         FF_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
         g_prof_ms[r.kind] += ms;
         g_prof_bytes[r.kind] += r.bytes;
+        g_prof_mults[r.kind] += r.mults;
         g_prof_n[r.kind]++;
         g_prof_spare.push_back(r.a);
         g_prof_spare.push_back(r.b);
@@ -250,7 +256,17 @@ extern "C" void ff_prof_reset(void) {
     prof_collect();
     memset(g_prof_ms, 0, sizeof g_prof_ms);
     memset(g_prof_bytes, 0, sizeof g_prof_bytes);
+    memset(g_prof_mults, 0, sizeof g_prof_mults);
     memset(g_prof_n, 0, sizeof g_prof_n);
+}
+
+extern "C" int ff_prof_read_mults(int k, double *mults) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (k < 0 || k >= FK_COUNT) return FF_ERR_ARG;
+    ff_ensure_init();
+    prof_collect();
+    *mults = g_prof_mults[k];
+    return FF_OK;
 }
 
 extern "C" int ff_prof_read(int k, const char **name, long *launches, double *ms, double *bytes) {

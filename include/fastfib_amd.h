@@ -277,6 +277,12 @@ FF_API int ff_set_option(const char *name, long long value);
 FF_API int ff_prof_enable(int on);
 FF_API void ff_prof_reset(void);
 FF_API int ff_prof_read(int k, const char **name, long *launches, double *ms, double *bytes);
+/* The same kind's total algorithmic 32-bit integer multiplies, booked by the
+   launchers of the transform kernels that count them (the inner blocks: 3 per
+   twiddled butterfly, a Montgomery product by a root with its precomputed
+   companion; 4 per pointwise Montgomery product), 0 for the others. No
+   counterpart in the reference. */
+FF_API int ff_prof_read_mults(int k, double *mults);
 
 #ifdef __cplusplus
 }

@@ -153,6 +153,34 @@ static void build_lists(ff_batch *B) {
                            (uint32_t)s.L, (int)s.bw};
     }
 
+    // one-block products sharing their b operand: b's spectrum once, by a
+    // forward-only item launched ahead of them, read by every sharer
+    std::vector<FFInnerItem> pre;
+    std::vector<const FFPlan *> pre_plan;
+    std::vector<int> bspec(n, 0);
+    {
+        std::map<std::tuple<const uint64_t *, uint64_t, uint64_t, unsigned, const FFPlan *>, std::vector<size_t>> users;
+        for (size_t i : order) {
+            const ff_prod &p = B->items[i];
+            if (!p.b || p.b == p.a || !plan[i]->inner.small) continue;
+            users[std::make_tuple(p.b, (uint64_t)p.nb, (uint64_t)p.bits_b, B->shape[i].bw, plan[i])].push_back(i);
+        }
+        for (auto &u : users) {
+            if (u.second.size() < 2) continue;
+            const FFPlan &pl = *plan[u.second[0]];
+            const ff_prod &p = B->items[u.second[0]];
+            uint32_t *spec = (uint32_t *)ff_dalloc(al(3 * (size_t)pl.L * 4));
+            B->work.push_back(spec);
+            FFInnerItem ii;
+            ii.in = pl.inner;
+            ii.c = FFConv{(const uint32_t *)p.b, nullptr, (uint32_t)(2 * p.nb), 0, spec, nullptr};
+            ii.c.bspec = 2;
+            pre.push_back(ii);
+            pre_plan.push_back(&pl);
+            for (size_t i : u.second) fb[i] = spec, bspec[i] = 1;
+        }
+    }
+
     // host images of every list, then one device buffer
     std::vector<char> img;
     auto put = [&](const void *src, size_t bytes) {
@@ -250,6 +278,7 @@ static void build_lists(ff_batch *B) {
             ii.in = pl.inner;
             ii.c = FFConv{(const uint32_t *)p.a, sq ? nullptr : (const uint32_t *)p.b, (uint32_t)(2 * p.na),
                           (uint32_t)(2 * (sq ? p.na : p.nb)), fa[i], fb[i]};
+            ii.c.bspec = bspec[i];
             its.push_back(ii);
             start.push_back(blocks);
             blocks += 3 * (small ? 1 : pl.L / pl.inner.S);
@@ -267,6 +296,36 @@ static void build_lists(ff_batch *B) {
         pend.push_back(pd);
     };
 
+    // the shared one-block spectra, by radix and occupancy class
+    auto pre_all = [&](int r3sel) {
+        std::set<int> occs;
+        for (size_t k = 0; k < pre.size(); k++)
+            if (pre_plan[k]->inner.r3 == r3sel) occs.insert(occ(ff_inner_lds_bytes(&pre_plan[k]->inner)));
+        for (int oc : occs) {
+            std::vector<FFInnerItem> its;
+            std::vector<uint32_t> start;
+            uint32_t blocks = 0;
+            size_t lds = 0;
+            double bytes = 0;
+            for (size_t k = 0; k < pre.size(); k++) {
+                const FFPlan &pl = *pre_plan[k];
+                if (pl.inner.r3 != r3sel || occ(ff_inner_lds_bytes(&pl.inner)) != oc) continue;
+                its.push_back(pre[k]);
+                start.push_back(blocks);
+                blocks += 3;
+                const size_t l = ff_inner_lds_bytes(&pl.inner);
+                lds = l > lds ? l : lds;
+                bytes += 3.0 * (4.0 * pre[k].c.anw + 4.0 * pl.L);
+            }
+            Pending pd{};
+            pd.L.kind = 1, pd.L.small = 1, pd.L.r3 = r3sel, pd.L.n = (int)its.size();
+            pd.L.blocks = blocks, pd.L.lds = lds, pd.L.bytes = bytes;
+            pd.items_off = put(its.data(), its.size() * sizeof(FFInnerItem));
+            pd.start_off = put(start.data(), start.size() * 4);
+            pend.push_back(pd);
+        }
+    };
+
     // every (pass, radix-3) list once per occupancy class
     auto outer_all = [&](int pass, bool inv, int r3sel) {
         for (int oc : outer_occs(pass, r3sel)) outer_list(pass, inv, r3sel, oc);
@@ -282,6 +341,8 @@ static void build_lists(ff_batch *B) {
             outer_all(pass, false, 0);
         }
     }
+    pre_all(1);
+    pre_all(0);
     inner_all(0, 0);
     inner_all(1, 1);
     inner_all(1, 0);

@@ -33,6 +33,9 @@ struct FFXformSet {
 // x = a^2 + 2ab, y = a^2 + b^2. pair = 1 (one output, degree 2, weights 0..2)
 // runs in the general product's middle round; the others in k_inner_pair
 // (weights below 8).
+// bspec (one-block plans): 1 when fb holds b's spectrum, tile position i at
+// fb[prime L + i], computed once for every product sharing b; 2 for the item
+// computing it: a's forward transform (from asrc) stored that way to fa.
 struct FFConv {
     const uint32_t *asrc, *bsrc;
     uint32_t anw, bnw;
@@ -40,6 +43,7 @@ struct FFConv {
     const uint32_t *fb;
     int pair;
     uint8_t cu[3], cv[3];
+    int bspec;
 };
 struct FFConvSet {
     FFConv c[FF_MAXB];

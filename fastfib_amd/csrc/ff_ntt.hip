@@ -306,14 +306,26 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
         }
         return;
     }
+    const bool bgiven = C.bspec == 1;  // B's spectrum read, not computed
+    if (C.bspec == 2) {
+        // a's spectrum alone, stored by tile position for the products sharing it
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds_a_st((int)i, 0, load_a((int)i, 0));
+        if (R3) ff_radix3_layer<false>(ta, T, pi, P, in.rstep3);
+        else __syncthreads();
+        ff_forward_io(ta, T->rt[pi], T->rtp[pi], p, pinv, false, false, lds_a, lds_a_st);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = lds_a((int)i, 0);
+        return;
+    }
     if (!R3) {
         // one block read from limbs: unpack element by element into LDS
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
             lds_a_st((int)i, 0, load_a((int)i, 0));
-            if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
+            if (bgiven) lds_b_st((int)i, 0, C.fb[off + i]);
+            else if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
         }
         __syncthreads();
-        if (!square) {
+        if (!square && !bgiven) {
             ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, false, false, lds_a, lds_b_st);
             __syncthreads();
         }
@@ -324,10 +336,11 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
     // 3 * 2^k in one block: the radix-3 layers work on the LDS tiles
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         lds_a_st((int)i, 0, load_a((int)i, 0));
-        if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
+        if (bgiven) lds_b_st((int)i, 0, C.fb[off + i]);
+        else if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
     }
     ff_radix3_layer<false>(ta, T, pi, P, in.rstep3);
-    if (!square) {
+    if (!square && !bgiven) {
         ff_radix3_layer<false>(tb, T, pi, P, in.rstep3);
         ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, false, false, lds_a, lds_b_st);
         __syncthreads();

@@ -146,6 +146,37 @@ def test_uneven_4096_residues(ff):
         bt.close()
 
 
+def test_shared_operand_one_block(ff):
+    """Products short enough for one-block transforms (L <= 4096, radix 2
+    and 3) sharing their b: one spectrum of b per group, computed ahead of
+    the products and read by each; groups of one, other operands and
+    squares mixed in; every form, then new values in place."""
+    rng = np.random.default_rng(77)
+    Bs = [operand_limbs(rng, 9000), operand_limbs(rng, 3000)]
+    dbs = [ff.Device(x) for x in Bs]
+    sizes = [int(s) for s in np.exp(rng.uniform(np.log(1000), np.log(60000), 24))]
+    As = [operand_limbs(rng, s) for s in sizes]
+    das = [ff.Device(a) for a in As]
+    pairs = [(das[k], dbs[k % 2]) for k in range(len(As))] + [(das[0], None), (dbs[1], None)]
+    for flags in (0, 1, 2):
+        want = [from_limbs(As[k]) * from_limbs(Bs[k % 2]) for k in range(len(As))]
+        want += [from_limbs(As[0]) ** 2, from_limbs(Bs[1]) ** 2]
+        bt = ff.Batch(pairs, flags)
+        bt.run()
+        bt.finish()
+        assert bt.results() == want, flags
+        # new values of the same lengths in place: the spectra are formed again
+        for k in range(2):
+            Bs[k] = operand_limbs(rng, 9000 if k == 0 else 3000)
+            dbs[k].assign(Bs[k])
+        want = [from_limbs(As[k]) * from_limbs(Bs[k % 2]) for k in range(len(As))]
+        want += [from_limbs(As[0]) ** 2, from_limbs(Bs[1]) ** 2]
+        bt.run()
+        bt.finish()
+        assert bt.results() == want, ("rerun", flags)
+        bt.close()
+
+
 def test_shared_operand(ff):
     """Many products with one common b (transformed once for all of them),
     mixed with squares and other products, in every form."""

@@ -7,12 +7,14 @@
 //   - the three primes' transforms of F(k) and F(k-1), stacked as the two
 //     sub-transforms of one tile per prime (threads [320 pi, 320 pi + 320)
 //     own prime pi, the last prime 384 threads: whole wavefronts), with the
-//     pointwise squares between forward and inverse (fused rounds);
-//   - the CRT of both squares and the linear step of the fused small step
-//     (fib.c:36-62), every limb in a thread;
+//     new pair formed pointwise between forward and inverse (fused rounds):
+//     F(2k - 1) = F(k)^2 + F(k-1)^2, F(2k) = F(k) (F(k) + 2 F(k-1)) and
+//     F(2k + 1) = F(2k) + F(2k - 1), the identities behind the reference's
+//     doubling step (fib.c:36-62) without its (-1)^k term;
+//   - the CRT of both new numbers, every limb in a thread;
 //   - the new pair (F(k'-1), F(k')) replaces the old one in LDS.
-// The host starts the chain at F(k0 - 1), F(k0) for the top ten bits of n
-// (k0 < 1024, at most 12 limbs) and passes each step's sizes, predicted
+// The host starts the chain at F(k0 - 1), F(k0) for the top bits of n
+// (k0 < 2^13, at most 89 limbs) and passes each step's sizes, predicted
 // from the closed form; every step books its new pair's length keys, which
 // the host checks against the predictions at the end of F(n) like those of
 // the other steps. The root tables' first 2^10 entries are copied into LDS
@@ -33,11 +35,77 @@ struct FFLdsTables : FFNoTw {
     static constexpr bool lds_tables = true;
 };
 
+// The middle round over both stacked sub-transforms at once: a = F(k) (sub
+// 0) and b = F(k - 1) (sub 1) at one frequency in one thread. The pointwise
+// step forms x = a (a + 2b) = F(2k) and y = a^2 + b^2 = F(2k - 1) and keeps
+// (y, x), or (x, x + y) = (F(2k), F(2k + 1)) for bit 1: the new pair comes
+// straight out of the inverse transforms, with no linear step after them.
+// Coefficients of x + y are below 5 N 2^(2 bw) <= 5 * 2^9 * 2^70 < 2^82 for
+// N <= 2^9 coefficients of F(k), well within p1 p2 p3 > 2^92.
+template <int NL>
+__device__ __forceinline__ void early_pair_mid(const FFTile &t, const uint32_t *const *tw, uint32_t p, uint32_t pinv,
+                                               int bit) {
+    constexpr int G = 1 << NL;
+    const int ngroups = 1 << (t.rho - NL), off = 1 << t.rho;
+    for (int g = ff_tile_tid(t); g < ngroups; g += ff_tile_nthr(t)) {
+        uint32_t va[G], vb[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            va[j] = t.s[ff_pos_addr(t, (g << NL) | j, 0)];
+            vb[j] = t.s[ff_pos_addr(t, off + ((g << NL) | j), 0)];
+        }
+        ff_dif_group<NL, true>(va, 0, 0, tw[0], tw[1], p, pinv);
+        ff_dif_group<NL, true>(vb, 0, 0, tw[0], tw[1], p, pinv);
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const uint32_t a = va[j], b = vb[j];
+            const uint32_t x = ff_mont(a, ff_add(a, ff_add(b, b, p), p), p, pinv);
+            const uint32_t y = ff_add(ff_mont(a, a, p, pinv), ff_mont(b, b, p, pinv), p);
+            va[j] = bit ? x : y;
+            vb[j] = bit ? ff_add(x, y, p) : x;
+        }
+        ff_dit_group<NL, true>(va, 0, 0, tw[2], tw[3], p, pinv);
+        ff_dit_group<NL, true>(vb, 0, 0, tw[2], tw[3], p, pinv);
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            t.s[ff_pos_addr(t, (g << NL) | j, 0)] = va[j];
+            t.s[ff_pos_addr(t, off + ((g << NL) | j), 0)] = vb[j];
+        }
+    }
+}
+
+// Forward transforms of both sub-transforms in radix-4 rounds, the pair
+// middle round, inverse rounds; the tables (rt, rtp, irt, irtp) in LDS.
+__device__ void early_pair_conv(const FFTile &t, const uint32_t *const *tw, uint32_t p, uint32_t pinv, int bit) {
+    constexpr int rl = 2;
+    auto ld = [&](int pos, int c) -> uint32_t { return t.s[ff_pos_addr(t, pos, c)]; };
+    auto sto = [&](int pos, int c, uint32_t v) { t.s[ff_pos_addr(t, pos, c)] = v; };
+    const int R = (t.rho + rl - 1) / rl;
+    int b = t.rho;
+    for (int r = 0; r < R - 1; r++) {
+        b -= rl;
+        if (r) __syncthreads();
+        ff_round_any<FF_R_DIF>(rl, false, false, t, b, tw[0], tw[1], nullptr, nullptr, nullptr, p, pinv, ld, sto,
+                               FFLdsTables{});
+    }
+    const int nl0 = t.rho % rl == 0 ? rl : t.rho % rl;
+    if (R > 1) __syncthreads();
+    if (nl0 == 2) early_pair_mid<2>(t, tw, p, pinv, bit);
+    else early_pair_mid<1>(t, tw, p, pinv, bit);
+    b = nl0;
+    for (int r = 1; r < R; r++) {
+        __syncthreads();
+        ff_round_any<FF_R_DIT>(rl, false, false, t, b, nullptr, nullptr, tw[2], tw[3], nullptr, p, pinv, ld, sto,
+                               FFLdsTables{});
+        b += rl;
+    }
+}
+
 struct EarlyShared {
     uint32_t tw[3][4][EARLY_L];       // per prime: rt, rtp, irt, irtp
     uint64_t num[2][FF_EARLY_LIMBS];  // F(k - 1), F(k)
     uint32_t tile[3][EARLY_TILE];     // per prime: F(k), F(k - 1) stacked
-    uint32_t y[2][3 * EARLY_L];       // residues of F(k)^2, F(k-1)^2: [prime][coefficient]
+    uint32_t y[2][3 * EARLY_L];       // residues of the new pair: [prime][coefficient]
     SSShared<EARLY_T> ss;
 };
 
@@ -76,21 +144,18 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
                 (i >> lg) ? ff_load_coef(w2, st.nw2, pos, st.bw, P) : ff_load_coef(w1, st.nw1, pos, st.bw, P);
         }
         __syncthreads();
-        auto ld = [&](int pos, int c) -> uint32_t { return tl.s[ff_pos_addr(tl, pos, c)]; };
-        auto sto = [&](int pos, int c, uint32_t v) { tl.s[ff_pos_addr(tl, pos, c)] = v; };
-        ff_conv_io(tl, E.tw[pi][0], E.tw[pi][1], E.tw[pi][2], E.tw[pi][3], nullptr, p, pinv, false, false, ld, sto,
-                   FFLdsTables{}, 2);
+        const uint32_t *tw[4] = {E.tw[pi][0], E.tw[pi][1], E.tw[pi][2], E.tw[pi][3]};
+        early_pair_conv(tl, tw, p, pinv, st.bit);
         __syncthreads();
         const uint32_t sc = a.sc[lg][pi];
         for (int i = t - tid0; i < 2 * L; i += nthr)
             E.y[i >> lg][pi * L + (i & (L - 1))] = ff_mont(tl.s[ff_pos_addr(tl, i, 0)], sc, p, pinv);
         __syncthreads();
-        uint64_t s1[SS_CH], s2[SS_CH], K[SS_CH], C[SS_CH];
+        // both products of F(k) by itself and by F(k - 1): conv1 coefficients
+        uint64_t s1[SS_CH], s2[SS_CH];
         ss_crt(E.ss, R, E.y[0], L, st.bw, st.conv1, st.len, G, s1);
-        ss_crt(E.ss, R, E.y[1], L, st.bw, st.conv2, st.len, G, s2);
-        ss_linear(E.ss, R, s1, s2, st.cb0, st.len, K, C, st.bit);
-        // bit 0: (F(2k-1), F(2k)) = (A, C); bit 1: (F(2k), F(2k+1)) = (C, B)
-        const uint64_t lo = st.bit ? C[0] : K[0], hi = st.bit ? K[0] : C[0];
+        ss_crt(E.ss, R, E.y[1], L, st.bw, st.conv1, st.len, G, s2);
+        const uint64_t lo = s1[0], hi = s2[0];  // len <= FF_EARLY_LIMBS: one chunk
         if (t < (int)st.len) {
             E.num[0][t] = lo;
             E.num[1][t] = hi;
@@ -110,7 +175,7 @@ hipError_t ff_launch_fib_early(const FFEarlyArgs *a, hipStream_t st) {
         const FFEarlyStep &e = a->st[s];
         // every size within the LDS arrays and the one-chunk CRT
         if (e.lg < 4 || e.lg > FF_EARLY_LG || e.bw < 32 || e.bw > 35 || e.len > FF_EARLY_LIMBS ||
-            e.conv1 > (1u << e.lg) || e.conv2 > e.conv1 || e.nw1 > 2 * FF_EARLY_LIMBS || e.nw2 > e.nw1)
+            e.conv1 > (1u << e.lg) || e.nw1 > 2 * FF_EARLY_LIMBS || e.nw2 > e.nw1)
             return hipErrorInvalidValue;
     }
     ff_prof_begin(FK_EARLY, st);

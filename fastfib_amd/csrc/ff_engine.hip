@@ -900,10 +900,8 @@ static bool dfib(uint64_t n, bool predict, DNum *res) {
             FFEarlyStep &e = ea.st[ea.nsteps++];
             e.lg = lg, e.bw = bw;
             e.conv1 = (uint32_t)conv;
-            e.conv2 = b2 ? (uint32_t)(2 * ((b2 + bw - 1) / bw) - 1) : 0;
             e.nw1 = (uint32_t)((b1 + 31) / 32), e.nw2 = (uint32_t)((b2 + 31) / 32);
             e.len = (uint32_t)len;
-            e.cb0 = kodd ? -2 : 2;
             e.bit = bit;
             blo = P(2 * kk - 1 + bit, FB_F), bhi = P(2 * kk + bit, FB_F);
             pred.push_back(blo), pred.push_back(bhi);

@@ -173,10 +173,9 @@ hipError_t ff_launch_small_step(const FFSmallStep *a, hipStream_t st);
 #define FF_EARLY_F0 96       // limbs of the starting pair (F(k0) for k0 < 2^13: 89 limbs)
 struct FFEarlyStep {
     int lg, bw;              // transform 2^lg words, coefficient width
-    uint32_t conv1, conv2;   // coefficients of F(k)^2 and F(k-1)^2
+    uint32_t conv1;          // coefficients of the products (those of F(k)^2)
     uint32_t nw1, nw2;       // 32-bit words of F(k) and F(k-1)
     uint32_t len;            // limbs of the step's results (n(F(k)^2) + 1)
-    int cb0;                 // 2 (-1)^k
     int bit;                 // next bit of n
 };
 struct FFEarlyArgs {

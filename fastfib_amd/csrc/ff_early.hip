@@ -75,17 +75,20 @@ __device__ __forceinline__ void early_pair_mid(const FFTile &t, const uint32_t *
 }
 
 // Forward transforms of both sub-transforms in radix-4 rounds, the pair
-// middle round, inverse rounds; the tables (rt, rtp, irt, irtp) in LDS.
-__device__ void early_pair_conv(const FFTile &t, const uint32_t *const *tw, uint32_t p, uint32_t pinv, int bit) {
+// middle round, inverse rounds; the tables (rt, rtp, irt, irtp) in LDS. The
+// first round takes its values from ld(pos, c) (the coefficients), the last
+// gives them to st(pos, c, v) (the scaled residues); t.rho >= 4, so those
+// are a forward and an inverse round apart from the middle one.
+template <class Ld, class St>
+__device__ void early_pair_conv(const FFTile &t, const uint32_t *const *tw, uint32_t p, uint32_t pinv, int bit, Ld ld,
+                                St st) {
     constexpr int rl = 2;
-    auto ld = [&](int pos, int c) -> uint32_t { return t.s[ff_pos_addr(t, pos, c)]; };
-    auto sto = [&](int pos, int c, uint32_t v) { t.s[ff_pos_addr(t, pos, c)] = v; };
     const int R = (t.rho + rl - 1) / rl;
     int b = t.rho;
     for (int r = 0; r < R - 1; r++) {
         b -= rl;
         if (r) __syncthreads();
-        ff_round_any<FF_R_DIF>(rl, false, false, t, b, tw[0], tw[1], nullptr, nullptr, nullptr, p, pinv, ld, sto,
+        ff_round_any<FF_R_DIF>(rl, r == 0, false, t, b, tw[0], tw[1], nullptr, nullptr, nullptr, p, pinv, ld, st,
                                FFLdsTables{});
     }
     const int nl0 = t.rho % rl == 0 ? rl : t.rho % rl;
@@ -95,7 +98,7 @@ __device__ void early_pair_conv(const FFTile &t, const uint32_t *const *tw, uint
     b = nl0;
     for (int r = 1; r < R; r++) {
         __syncthreads();
-        ff_round_any<FF_R_DIT>(rl, false, false, t, b, nullptr, nullptr, tw[2], tw[3], nullptr, p, pinv, ld, sto,
+        ff_round_any<FF_R_DIT>(rl, false, r == R - 1, t, b, nullptr, nullptr, tw[2], tw[3], nullptr, p, pinv, ld, st,
                                FFLdsTables{});
         b += rl;
     }
@@ -136,20 +139,22 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
         const int lg = st.lg;
         const int L = 1 << lg;
         const FFTile tl{E.tile[pi], 1, 1, 0, lg, 2, 1, tid0, nthr};
-        // sub-transform 0: F(k), 1: F(k - 1), coefficients of bw bits
+        // sub-transform 0: F(k), 1: F(k - 1), coefficients of bw bits, read
+        // from the pair by the first forward round; the last inverse round
+        // writes the residues, scaled, to y
         const uint32_t *w1 = (const uint32_t *)E.num[1], *w2 = (const uint32_t *)E.num[0];
-        for (int i = t - tid0; i < 2 * L; i += nthr) {
-            const int pos = i & (L - 1);
-            tl.s[ff_pos_addr(tl, i, 0)] =
-                (i >> lg) ? ff_load_coef(w2, st.nw2, pos, st.bw, P) : ff_load_coef(w1, st.nw1, pos, st.bw, P);
-        }
-        __syncthreads();
-        const uint32_t *tw[4] = {E.tw[pi][0], E.tw[pi][1], E.tw[pi][2], E.tw[pi][3]};
-        early_pair_conv(tl, tw, p, pinv, st.bit);
-        __syncthreads();
+        auto ld = [&](int pos, int c) -> uint32_t {
+            (void)c;
+            const int j = pos & (L - 1);
+            return (pos >> lg) ? ff_load_coef(w2, st.nw2, j, st.bw, P) : ff_load_coef(w1, st.nw1, j, st.bw, P);
+        };
         const uint32_t sc = a.sc[lg][pi];
-        for (int i = t - tid0; i < 2 * L; i += nthr)
-            E.y[i >> lg][pi * L + (i & (L - 1))] = ff_mont(tl.s[ff_pos_addr(tl, i, 0)], sc, p, pinv);
+        auto sto = [&](int pos, int c, uint32_t v) {
+            (void)c;
+            E.y[pos >> lg][pi * L + (pos & (L - 1))] = ff_mont(v, sc, p, pinv);
+        };
+        const uint32_t *tw[4] = {E.tw[pi][0], E.tw[pi][1], E.tw[pi][2], E.tw[pi][3]};
+        early_pair_conv(tl, tw, p, pinv, st.bit, ld, sto);
         __syncthreads();
         // both products of F(k) by itself and by F(k - 1): conv1 coefficients
         uint64_t s1[SS_CH], s2[SS_CH];

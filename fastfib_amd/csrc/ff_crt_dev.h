@@ -99,7 +99,8 @@ struct SSShared {
     uint32_t xhi[MAXC];
     uint64_t ex[SS_T];
     uint32_t wg[2][SS_T / 64], wp[2][SS_T / 64];
-    uint64_t hi_last;
+    uint32_t wg2[2][SS_T / 64], wp2[2][SS_T / 64];  // a second chain (ss_crt_pair)
+    uint64_t hi_last, hi_last2;
 };
 
 // Per-thread state of a sequence of lookaheads (the same in every thread):
@@ -172,6 +173,96 @@ __device__ void ss_crt(SSShared<SS_T> &S, SSRun &R, const uint32_t *y, uint32_t 
         if (t == SS_T - 1) S.hi_last = S.ex[t];
         const uint64_t v = ss_lookahead(S, R, (uint64_t)u, g, p);
         out[c] = live ? v : 0;
+    }
+}
+
+// ss_lookahead for two independent chains at once (u1, g1, p1 with carry
+// R.cin; u2, g2, p2 with carry cin2): both chains' wave summaries behind one
+// barrier.
+template <int SS_T>
+__device__ __forceinline__ void ss_lookahead2(SSShared<SS_T> &S, SSRun &R, uint32_t &cin2, uint64_t &u1, bool g1,
+                                              bool p1, uint64_t &u2, bool g2, bool p2) {
+    constexpr int NW = SS_T / 64;
+    static_assert(NW < 64, "wave summaries fit one ballot");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t Ga = __ballot(g1), Ba = Ga | __ballot(p1);
+    const uint64_t Gb = __ballot(g2), Bb = Gb | __ballot(p2);
+    if (lane == 0) {
+        S.wg[R.par][w] = (Ga + Ba) < Ga;
+        S.wp[R.par][w] = (Ba == ~0ull && Ga == 0);
+        S.wg2[R.par][w] = (Gb + Bb) < Gb;
+        S.wp2[R.par][w] = (Bb == ~0ull && Gb == 0);
+    }
+    __syncthreads();
+    {
+        const uint64_t WG = __ballot(lane < NW && S.wg[R.par][lane]);
+        const uint64_t WB = WG | __ballot(lane < NW && S.wp[R.par][lane]);
+        const uint64_t WC = (WG + WB + (uint64_t)R.cin) ^ WG ^ WB;
+        R.cin = (uint32_t)((WC >> NW) & 1);
+        const uint64_t C = (Ga + Ba + ((WC >> w) & 1)) ^ Ga ^ Ba;
+        u1 += (C >> lane) & 1;
+    }
+    {
+        const uint64_t WG = __ballot(lane < NW && S.wg2[R.par][lane]);
+        const uint64_t WB = WG | __ballot(lane < NW && S.wp2[R.par][lane]);
+        const uint64_t WC = (WG + WB + (uint64_t)cin2) ^ WG ^ WB;
+        cin2 = (uint32_t)((WC >> NW) & 1);
+        const uint64_t C = (Gb + Bb + ((WC >> w) & 1)) ^ Gb ^ Bb;
+        u2 += (C >> lane) & 1;
+    }
+    R.par ^= 1;
+}
+
+// ss_crt for two products of conv coefficients each at once (residues y1,
+// y2, [3][L]), limbs [0, len) into o1 and o2: one pass of barriers for both.
+// The second product's coefficients and high words go to xlo2 / xhi2 (as
+// many entries as S.xlo) and ex2 (SS_T entries), the caller's memory.
+template <int SS_T>
+__device__ void ss_crt_pair(SSShared<SS_T> &S, SSRun &R, uint64_t *xlo2, uint32_t *xhi2, uint64_t *ex2,
+                            const uint32_t *y1, const uint32_t *y2, uint32_t L, int bw, uint32_t conv, uint32_t len,
+                            const FFGarner &G, uint64_t o1[SS_CH], uint64_t o2[SS_CH]) {
+    const int t = threadIdx.x;
+    R.cin = 0;
+    uint32_t cin2 = 0;
+    for (int c = 0; c < SS_CH; c++) {
+        o1[c] = o2[c] = 0;
+        const uint64_t i0 = (uint64_t)c * SS_T;
+        if (i0 >= len || conv == 0) continue;  // uniform across the block
+        const uint64_t i1 = i0 + SS_T < len ? i0 + SS_T : len;
+        const uint64_t jbase = first_coef(i0 ? i0 - 1 : 0, bw);
+        uint64_t jend = (64 * i1 + bw - 1) / bw;
+        if (jend > conv) jend = conv;
+        for (uint64_t j = jbase + t; j < jend; j += SS_T) {
+            const u128 X1 = ff_garner(y1[j], y1[L + j], y1[2 * (size_t)L + j], G);
+            const u128 X2 = ff_garner(y2[j], y2[L + j], y2[2 * (size_t)L + j], G);
+            S.xlo[j - jbase] = (uint64_t)X1;
+            S.xhi[j - jbase] = (uint32_t)(X1 >> 64);
+            xlo2[j - jbase] = (uint64_t)X2;
+            xhi2[j - jbase] = (uint32_t)(X2 >> 64);
+        }
+        __syncthreads();
+        const uint64_t i = i0 + t;
+        const bool live = i < i1;
+        const u128 T1 = live ? limb_sum(i, bw, jbase, S.xlo, S.xhi, jbase, jend) : 0;
+        const u128 T2 = live ? limb_sum(i, bw, jbase, xlo2, xhi2, jbase, jend) : 0;
+        S.ex[t] = (uint64_t)(T1 >> 64);
+        ex2[t] = (uint64_t)(T2 >> 64);
+        __syncthreads();
+        // hi_last, hi_last2: the previous chunk's last high words (as in ss_crt)
+        const uint64_t h1 = t ? S.ex[t - 1] : (c ? S.hi_last : 0);
+        const uint64_t h2 = t ? ex2[t - 1] : (c ? S.hi_last2 : 0);
+        const u128 v1 = (u128)(uint64_t)T1 + h1, v2 = (u128)(uint64_t)T2 + h2;
+        const bool g1 = live && (v1 >> 64) != 0, g2 = live && (v2 >> 64) != 0;
+        const bool p1 = !live || (!g1 && (uint64_t)v1 == ~0ull), p2 = !live || (!g2 && (uint64_t)v2 == ~0ull);
+        __syncthreads();
+        if (t == SS_T - 1) {
+            S.hi_last = S.ex[t];
+            S.hi_last2 = ex2[t];
+        }
+        uint64_t u1 = (uint64_t)v1, u2 = (uint64_t)v2;
+        ss_lookahead2(S, R, cin2, u1, g1, p1, u2, g2, p2);
+        o1[c] = live ? u1 : 0;
+        o2[c] = live ? u2 : 0;
     }
 }
 

@@ -104,13 +104,30 @@ __device__ void early_pair_conv(const FFTile &t, const uint32_t *const *tw, uint
     }
 }
 
+// Coefficients of a product of len <= FF_EARLY_LIMBS limbs at bw >= 32, as
+// SSShared::MAXC for that many limbs.
+#define EARLY_MAXC (((FF_EARLY_LIMBS + 1) * 64 + 93) / 32 + 64)
+
+// The second CRT chain's arrays (ss_crt_pair) share the tiles' memory: the
+// tiles are dead from the last inverse round (which writes y) to the next
+// step's first forward round.
+struct EarlyCrt2 {
+    uint64_t xlo[EARLY_MAXC];
+    uint32_t xhi[EARLY_MAXC];
+    uint64_t ex[EARLY_T];
+};
+
 struct EarlyShared {
     uint32_t tw[3][4][EARLY_L];       // per prime: rt, rtp, irt, irtp
     uint64_t num[2][FF_EARLY_LIMBS];  // F(k - 1), F(k)
-    uint32_t tile[3][EARLY_TILE];     // per prime: F(k), F(k - 1) stacked
+    union {
+        uint32_t tile[3][EARLY_TILE];  // per prime: F(k), F(k - 1) stacked
+        EarlyCrt2 c2;
+    };
     uint32_t y[2][3 * EARLY_L];       // residues of the new pair: [prime][coefficient]
     SSShared<EARLY_T> ss;
 };
+static_assert(sizeof(EarlyCrt2) <= sizeof(uint32_t) * 3 * EARLY_TILE, "second chain fits the tiles");
 
 __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTables *T, FFPrimes PR, FFGarner G) {
     __shared__ EarlyShared E;
@@ -157,9 +174,9 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
         early_pair_conv(tl, tw, p, pinv, st.bit, ld, sto);
         __syncthreads();
         // both products of F(k) by itself and by F(k - 1): conv1 coefficients
+        // each, at most (64 len + bw - 1) / bw <= EARLY_MAXC of them
         uint64_t s1[SS_CH], s2[SS_CH];
-        ss_crt(E.ss, R, E.y[0], L, st.bw, st.conv1, st.len, G, s1);
-        ss_crt(E.ss, R, E.y[1], L, st.bw, st.conv1, st.len, G, s2);
+        ss_crt_pair(E.ss, R, E.c2.xlo, E.c2.xhi, E.c2.ex, E.y[0], E.y[1], L, st.bw, st.conv1, st.len, G, s1, s2);
         const uint64_t lo = s1[0], hi = s2[0];  // len <= FF_EARLY_LIMBS: one chunk
         if (t < (int)st.len) {
             E.num[0][t] = lo;

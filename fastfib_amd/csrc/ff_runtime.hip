@@ -27,7 +27,7 @@ int g_dispatch = 0;
 int g_max_lg = 25;
 int g_pair_steps = 2;
 int g_quad_max_lg = 18;
-int g_quad_inner_log = 0;
+int g_quad_inner_log = 10;
 int g_host_bits = 13;
 int g_dispatch_gen = 0;
 

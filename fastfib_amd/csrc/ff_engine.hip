@@ -461,11 +461,17 @@ static bool pair_enabled(int bit) { return pair_free() && (g_pair_steps >> bit &
 // nonzero limb is among the last three (lb_keys), so nout must be tight:
 // the callers derive it from an upper bound within a few bits of the
 // outputs' true lengths.
+// Largest quadrupling transform that takes the quad_inner_log blocks: up to
+// 2^17 words a quadrupling has fewer than 128 pair workgroups of 2^12 words
+// (3 L / 4096), each a serial chain of both operands' forward and both
+// outputs' inverse levels; from 2^18 on there are enough of them.
+#define FF_QUAD_SMALL_L (1u << 17)
 static bool pair_run(const DNum &a, const DNum &b, const ProdShape &s, int deg4, const uint8_t *cu,
                      const uint8_t *cv, uint64_t nout, DNum *u, DNum *v, const uint64_t *pred) {
     if (b.bits == 0 || b.bits > a.bits || s.split || s.L <= FF_TW) return false;
     FFPlan pl;
-    const int il = deg4 && g_quad_inner_log ? g_quad_inner_log : g_inner_log;
+    // quadruplings of few inner blocks: smaller blocks, more workgroups
+    const int il = deg4 && g_quad_inner_log && s.L <= FF_QUAD_SMALL_L ? g_quad_inner_log : g_inner_log;
     if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, il, g_tile_words) != 0 ||
         pl.npass == 0)
         return false;

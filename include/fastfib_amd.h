@@ -258,11 +258,13 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
                    (F(2k) = a^2 + 2ab, F(2k+1) = 2a^2 + 2ab + b^2 formed in
                    the product's middle round, no factor pass). Default 2:
                    the doubling steps measure faster as squarings
-     "quad_inner_log" inner block (log2, 8..12) of quadrupling transforms;
-                   0: "inner_log". Default 10: a quadrupling's inner
-                   workgroup transforms both operands and two outputs, and
-                   the steps that quadruple have few blocks, so smaller
-                   blocks spread that serial work over more workgroups
+     "quad_inner_log" inner block (log2, 8..12) of quadrupling transforms
+                   of at most 2^17 words (larger ones use "inner_log");
+                   0: "inner_log" for all. Default 10: a quadrupling's
+                   inner workgroup transforms both operands and two
+                   outputs, and those short steps have few blocks, so
+                   smaller blocks spread that serial work over more
+                   workgroups
      "host_bits"   F(n): the top bits of n (10..13, default 13) done on the
                    host (F(k0), k0 < 2^host_bits, at most 89 limbs) before
                    the device chain; each bit replaces one one-workgroup

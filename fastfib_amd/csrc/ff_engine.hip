@@ -129,13 +129,21 @@ static int slot_new(void) {
     return g_slot_used++;
 }
 
+// The length keys s[0, n) and the look-back time-out word to the pinned
+// host buffer h[0, n] in one launch (two copies are two blit launches).
+__global__ void k_slots_out(uint64_t *h, const unsigned long long *s, int n, const uint32_t *fault) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) h[i] = s[i];
+    if (threadIdx.x == 0) h[n] = *fault;
+    __threadfence_system();
+}
+
 // Reads n length keys back (waiting for the stream), and with them the
 // look-back time-out word, which must still be clear.
 static void slots_read(int s0, int n, uint64_t *out) {
     uint64_t *h = ff_pinned_results((size_t)n + 1);
-    h[n] = 0;
-    FF_CHECK(hipMemcpyAsync(h, g_slots + s0, (size_t)n * 8, hipMemcpyDeviceToHost, g_st));
-    FF_CHECK(hipMemcpyAsync(h + n, ff_lb_fault(), 4, hipMemcpyDeviceToHost, g_st));
+    hipLaunchKernelGGL(k_slots_out, dim3(1), dim3(256), 0, g_st, ff_pinned_results_dev(), g_slots + s0, n,
+                       ff_lb_fault());
+    FF_CHECK(hipGetLastError());
     FF_CHECK(hipStreamSynchronize(g_st));
     if ((uint32_t)h[n]) {
         fprintf(stderr, "fastfib_amd: a carry look-back wait timed out\n");

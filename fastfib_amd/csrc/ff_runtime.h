@@ -36,6 +36,8 @@ void ff_prof_pause(int pause);
 
 // Pinned host buffer for reading small results back.
 uint64_t *ff_pinned_results(size_t words);
+// The same buffer's address for kernels (valid after ff_pinned_results).
+uint64_t *ff_pinned_results_dev(void);
 
 // Workspace of the single-pass carry kernels: zeroed regions (left zeroed
 // by the kernels), handed out from the start again by ff_lb_begin; the word

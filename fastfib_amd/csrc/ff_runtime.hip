@@ -159,7 +159,7 @@ uint32_t *ff_lb_fault(void) {
     return g_lb_fault;
 }
 
-static uint64_t *g_res_h = nullptr;
+static uint64_t *g_res_h = nullptr, *g_res_d = nullptr;
 static size_t g_res_words = 0;
 
 uint64_t *ff_pinned_results(size_t words) {
@@ -167,9 +167,12 @@ uint64_t *ff_pinned_results(size_t words) {
         if (g_res_h) (void)hipHostFree(g_res_h);
         g_res_words = words < 4096 ? 4096 : words;
         FF_CHECK(hipHostMalloc((void **)&g_res_h, g_res_words * sizeof(uint64_t), hipHostMallocDefault));
+        FF_CHECK(hipHostGetDevicePointer((void **)&g_res_d, g_res_h, 0));
     }
     return g_res_h;
 }
+
+uint64_t *ff_pinned_results_dev(void) { return g_res_d; }
 
 // ------------------------------------------------------------------ live kernel timer
 // When enabled, every launch is bracketed by two HIP events on its stream;

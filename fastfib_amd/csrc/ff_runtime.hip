@@ -174,6 +174,16 @@ uint64_t *ff_pinned_results(size_t words) {
 
 uint64_t *ff_pinned_results_dev(void) { return g_res_d; }
 
+extern "C" int ff_test_set_lb_fault(void) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    ff_ensure_init();
+    const uint32_t one = 1;
+    if (hipMemcpyAsync(ff_lb_fault(), &one, 4, hipMemcpyHostToDevice, g_st) != hipSuccess ||
+        hipStreamSynchronize(g_st) != hipSuccess)
+        return FF_ERR_DEVICE;
+    return FF_OK;
+}
+
 // ------------------------------------------------------------------ live kernel timer
 // When enabled, every launch is bracketed by two HIP events on its stream;
 // ff_prof_read syncs and books the elapsed time and the launch's

@@ -289,6 +289,12 @@ FF_API int ff_prof_read(int k, const char **name, long *launches, double *ms, do
    counterpart in the reference. */
 FF_API int ff_prof_read_mults(int k, double *mults);
 
+/* Testing hook (no counterpart in the reference): marks the carry look-back
+   time-out word as set, as a bounded wait that ran out would, without any
+   kernel waiting. The next length read-back then fails loudly: a message on
+   stderr and exit(1), as after a real time-out. For failure-path tests only. */
+FF_API int ff_test_set_lb_fault(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@ low 64 bits, F(n) mod four primes) at full size. The linear step and the
 last-step factors are also checked on their own with carry-adversarial
 operands (all-ones and all-zero limb runs), which force the exact carry scan.
 """
+import os
 import random
 
 import numpy as np
@@ -170,3 +171,20 @@ def test_last_factors_adversarial(ff, nl):
         assert from_limbs(y) == 2 * u - v
         x, none = ff.last_factors(v, u, with_y=False)
         assert from_limbs(x) == 2 * v + u and none is None
+
+
+def test_lookback_timeout_fails_loudly():
+    """A carry look-back wait that ran out (forced from the host: the testing
+    hook sets the time-out word, no kernel waits) makes the next length
+    read-back print a message and exit(1), in a process of its own."""
+    import subprocess
+    import sys
+
+    code = ("import fastfib_amd as ff; ff.init(); "
+            "assert ff.lib().ff_test_set_lb_fault() == 0; "
+            "ff.dfibonacci(10 ** 6); print('returned')")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert r.returncode == 1, (r.returncode, r.stdout[-400:], r.stderr[-400:])
+    assert "look-back wait timed out" in r.stderr
+    assert "returned" not in r.stdout

@@ -173,8 +173,8 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
         const uint32_t *tw[4] = {E.tw[pi][0], E.tw[pi][1], E.tw[pi][2], E.tw[pi][3]};
         early_pair_conv(tl, tw, p, pinv, st.bit, ld, sto);
         __syncthreads();
-        // both products of F(k) by itself and by F(k - 1): conv1 coefficients
-        // each, at most (64 len + bw - 1) / bw <= EARLY_MAXC of them
+        // both new numbers, sums of products of F(k) and F(k - 1): conv1
+        // coefficients each, at most (64 len + bw - 1) / bw <= EARLY_MAXC
         uint64_t s1[SS_CH], s2[SS_CH];
         ss_crt_pair(E.ss, R, E.c2.xlo, E.c2.xhi, E.c2.ex, E.y[0], E.y[1], L, st.bw, st.conv1, st.len, G, s1, s2);
         const uint64_t lo = s1[0], hi = s2[0];  // len <= FF_EARLY_LIMBS: one chunk

@@ -217,8 +217,13 @@ __device__ __forceinline__ void crt_body(const FFCrtJob &J, uint32_t bx, const F
 #define CRT_ONE_WAVE_LIMBS (1280u * CRT_LIMBS)
 __host__ __device__ static inline int crt_lpt(int bw, uint64_t nout) { return bw >= 32 && nout > CRT_ONE_WAVE_LIMBS ? 4 : 1; }
 // A work list (a batch) fills the GPU with many jobs: four limbs per thread
-// wherever the width allows, for throughput.
-__host__ __device__ static inline int crt_lpt_list(int bw) { return bw >= 32 ? 4 : 1; }
+// wherever the width allows, for throughput, except in jobs of at most one
+// block of limbs (the low levels of the decimal tree, small batch items):
+// one workgroup holds such a job either way, and one limb per thread keeps
+// its serial part short.
+__host__ __device__ static inline int crt_lpt_list(int bw, uint64_t nout) {
+    return bw >= 32 && nout > CRT_LIMBS ? 4 : 1;
+}
 
 template <int LPT>
 __global__ __launch_bounds__(CRT_LIMBS) void k_crt(FFCrtSet js, FFGarner G, uint32_t *fault, int resident) {
@@ -242,13 +247,12 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_crt_list(const FFCrtJob *jobs, co
                                                         FFGarner G, uint32_t *fault) {
     __shared__ CrtShared sh;
     const int k = crt_find(start, n, blockIdx.x);
-    if (crt_lpt_list(jobs[k].bw) == 4) crt_body<4>(jobs[k], blockIdx.x - start[k], G, fault, sh, false);
+    if (crt_lpt_list(jobs[k].bw, jobs[k].nout) == 4) crt_body<4>(jobs[k], blockIdx.x - start[k], G, fault, sh, false);
     else crt_body<1>(jobs[k], blockIdx.x - start[k], G, fault, sh, false);
 }
 
 uint64_t ff_crt_blocks(uint64_t nout, int bw) {
-    (void)nout;
-    const uint64_t per = (uint64_t)CRT_LIMBS * crt_lpt_list(bw);
+    const uint64_t per = (uint64_t)CRT_LIMBS * crt_lpt_list(bw, nout);
     return (nout + per - 1) / per;
 }
 

@@ -307,6 +307,9 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
         return;
     }
     const bool bgiven = C.bspec == 1;  // B's spectrum read, not computed
+    // rounds of two levels up to 2^10 points: a radix-16 round of a short
+    // transform keeps only n / 16 of the threads busy (16 at n = 256)
+    const int small_rl = ta.rho <= 10 ? 2 : 4;
     if (C.bspec == 2) {
         // a's spectrum alone, stored by tile position for the products sharing it
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds_a_st((int)i, 0, load_a((int)i, 0));
@@ -330,7 +333,7 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
             __syncthreads();
         }
         ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, false, true,
-                   lds_a, store_a);
+                   lds_a, store_a, FFNoTw{}, small_rl);
         return;
     }
     // 3 * 2^k in one block: the radix-3 layers work on the LDS tiles
@@ -346,7 +349,7 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
         __syncthreads();
     }
     ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, false, false,
-               lds_a, lds_a_st);
+               lds_a, lds_a_st, FFNoTw{}, small_rl);
     ff_radix3_layer<true>(ta, T, pi, P, in.rstep3);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) store_a((int)i, 0, lds_a((int)i, 0));
 }

@@ -662,7 +662,17 @@ extern "C" hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerIte
                                            int n, uint32_t blocks, size_t lds, double bytes, hipStream_t st) {
     if (!blocks) return hipSuccess;
     const int kind = small ? FK_SMALL : FK_INNER;
-    const dim3 g(blocks), b(FF_THREADS);
+    // one-block lists of short transforms: n / 4 threads (what radix-4
+    // rounds keep busy), whole wavefronts, so that more items run per CU.
+    // The list's LDS size bounds its longest item: lds = 8 (n + n/16 + 16).
+    uint32_t threads = FF_THREADS;
+    if (small) {
+        const size_t words = lds / 8;
+        const size_t nmax = words > 16 ? (words - 16) * 16 / 17 + 1 : 1;
+        const size_t want = (nmax / 4 + 63) / 64 * 64;
+        threads = want < 64 ? 64 : (want < FF_THREADS ? (uint32_t)want : FF_THREADS);
+    }
+    const dim3 g(blocks), b(threads);
     ff_prof_begin(kind, st);
     if (!small) hipLaunchKernelGGL((k_inner_list<false, false>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);
     else if (r3) hipLaunchKernelGGL((k_inner_list<true, true>), g, b, lds, st, items, start, n, g_tables, g_primes_arg);

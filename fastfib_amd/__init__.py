@@ -273,6 +273,20 @@ def dmul(a: Device, b: Device | None = None) -> Device:
     return out
 
 
+def dmul_into(out: Device, a: Device, b: Device | None = None) -> None:
+    """out = a * b in place (out may be a itself: ff_dmul(&x, &x, NULL))."""
+    _check(lib().ff_dmul(ctypes.byref(out.h), ctypes.byref(a.h), None if b is None else ctypes.byref(b.h)))
+
+
+def mem_stats() -> dict:
+    """The library's allocation counters (ff_mem_stats)."""
+    L = lib()
+    L.ff_mem_stats.argtypes = [_u64p, _u64p, ctypes.POINTER(ctypes.c_size_t)]
+    dev, host, cached = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_size_t()
+    _check(L.ff_mem_stats(ctypes.byref(dev), ctypes.byref(host), ctypes.byref(cached)))
+    return {"device_allocs": dev.value, "host_allocs": host.value, "cached_bytes": cached.value}
+
+
 def dfibonacci(n: int) -> Device:
     out = Device()
     _check(lib().ff_dfibonacci(n, ctypes.byref(out.h)))

@@ -4,7 +4,8 @@
 
 hipcc compiles each HIP source for gfx950 only, with packed FP32 ops
 disabled, then links one shared library whose exported symbols are the
-C-ABI declared in include/fastfib_amd.h.
+C-ABI declared in include/fastfib_amd.h; gcc then builds the fastfib
+command line (cli/fastfib.c) against that C-ABI.
 """
 from __future__ import annotations
 
@@ -51,9 +52,29 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_cli()
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {CLI_OUT}")
     return OUT
+
+
+CLI_SRC = os.path.join(HERE, "cli", "fastfib.c")
+CLI_OUT = os.path.join(HERE, "fastfib")
+
+
+def build_cli() -> str:
+    """The fastfib command line (cli/fastfib.c), a C program linked to the
+    library beside it (found at run time through its rpath)."""
+    cc = os.environ.get("CC", "gcc")
+    include = os.path.join(os.path.dirname(HERE), "include")
+    tmp = CLI_OUT + ".tmp"
+    cmd = [cc, "-O2", "-std=c11", "-D_POSIX_C_SOURCE=200809L", "-Wall", "-Wextra", "-Werror", f"-I{include}",
+           "-o", tmp, CLI_SRC, f"-L{HERE}", "-lfastfib_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed:\n{r.stderr}")
+    os.replace(tmp, CLI_OUT)
+    return CLI_OUT
 
 
 if __name__ == "__main__":

@@ -1,6 +1,6 @@
 // Modular arithmetic for the three NTT primes on gfx950 (32-bit lanes).
 //
-// The primes and generators are those of ntt_arith.c:96-100 (reference):
+// The primes and generators are those of ntt_arith.c:44-52 (reference):
 // p = c * 2^k + 1 < 2^31, with 3 * 2^25 | p - 1, so every transform length
 // 2^k and 3 * 2^k up to 2^25 has its roots of unity. Values are canonical in
 // [0, p). Multiplication is Montgomery with R = 2^32 computed from the high

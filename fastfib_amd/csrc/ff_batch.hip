@@ -376,7 +376,7 @@ static void build_lists(ff_batch *B) {
         }
     }
     B->desc = (char *)ff_dalloc(img.size() + 256);
-    FF_CHECK(hipMemcpy(B->desc, img.data(), img.size(), hipMemcpyHostToDevice));
+    ff_copy(B->desc, img.data(), img.size(), hipMemcpyHostToDevice);
     for (auto &pd : pend) {
         BLaunch L = pd.L;
         L.items = B->desc + pd.items_off;
@@ -516,8 +516,11 @@ extern "C" int ff_batch_finish(ff_batch *B, ff_prod *items) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     const size_t n = B->items.size();
     std::vector<uint64_t> keys(n ? n : 1);
+    uint32_t fault = 0;
     if (n) FF_CHECK(hipMemcpyAsync(keys.data(), B->slots, n * 8, hipMemcpyDeviceToHost, g_st));
+    FF_CHECK(hipMemcpyAsync(&fault, ff_lb_fault(), 4, hipMemcpyDeviceToHost, g_st));
     FF_CHECK(hipStreamSynchronize(g_st));
+    if (fault) ff_lb_timeout_fail();
     if (items) {
         for (size_t i = 0; i < n; i++) items[i].nout = B->zero[i] || keys[i] == 0 ? 1 : (size_t)(keys[i] >> 8);
     }

@@ -2,8 +2,8 @@
 //
 // The three residue arrays y_p[i] (exact convolution coefficient i mod p,
 // natural order, as the inverse top pass leaves them) are combined by
-// Garner's method into X_j < p1 p2 p3 < 2^93 (crt.c:180-206, reference),
-// and the product is N = sum_j X_j 2^(j bw) (crt.c:208-296). The reference
+// Garner's method into X_j < p1 p2 p3 < 2^93 (crt.c:33-64, reference),
+// and the product is N = sum_j X_j 2^(j bw) (crt.c:66-154). The reference
 // packs coefficients into limbs with one running carry per chunk; here every
 // output limb has its own thread:
 //   N = sum_i T_i 2^(64 i),  T_i = sum_j digit_i(X_j 2^(j bw)),
@@ -16,7 +16,7 @@
 // from their summaries, across workgroups by decoupled look-back
 // (ff_lookback.h), and the job's last workgroup books the top limb. This
 // replaces the reference's 8 sequential chunks plus carry fix
-// (crt.c:326-336).
+// (crt.c:184-194).
 //
 // The same machinery fuses a whole doubling step after its two squarings
 // (k_step_fused): the CRT of both squares and the linear combination of

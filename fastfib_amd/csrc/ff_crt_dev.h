@@ -17,7 +17,7 @@ struct FFGarner {
 };
 FFGarner ff_garner_consts(void);  // ff_crt.hip
 
-// X = x1 + x2 p1 + x3 p1 p2 from canonical residues (crt.c:180-206).
+// X = x1 + x2 p1 + x3 p1 p2 from canonical residues (crt.c:33-64).
 __device__ __forceinline__ u128 ff_garner(uint32_t r1, uint32_t r2, uint32_t r3, const FFGarner &G) {
     const uint32_t x1 = r1;
     const uint32_t x1m2 = x1 >= G.p2 ? x1 - G.p2 : x1;  // p1 < 2 p2

@@ -494,10 +494,15 @@ void decimal_dev(const uint64_t *a, uint64_t na, uint64_t bits, const char **dev
         first_nz = pl->first_nz;
         err = pl->err;
     }
-    uint64_t *h = ff_pinned_results(2);
+    uint64_t *h = ff_pinned_results(3);
+    h[2] = 0;
     FF_CHECK(hipMemcpyAsync(h, first_nz, 8, hipMemcpyDeviceToHost, g_st));
     FF_CHECK(hipMemcpyAsync(h + 1, err, 4, hipMemcpyDeviceToHost, g_st));
+    // the levels' CRTs resolve carries by look-back: a wait that timed out
+    // leaves wrong limbs behind, so the conversion must not report success
+    FF_CHECK(hipMemcpyAsync(h + 2, ff_lb_fault(), 4, hipMemcpyDeviceToHost, g_st));
     FF_CHECK(hipStreamSynchronize(g_st));
+    if ((uint32_t)h[2]) ff_lb_timeout_fail();
     const uint64_t first = h[0];
     if ((uint32_t)h[1] != 0 || first >= nd) {
         fprintf(stderr, "fastfib_amd: decimal conversion failed its leaf check\n");
@@ -524,7 +529,7 @@ char *to_string_dev(const uint64_t *a, uint64_t na, uint64_t bits, size_t *len) 
         fprintf(stderr, "out of memory (%zu bytes)\n", n + 1);
         exit(1);
     }
-    FF_CHECK(hipMemcpy(s, dev, n, hipMemcpyDeviceToHost));
+    ff_copy(s, dev, n, hipMemcpyDeviceToHost);
     s[n] = 0;
     *len = n;
     return s;
@@ -548,7 +553,7 @@ extern "C" char *ff_big_to_string(const Big *a) {
     const uint64_t bits = top ? 64 * (n - 1) + (64 - __builtin_clzll(top)) : 0;
     uint64_t *d = (uint64_t *)ff_dalloc(n * 8);
     FF_CHECK(hipStreamSynchronize(g_st));
-    FF_CHECK(hipMemcpy(d, a->d, n * 8, hipMemcpyHostToDevice));
+    ff_copy(d, a->d, n * 8, hipMemcpyHostToDevice);
     size_t len;
     char *s = to_string_dev(d, n, bits, &len);
     ff_dfree(d);
@@ -576,7 +581,7 @@ extern "C" int ff_dbig_to_decimal(const ff_dbig *a, const char **dev_digits, siz
             g_small_first = (unsigned long long *)ff_dalloc(8);
             g_small_err = (uint32_t *)ff_dalloc(8);
         }
-        FF_CHECK(hipMemcpy(g_small_dig, &zero, 1, hipMemcpyHostToDevice));
+        ff_copy(g_small_dig, &zero, 1, hipMemcpyHostToDevice);
         *dev_digits = g_small_dig;
         *len = 1;
         return FF_OK;
@@ -592,7 +597,7 @@ extern "C" Big ff_big_recip(const Big *P, size_t s) {
     while (n > 1 && P->d[n - 1] == 0) n--;
     FFNum p = ff_num_alloc(n);
     FF_CHECK(hipStreamSynchronize(g_st));
-    FF_CHECK(hipMemcpy(p.d, P->d, n * 8, hipMemcpyHostToDevice));
+    ff_copy(p.d, P->d, n * 8, hipMemcpyHostToDevice);
     p.n = n;
     const uint64_t top = P->d[n - 1];
     p.bits = top ? 64 * (n - 1) + (64 - __builtin_clzll(top)) : 0;
@@ -601,7 +606,7 @@ extern "C" Big ff_big_recip(const Big *P, size_t s) {
     out.n = r.n;
     out.d = (uint64_t *)malloc(r.n * 8);
     FF_CHECK(hipStreamSynchronize(g_st));
-    FF_CHECK(hipMemcpy(out.d, r.d, r.n * 8, hipMemcpyDeviceToHost));
+    ff_copy(out.d, r.d, r.n * 8, hipMemcpyDeviceToHost);
     ff_num_free(r);
     ff_num_free(p);
     return out;

@@ -46,6 +46,8 @@ struct MulItem {
 
 // One product past the longest transform (split into transformed-once pieces).
 void ff_mul_split(const MulItem &it);
+// A split product's coefficient width and piece transform length (log2).
+void ff_split_plan(uint64_t bits_a, uint64_t bits_b, bool sq, unsigned *bw, int *lgp);
 
 // Runs up to FF_MAXB products sharing one shape and schedule, one launch
 // per pass (transform arrays come from the memory pool, stream-ordered).

@@ -145,10 +145,7 @@ static void slots_read(int s0, int n, uint64_t *out) {
                        ff_lb_fault());
     FF_CHECK(hipGetLastError());
     FF_CHECK(hipStreamSynchronize(g_st));
-    if ((uint32_t)h[n]) {
-        fprintf(stderr, "fastfib_amd: a carry look-back wait timed out\n");
-        exit(1);
-    }
+    if ((uint32_t)h[n]) ff_lb_timeout_fail();
     memcpy(out, h, (size_t)n * 8);
 }
 
@@ -1099,7 +1096,7 @@ static DNum upload(const uint64_t *h, size_t n) {
     DNum x = dnum_alloc(n);
     const uint64_t zero = 0;
     FF_CHECK(hipStreamSynchronize(g_st));
-    FF_CHECK(hipMemcpy(x.d, h ? h : &zero, n * 8, hipMemcpyHostToDevice));
+    ff_copy(x.d, h ? h : &zero, n * 8, hipMemcpyHostToDevice);
     x.n = n;
     const uint64_t top = h ? h[n - 1] : 0;
     x.bits = top ? 64 * (n - 1) + (64 - __builtin_clzll(top)) : 0;
@@ -1115,7 +1112,7 @@ static Big download(DNum &x) {
         exit(1);
     }
     FF_CHECK(hipStreamSynchronize(g_st));
-    FF_CHECK(hipMemcpy(r.d, x.d, r.n * 8, hipMemcpyDeviceToHost));
+    ff_copy(r.d, x.d, r.n * 8, hipMemcpyDeviceToHost);
     return r;
 }
 
@@ -1139,6 +1136,37 @@ static Big host_mul(const Big *a, const Big *b, bool square) {
     dnum_free(A);
     if (!square) dnum_free(B);
     return r;
+}
+
+extern "C" int ff_product_plan(uint64_t bits_a, uint64_t bits_b, int square, ff_plan_info *out) {
+    if (!out) return FF_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    memset(out, 0, sizeof *out);
+    if (square) bits_b = bits_a;
+    if (bits_a == 0 || bits_b == 0) {
+        out->zero = 1;  // written as one zero limb, no transform
+        return FF_OK;
+    }
+    const ProdShape s = product_shape(bits_a, bits_b);
+    out->width = s.bw;
+    out->length = s.L;
+    out->conv = s.conv;
+    out->radix3 = s.r3;
+    out->split = s.split;
+    if (s.split) {
+        unsigned bw;
+        int lgp;
+        ff_split_plan(bits_a, bits_b, square != 0, &bw, &lgp);
+        const uint64_t H = 1ull << (lgp - 1);
+        const uint64_t ca = (bits_a + bw - 1) / bw, cb = (bits_b + bw - 1) / bw;
+        out->width = bw;
+        out->length = 1ull << lgp;
+        out->conv = ca + cb - 1;
+        out->radix3 = 0;
+        out->pieces_a = (ca + H - 1) / H;
+        out->pieces_b = square ? out->pieces_a : (cb + H - 1) / H;
+    }
+    return FF_OK;
 }
 
 extern "C" Big ff_big_mul(const Big *a, const Big *b) { return host_mul(a, b, a == b); }
@@ -1226,7 +1254,7 @@ extern "C" int ff_dbig_upload(ff_dbig *x, const uint64_t *limbs, size_t n) {
         // in place: device pointers held elsewhere (a batch) stay valid
         const uint64_t zero = 0;
         FF_CHECK(hipStreamSynchronize(g_st));
-        FF_CHECK(hipMemcpy(x->d, limbs ? limbs : &zero, m * 8, hipMemcpyHostToDevice));
+        ff_copy(x->d, limbs ? limbs : &zero, m * 8, hipMemcpyHostToDevice);
         const uint64_t top = limbs ? limbs[m - 1] : 0;
         x->n = m;
         x->bits = top ? 64 * (m - 1) + (64 - __builtin_clzll(top)) : 0;
@@ -1254,10 +1282,16 @@ extern "C" int ff_dmul(ff_dbig *out, const ff_dbig *a, const ff_dbig *b) {
     DNum A = from_dbig(a), B = b ? from_dbig(b) : DNum{};
     const bool sq = (b == nullptr || b == a || b->d == a->d);
     const DNum *pa[1] = {&A}, *pb[1] = {sq ? nullptr : &B};
+    // out's old block goes back to the pool after the product is queued (the
+    // pool reuses blocks in stream order, so the product still reads it
+    // first) - also when out is the operand itself (out == a squares in
+    // place), but not when another handle (a or b) still names that block
+    uint64_t *old = out->d;
+    const bool keep_old = (out != a && old == a->d) || (b && out != b && old == b->d);
     DNum R;
     mul_many(pa, pb, &R, 1, nullptr);
-    if (out->d && out->d != a->d && (!b || out->d != b->d)) ff_dbig_release(out);
     to_dbig(R, out);
+    if (old && old != R.d && !keep_old) ff_dfree(old);
     return FF_OK;
 }
 

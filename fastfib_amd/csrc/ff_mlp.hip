@@ -269,15 +269,15 @@ extern "C" int ff_mlp_grad(const float *x, const float *t, size_t n, float *loss
     ff_ensure_init();
     MlpDev D(g_K, (int)n);
     const Layout &l = D.l;
-    FF_CHECK(hipMemcpy(D.P, g_params.data(), l.total * 4, hipMemcpyHostToDevice));
-    FF_CHECK(hipMemcpy(D.X, x, n * NI * 4, hipMemcpyHostToDevice));
-    FF_CHECK(hipMemcpy(D.T, t, n * g_K * 4, hipMemcpyHostToDevice));
+    ff_copy(D.P, g_params.data(), l.total * 4, hipMemcpyHostToDevice);
+    ff_copy(D.X, x, n * NI * 4, hipMemcpyHostToDevice);
+    ff_copy(D.T, t, n * g_K * 4, hipMemcpyHostToDevice);
     FF_CHECK(hipMemsetAsync(D.dP, 0, l.total * 4, g_st));
     D.step((int)n);
     std::vector<float> lo(n);
     FF_CHECK(hipStreamSynchronize(g_st));
-    if (grad) FF_CHECK(hipMemcpy(grad, D.dP, l.total * 4, hipMemcpyDeviceToHost));
-    FF_CHECK(hipMemcpy(lo.data(), D.loss, n * 4, hipMemcpyDeviceToHost));
+    if (grad) ff_copy(grad, D.dP, l.total * 4, hipMemcpyDeviceToHost);
+    ff_copy(lo.data(), D.loss, n * 4, hipMemcpyDeviceToHost);
     if (loss) {
         double s = 0;
         for (float v : lo) s += v;
@@ -297,9 +297,9 @@ extern "C" int ff_mlp_train(const float *x, const float *t, size_t n, int epochs
     const Layout &l = D.l;
     float *Xall = D.alloc(n * NI), *Tall = D.alloc(n * K);
     int *idx = (int *)D.alloc(B);
-    FF_CHECK(hipMemcpy(Xall, x, n * NI * 4, hipMemcpyHostToDevice));
-    FF_CHECK(hipMemcpy(Tall, t, n * K * 4, hipMemcpyHostToDevice));
-    FF_CHECK(hipMemcpy(D.P, g_params.data(), l.total * 4, hipMemcpyHostToDevice));
+    ff_copy(Xall, x, n * NI * 4, hipMemcpyHostToDevice);
+    ff_copy(Tall, t, n * K * 4, hipMemcpyHostToDevice);
+    ff_copy(D.P, g_params.data(), l.total * 4, hipMemcpyHostToDevice);
     FF_CHECK(hipMemsetAsync(D.m, 0, l.total * 4, g_st));
     FF_CHECK(hipMemsetAsync(D.v, 0, l.total * 4, g_st));
     FF_CHECK(hipMemsetAsync(D.dP, 0, l.total * 4, g_st));
@@ -337,7 +337,7 @@ extern "C" int ff_mlp_train(const float *x, const float *t, size_t n, int epochs
         if (e == epochs - 1 && seen) last_loss = sum / seen;
     }
     FF_CHECK(hipStreamSynchronize(g_st));
-    FF_CHECK(hipMemcpy(g_params.data(), D.P, l.total * 4, hipMemcpyDeviceToHost));
+    ff_copy(g_params.data(), D.P, l.total * 4, hipMemcpyDeviceToHost);
     g_dstale = true;
     g_dispatch_gen++;
     if (loss_out) *loss_out = (float)last_loss;
@@ -354,7 +354,7 @@ int ff_mlp_infer_dev(const float *dx, size_t n, const uint32_t *dmask, float *dy
     if (!g_dparams) g_dparams = (float *)ff_dalloc(layout(FF_MLP_MAXK).total * 4);
     if (g_dstale) {
         FF_CHECK(hipStreamSynchronize(g_st));
-        FF_CHECK(hipMemcpy(g_dparams, g_params.data(), l.total * 4, hipMemcpyHostToDevice));
+        ff_copy(g_dparams, g_params.data(), l.total * 4, hipMemcpyHostToDevice);
         g_dstale = false;
     }
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n)), dim3(256), l.total * 4, g_st, dx, (const float *)g_dparams,
@@ -379,8 +379,8 @@ extern "C" int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, fl
     int rc = ff_mlp_infer_dev(dx, n, mask ? dm : nullptr, dy, dc, 0.0f, -1);
     if (rc == FF_OK) {
         FF_CHECK(hipStreamSynchronize(g_st));
-        if (y) FF_CHECK(hipMemcpy(y, dy, n * K * 4, hipMemcpyDeviceToHost));
-        if (choice) FF_CHECK(hipMemcpy(choice, dc, n * 4, hipMemcpyDeviceToHost));
+        if (y) ff_copy(y, dy, n * K * 4, hipMemcpyDeviceToHost);
+        if (choice) ff_copy(choice, dc, n * 4, hipMemcpyDeviceToHost);
     }
     ff_dfree(buf);
     return rc;

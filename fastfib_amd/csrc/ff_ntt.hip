@@ -26,6 +26,7 @@
 
 #include "ff_internal.h"
 #include "ff_ntt_dev.h"
+#include "ff_runtime.h"
 
 static const uint32_t kP[3] = {FF_P1, FF_P2, FF_P3};
 static const uint32_t kG[3] = {5, 31, 13};
@@ -99,7 +100,7 @@ extern "C" int ff_tables_init(void) {
     }
     memcpy(g_primes_arg.p, g_prime, sizeof g_prime);
     FFTables *d = nullptr;
-    if (hipMalloc(&d, sizeof(FFTables)) != hipSuccess) {
+    if (ff_hip_malloc((void **)&d, sizeof(FFTables)) != hipSuccess) {
         free(h);
         return -1;
     }
@@ -543,7 +544,7 @@ static int ilog2(uint64_t x) { return 63 - __builtin_clzll(x); }
 static uint32_t h_scale(uint32_t L, int i) {
     // R^2 / L mod p: Montgomery form of R / L, undoes the pointwise R^-1 and
     // the inverse transform's factor L (the reference's scale[] and scale3[],
-    // ntt_tables.c:299-309).
+    // ntt_tables.c:103-114).
     const uint32_t p = kP[i];
     const uint64_t R = (1ull << 32) % p;
     const uint64_t R2 = R * R % p;

@@ -10,7 +10,7 @@
 // natural order in, bit-reversed order out); inverse rounds run DIT levels
 // b = 0 .. rho-1 with inverse roots (bit-reversed in, natural out), so that
 // inverse(forward(x)) = M * x. Twiddles of level b are rt[2^b + j], j = row
-// mod 2^b, the table layout of ntt_tables.c:204-209 (reference).
+// mod 2^b, the table layout of ntt_arith.c:37 and ntt_tables.c:38-65 (reference).
 #pragma once
 #include "ff_arith.h"
 

@@ -27,12 +27,21 @@ void ff_ensure_init(void);
 // order (all device work runs on g_st), hipMalloc only on a miss.
 void *ff_dalloc(size_t bytes);
 void ff_dfree(void *p);
+// The library's only allocation calls (counted; see ff_mem_stats).
+hipError_t ff_hip_malloc(void **p, size_t bytes);
+hipError_t ff_host_malloc(void **p, size_t bytes);
 // Bytes the pool holds for reuse (free for the library, not for the driver).
 size_t ff_pool_cached(void);
 
 // Kernel timing is suspended while a graph is being captured (event
 // records and their readback do not belong in a replayed graph).
 void ff_prof_pause(int pause);
+
+// Host <-> device copy ordered after everything queued on the library
+// stream and complete on return. The stream is non-blocking, so a plain
+// hipMemcpy (null stream) is not ordered after its kernels: a recycled pool
+// block could be written while a queued kernel still reads it.
+void ff_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
 
 // Pinned host buffer for reading small results back.
 uint64_t *ff_pinned_results(size_t words);
@@ -45,5 +54,9 @@ uint64_t *ff_pinned_results_dev(void);
 void ff_lb_begin(void);
 uint32_t *ff_lb_take(size_t words);
 uint32_t *ff_lb_fault(void);
+// What a set time-out word means for the caller: results are wrong and the
+// workspace is no longer zeroed, so the process stops (a message on stderr
+// and exit(1), like every other device failure of the drop-in surface).
+[[noreturn]] void ff_lb_timeout_fail(void);
 // Workgroups of the kernel (at this block size) the GPU holds at once.
 uint64_t ff_resident_blocks(const void *kernel, int threads);

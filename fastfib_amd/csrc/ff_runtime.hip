@@ -48,6 +48,35 @@ extern "C" void ff_fail(const char *file, int line, const char *what, hipError_t
 
 // ------------------------------------------------------------------ device memory
 
+// Every device or pinned-host allocation of the library goes through
+// ff_hip_malloc / ff_host_malloc, which count them (a caller checks that
+// repeated work allocates nothing new) and can be told to fail (the
+// "test_fail_alloc" option: after that many more allocations the next ones
+// report out-of-memory, so the failure path runs without exhausting HBM).
+static uint64_t g_n_dev_mallocs = 0, g_n_host_mallocs = 0;
+static long long g_fail_after = -1;
+
+static bool alloc_injected_failure(void) {
+    if (g_fail_after < 0) return false;
+    if (g_fail_after == 0) return true;
+    g_fail_after--;
+    return false;
+}
+
+hipError_t ff_hip_malloc(void **p, size_t bytes) {
+    if (alloc_injected_failure()) return hipErrorOutOfMemory;
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) g_n_dev_mallocs++;
+    return e;
+}
+
+hipError_t ff_host_malloc(void **p, size_t bytes) {
+    if (alloc_injected_failure()) return hipErrorOutOfMemory;
+    const hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) g_n_host_mallocs++;
+    return e;
+}
+
 static std::multimap<size_t, void *> g_free;
 static std::unordered_map<void *, size_t> g_sizes;
 static size_t g_cached = 0;  // bytes held in g_free
@@ -72,7 +101,7 @@ void *ff_dalloc(size_t bytes) {
         return p;
     }
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, sz);
+    hipError_t e = ff_hip_malloc(&p, sz);
     if (e != hipSuccess) {
         // give the cached blocks back and retry once
         (void)hipStreamSynchronize(g_st);
@@ -83,7 +112,7 @@ void *ff_dalloc(size_t bytes) {
         g_free.clear();
         g_cached = 0;
         (void)hipGetLastError();
-        e = hipMalloc(&p, sz);
+        e = ff_hip_malloc(&p, sz);
         if (e != hipSuccess) {
             fprintf(stderr, "out of memory (%zu bytes)\n", bytes);
             exit(1);
@@ -99,6 +128,12 @@ void ff_dfree(void *p) {
     if (it == g_sizes.end()) return;
     g_free.emplace(it->second, p);
     g_cached += it->second;
+}
+
+void ff_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    if (!bytes) return;
+    FF_CHECK(hipMemcpyAsync(dst, src, bytes, kind, g_st));
+    FF_CHECK(hipStreamSynchronize(g_st));
 }
 
 // ------------------------------------------------------------------ look-back workspace
@@ -127,7 +162,7 @@ uint32_t *ff_lb_take(size_t words) {
         size_t cap = g_lb_cap ? 2 * g_lb_cap : (1u << 20);
         while (cap < words) cap *= 2;
         uint32_t *p = nullptr;
-        FF_CHECK(hipMalloc(&p, cap * 4));
+        FF_CHECK(ff_hip_malloc((void **)&p, cap * 4));
         FF_CHECK(hipMemsetAsync(p, 0, cap * 4, g_st));
         if (g_lb) g_lb_old.push_back(g_lb);
         g_lb = p, g_lb_cap = cap, g_lb_off = 0;
@@ -153,10 +188,15 @@ uint64_t ff_resident_blocks(const void *kernel, int threads) {
 
 uint32_t *ff_lb_fault(void) {
     if (!g_lb_fault) {
-        FF_CHECK(hipMalloc(&g_lb_fault, 256));
+        FF_CHECK(ff_hip_malloc((void **)&g_lb_fault, 256));
         FF_CHECK(hipMemsetAsync(g_lb_fault, 0, 256, g_st));
     }
     return g_lb_fault;
+}
+
+void ff_lb_timeout_fail(void) {
+    fprintf(stderr, "fastfib_amd: a carry look-back wait timed out\n");
+    exit(1);
 }
 
 static uint64_t *g_res_h = nullptr, *g_res_d = nullptr;
@@ -166,13 +206,21 @@ uint64_t *ff_pinned_results(size_t words) {
     if (words > g_res_words) {
         if (g_res_h) (void)hipHostFree(g_res_h);
         g_res_words = words < 4096 ? 4096 : words;
-        FF_CHECK(hipHostMalloc((void **)&g_res_h, g_res_words * sizeof(uint64_t), hipHostMallocDefault));
+        FF_CHECK(ff_host_malloc((void **)&g_res_h, g_res_words * sizeof(uint64_t)));
         FF_CHECK(hipHostGetDevicePointer((void **)&g_res_d, g_res_h, 0));
     }
     return g_res_h;
 }
 
 uint64_t *ff_pinned_results_dev(void) { return g_res_d; }
+
+extern "C" int ff_mem_stats(uint64_t *device_allocs, uint64_t *host_allocs, size_t *cached_bytes) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (device_allocs) *device_allocs = g_n_dev_mallocs;
+    if (host_allocs) *host_allocs = g_n_host_mallocs;
+    if (cached_bytes) *cached_bytes = g_cached;
+    return FF_OK;
+}
 
 extern "C" int ff_test_set_lb_fault(void) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -369,6 +417,9 @@ extern "C" int ff_set_option(const char *name, long long v) {
     } else if (!strcmp(name, "host_bits")) {
         if (v < 10 || v > 13) return FF_ERR_ARG;
         g_host_bits = (int)v;
+    } else if (!strcmp(name, "test_fail_alloc")) {
+        if (v < -1) return FF_ERR_ARG;
+        g_fail_after = v;
     } else if (!strcmp(name, "dispatch")) {
         if (v != 0 && v != FF_BATCH_TUNED && v != FF_BATCH_LEARNED) return FF_ERR_ARG;
         g_dispatch = (int)v;

@@ -108,8 +108,8 @@ void *upload_list(const std::vector<Item> &items, const std::vector<uint32_t> &s
     const size_t bi = (items.size() * sizeof(Item) + 255) / 256 * 256;
     char *d = (char *)ff_dalloc(bi + start.size() * 4 + 256);
     FF_CHECK(hipStreamSynchronize(g_st));
-    FF_CHECK(hipMemcpy(d, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice));
-    FF_CHECK(hipMemcpy(d + bi, start.data(), start.size() * 4, hipMemcpyHostToDevice));
+    ff_copy(d, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice);
+    ff_copy(d + bi, start.data(), start.size() * 4, hipMemcpyHostToDevice);
     *di = (const Item *)d;
     *ds = (const uint32_t *)(d + bi);
     return d;
@@ -135,13 +135,20 @@ static int split_lg(uint64_t bits_a, uint64_t bits_b, unsigned bw, bool sq) {
     return lgp;
 }
 
+void ff_split_plan(uint64_t bits_a, uint64_t bits_b, bool sq, unsigned *bw_out, int *lgp_out) {
+    // the widest width the CRT bound allows (mul_split.c:152-154)
+    unsigned bw = 35;
+    while (bw > 8 && !ff_width_ok(bits_a, bits_b, bw)) bw--;
+    *bw_out = bw;
+    *lgp_out = split_lg(bits_a, bits_b, bw, sq);
+}
+
 void ff_mul_split(const MulItem &it) {
     const bool sq = it.b == nullptr;
     const uint64_t bits_b = sq ? it.bits_a : it.bits_b;
-    // the widest width the CRT bound allows (mul_split.c:152-154)
-    unsigned bw = 35;
-    while (bw > 8 && !ff_width_ok(it.bits_a, bits_b, bw)) bw--;
-    const int lgp = split_lg(it.bits_a, bits_b, bw, sq);
+    unsigned bw;
+    int lgp;
+    ff_split_plan(it.bits_a, bits_b, sq, &bw, &lgp);
     const uint64_t Lp = 1ull << lgp, H = Lp / 2;
     const uint64_t ca = (it.bits_a + bw - 1) / bw, cb = (bits_b + bw - 1) / bw;
     const int ka = (int)((ca + H - 1) / H), kb = sq ? ka : (int)((cb + H - 1) / H);

@@ -31,17 +31,22 @@ extern "C" {
 
 /* value = sum d[i] * 2^(64 i); normalized: n >= 1, no leading zero limb,
    zero is {0} with n = 1. Layout-identical to the reference's Big
-   (src/common.h:29-33). */
+   (src/common.h:29-33). A caller that already has that Big in scope (the
+   reference's unity build, which typedefs it as an anonymous struct that C
+   cannot redeclare) defines FF_CALLER_BIG before including this header, and
+   every declaration below then takes the caller's own Big. */
+#ifndef FF_CALLER_BIG
 typedef struct {
     uint64_t *d;
     size_t n;
 } Big;
+#endif
 
 /* ------------------------------------------------------------------------
  * Drop-in surface. Host pointers in and out; failures abort the process.
  * --------------------------------------------------------------------- */
 
-/* Replaces ntt_init (src/ntt_tables.c:268): prepares the device and the
+/* Replaces ntt_init (src/ntt_tables.c:72): prepares the device and the
    root tables for products of up to max_limbs limbs. Idempotent and
    grow-only; every other entry point also initializes on first use. */
 FF_API void ff_ntt_init(size_t max_limbs);
@@ -71,6 +76,26 @@ FF_API char *ff_big_to_string(const Big *a);
    4, for P of exactly s bits, by the reference's Newton recursion with
    every product on the GPU (exported for its parity tests). */
 FF_API Big ff_big_recip(const Big *P, size_t s);
+
+/* The schedule the product of a bits_a-bit and a bits_b-bit number gets
+   (square: one operand), as big_mul_ntt chooses it (src/mul_ntt.c:99-118):
+   coefficient width 32..35 while the CRT bound holds, the shortest length
+   2^k or 3 * 2^k (at least 16) not below conv = ca + cb - 1 coefficients,
+   the narrowest width among equal lengths; past 2^max_lg the split multiply
+   of src/mul_split.c:146-177 (widest width the bound allows, pieces of
+   length/2 coefficients transformed at `length` = 2^lg by the reference's
+   cost model). A zero operand gives zero = 1 and no transform. Host-only:
+   touches no device. */
+typedef struct {
+    unsigned width;
+    uint64_t length;
+    uint64_t conv;
+    int radix3;
+    int split;
+    uint64_t pieces_a, pieces_b;
+    int zero;
+} ff_plan_info;
+FF_API int ff_product_plan(uint64_t bits_a, uint64_t bits_b, int square, ff_plan_info *out);
 
 /* Replaces big_free (src/bigint.c:26): frees x's limbs, sets d = NULL, n = 0. */
 FF_API void ff_big_free(Big *x);
@@ -271,6 +296,11 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
                    step (about 10 us) by microseconds of host squaring
      "dispatch"    schedules of single products and F(n): 0 the reference
                    rule (default), FF_BATCH_TUNED or FF_BATCH_LEARNED
+     "test_fail_alloc" testing hook: after this many more device or pinned
+                   allocations, every further one fails as if memory were
+                   exhausted, so the drop-in's "out of memory (N bytes)" and
+                   exit(1) (alloc.c:142-146) can be exercised; -1 (default)
+                   turns it off
    Returns FF_ERR_ARG for an unknown name or value. */
 FF_API int ff_set_option(const char *name, long long value);
 
@@ -288,6 +318,14 @@ FF_API int ff_prof_read(int k, const char **name, long *launches, double *ms, do
    companion; 4 per pointwise Montgomery product), 0 for the others. No
    counterpart in the reference. */
 FF_API int ff_prof_read_mults(int k, double *mults);
+
+/* Allocation counters (no counterpart in the reference, whose arena
+   reuses warm pages, alloc.c:19-23,74-140): the device and pinned-host
+   allocations the library has made since it started, and the bytes its
+   device pool holds for reuse. Device memory is reused from the pool in
+   stream order, so repeating the same work (F(n), a batch run, a decimal
+   conversion) leaves both counts unchanged. */
+FF_API int ff_mem_stats(uint64_t *device_allocs, uint64_t *host_allocs, size_t *cached_bytes);
 
 /* Testing hook (no counterpart in the reference): marks the carry look-back
    time-out word as set, as a bounded wait that ran out would, without any

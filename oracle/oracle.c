@@ -14,7 +14,7 @@
 
 typedef unsigned __int128 u128;
 
-const uint32_t ffo_prime[3] = {2113929217u, 2013265921u, 1811939329u}; /* ntt_arith.c:97-99 */
+const uint32_t ffo_prime[3] = {2113929217u, 2013265921u, 1811939329u}; /* ntt_arith.c:48-52 */
 const uint32_t ffo_gen[3] = {5, 31, 13};
 
 static int g_max_lg = 25; /* NTT_MAX_LG, common.h:46-48 */
@@ -41,7 +41,7 @@ static uint32_t mulm(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)((ui
 static uint32_t addm(uint32_t a, uint32_t b, uint32_t p) { uint32_t s = a + b; return s >= p ? s - p : s; }
 static uint32_t subm(uint32_t a, uint32_t b, uint32_t p) { return a >= b ? a - b : a + p - b; }
 
-/* a^e mod p (ntt_arith.c:106-114). */
+/* a^e mod p (ntt_arith.c:57-66). */
 static uint32_t powm(uint32_t a, uint64_t e, uint32_t p) {
     uint64_t r = 1, x = a % p;
     while (e) {
@@ -114,7 +114,7 @@ void ffo_ntt_forward(int pi, uint32_t *x, size_t L) { dft_any(x, L, root_of(pi, 
 
 /* The contract of ntt_conv / ntt_conv_any (ntt_conv.c:198-226,
    ntt_radix3.c:196-219) up to the reference's index reversal and factor L,
-   which its CRT undoes (crt.c:180-186); here both are undone directly. */
+   which its CRT undoes (crt.c:38-43); here both are undone directly. */
 void ffo_cyclic_conv(int pi, const uint32_t *a, const uint32_t *b, size_t L, uint32_t *out) {
     uint32_t p = ffo_prime[pi], w = root_of(pi, L), wi = invm(w, p), li = invm((uint32_t)(L % p), p);
     uint32_t *fa = xalloc(L * sizeof(uint32_t)), *fb = xalloc(L * sizeof(uint32_t));
@@ -227,8 +227,8 @@ uint64_t ffo_field(const uint64_t *d, size_t n, size_t k, unsigned bw) {
 
 /* --------------------------------------------------------------- CRT */
 
-/* Garner (crt.c:180-200) and packing at bit i*bw with one running carry
-   (crt.c:272-295), on natural-order normalized residues. */
+/* Garner (crt.c:33-64) and packing at bit i*bw with one running carry
+   (crt.c:66-154), on natural-order normalized residues. */
 void ffo_crt_pack(const uint32_t *const y[3], size_t conv, unsigned bw, uint64_t *out, size_t nout) {
     const uint32_t p1 = ffo_prime[0], p2 = ffo_prime[1], p3 = ffo_prime[2];
     const uint32_t i12 = invm(p1 % p2, p2), i13 = invm(p1 % p3, p3), i23 = invm(p2 % p3, p3);

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-/* The three NTT primes and generators (ntt_arith.c:96-100). */
+/* The three NTT primes and generators (ntt_arith.c:44-52). */
 extern const uint32_t ffo_prime[3];
 extern const uint32_t ffo_gen[3];
 
@@ -59,7 +59,7 @@ void ffo_cyclic_conv(int pi, const uint32_t *a, const uint32_t *b, size_t L, uin
 void ffo_ntt_forward(int pi, uint32_t *x, size_t L);
 
 /* Garner CRT of three residue arrays and packing at bit i*bw with carry
-   (crt.c:180-336): out gets nout limbs. */
+   (crt.c:33-194): out gets nout limbs. */
 void ffo_crt_pack(const uint32_t *const y[3], size_t conv, unsigned bw, uint64_t *out, size_t nout);
 
 /* Coefficient k of a number read as bw-bit fields (ntt_kernels.c:40-74). */

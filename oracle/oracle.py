@@ -94,7 +94,19 @@ class Oracle(_Lib):
         L.ffo_field.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_uint]
         L.ffo_fingerprint.argtypes = [_u64p, ctypes.c_size_t, _u64p, _u64p, _u64p]
         L.ffo_plan_product.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+        L.ffo_fib_step.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(_u64p), _szp, ctypes.POINTER(_u64p), _szp]
         self.primes = [2113929217, 2013265921, 1811939329]
+
+    def fib_step(self, s1, s2, k_odd: int, bit: int) -> tuple[np.ndarray, np.ndarray]:
+        """fib_step_linear (fib.c:81-109) from s1 = F(k)^2, s2 = F(k-1)^2."""
+        A, pa, na = _arr(s1)
+        B, pb, nb = _arr(s2)
+        lo, hi = _u64p(), _u64p()
+        nlo, nhi = ctypes.c_size_t(), ctypes.c_size_t()
+        self.lib.ffo_fib_step(pa, na, pb, nb, int(k_odd), int(bit), ctypes.byref(lo), ctypes.byref(nlo),
+                              ctypes.byref(hi), ctypes.byref(nhi))
+        return self._ret(lo, nlo.value), self._ret(hi, nhi.value)
 
     def _ret(self, ptr, n) -> np.ndarray:
         return self._take(ptr, n, self.lib.ffo_free)
@@ -205,6 +217,9 @@ class Reference(_Lib):
         L.ref_time_mul_list.argtypes = [_u64p, _szp, _szp, _u64p, _szp, _szp, ctypes.POINTER(ctypes.c_int),
                                         ctypes.c_size_t]
         L.ref_pool_cpus.restype = ctypes.c_int
+        L.ref_mul_list.restype = None
+        L.ref_mul_list.argtypes = [_u64p, _szp, _szp, _u64p, _szp, _szp, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t,
+                                   _u64p, _szp, _szp]
 
     def _ret(self, ptr, n) -> np.ndarray:
         return self._take(ptr, n, self.lib.ref_free)
@@ -254,6 +269,34 @@ class Reference(_Lib):
     def time_to_string(self, a) -> float:
         A, pa, na = _arr(a)
         return float(self.lib.ref_time_to_string(pa, na))
+
+    def _list_args(self, items):
+        a_parts, b_parts, oa, ob, la, lb, sq = [], [], [], [], [], [], []
+        pos_a = pos_b = 0
+        maxl = 0
+        for a, b, s in items:
+            A = np.ascontiguousarray(a, dtype=np.uint64)
+            B = A if (b is None or s) else np.ascontiguousarray(b, dtype=np.uint64)
+            a_parts.append(A), b_parts.append(B)
+            oa.append(pos_a), ob.append(pos_b), la.append(A.size), lb.append(B.size), sq.append(int(s))
+            pos_a += A.size
+            pos_b += B.size
+            maxl = max(maxl, A.size + B.size)
+        self.ntt_init(maxl)
+        return np.concatenate(a_parts), np.concatenate(b_parts), oa, ob, la, lb, sq
+
+    def mul_list(self, items) -> list:
+        """big_mul / big_sqr over items [(a_limbs, b_limbs_or_None, square)];
+        returns the products' limb arrays."""
+        AA, BB, oa, ob, la, lb, sq = self._list_args(items)
+        room = [x + y + 1 for x, y in zip(la, lb)]
+        oo = np.concatenate([[0], np.cumsum(room)[:-1]]).astype(np.uint64).tolist()
+        out = np.zeros(int(sum(room)), dtype=np.uint64)
+        nout = (ctypes.c_size_t * len(items))()
+        sz = lambda v: (ctypes.c_size_t * len(v))(*[int(x) for x in v])  # noqa: E731
+        self.lib.ref_mul_list(AA.ctypes.data_as(_u64p), sz(oa), sz(la), BB.ctypes.data_as(_u64p), sz(ob), sz(lb),
+                              (ctypes.c_int * len(sq))(*sq), len(items), out.ctypes.data_as(_u64p), sz(oo), nout)
+        return [out[int(o):int(o) + int(n)] for o, n in zip(oo, nout)]
 
     def time_mul_list(self, items) -> float:
         """items: list of (a_limbs, b_limbs_or_None, square)."""
@@ -313,6 +356,28 @@ def fib_bits(n: int) -> int:
     mp.dps = 60
     phi = (1 + sqrt(5)) / 2
     return int(floor(n * log(phi, 2) - log(sqrt(5), 2))) + 1
+
+
+def fib_digits(n: int) -> int:
+    """Decimal digits of F(n) from Binet's formula (exact for n >= 100)."""
+    if n < 100:
+        return len(str(fib_int(n)))
+    from mpmath import floor, log10, mp, sqrt
+    mp.dps = 60
+    phi = (1 + sqrt(5)) / 2
+    return int(floor(n * log10(phi) - log10(sqrt(5)))) + 1
+
+
+def decimal_mod(s: str, p: int) -> int:
+    """The value of the decimal string s modulo p (Horner over 18-digit chunks)."""
+    v = 0
+    head = len(s) % 18
+    if head:
+        v = int(s[:head]) % p
+    m = 10 ** 18 % p
+    for i in range(head, len(s), 18):
+        v = (v * m + int(s[i:i + 18])) % p
+    return v
 
 
 def fib_fingerprint(n: int) -> dict:

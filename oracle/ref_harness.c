@@ -32,7 +32,7 @@ static uint64_t *export_big(Big *r, size_t *n_out) {
     return o;
 }
 
-/** ntt_init (ntt_tables.c:268): grow-only tables for max_limbs-limb products. */
+/** ntt_init (ntt_tables.c:72): grow-only tables for max_limbs-limb products. */
 EXPORT void ref_ntt_init(size_t max_limbs) { ntt_init(max_limbs); }
 
 /** Largest transform the tables allow right now (g_ntt_max_L, mul_basic.c:26). */
@@ -110,6 +110,22 @@ EXPORT double ref_time_mul_list(const uint64_t *a, const size_t *off_a, const si
         big_free(&r);
     }
     return plat_seconds() - t0;
+}
+
+/** big_mul/big_sqr over the same kind of list, keeping every product:
+    product k is written to out + off_out[k] (room for na[k] + nb[k] + 1
+    limbs, what ntt_result allocates, mul_ntt.c:61-73) and its normalized
+    length to n_out[k]. */
+EXPORT void ref_mul_list(const uint64_t *a, const size_t *off_a, const size_t *na, const uint64_t *b,
+                         const size_t *off_b, const size_t *nb, const int *sq, size_t count, uint64_t *out,
+                         const size_t *off_out, size_t *n_out) {
+    for (size_t k = 0; k < count; k++) {
+        Big A = view_of(a + off_a[k], na[k]), B = view_of(b + off_b[k], nb[k]);
+        Big r = sq[k] ? big_sqr(&A) : big_mul(&A, &B);
+        memcpy(out + off_out[k], r.d, r.n * sizeof(uint64_t));
+        n_out[k] = r.n;
+        big_free(&r);
+    }
 }
 
 /** Seconds for one big_to_string (decimal.c:229), string discarded. */

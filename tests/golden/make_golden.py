@@ -12,8 +12,11 @@ here against Python integers or an independent method before it is written:
 * the 2^26-bit product workload: SHA-256 of the little-endian limb bytes of
   a*a and a*b for the xorshift64 operands of tests/nttfuzz.c:10-30 (seed
   88172645463325252, 2^20 limbs, top bit forced);
+* F(10^9): SHA-256 of the hex string plus the fingerprint (checked as above);
 * decimal strings: SHA-256 and length of big_to_string(F(10^5)) and of a few
-  edge values of tests/dectest.py:8 (checked against str()).
+  edge values of tests/dectest.py:8 (checked against str()), and of
+  big_to_string(F(10^8)) (checked by its digit count from Binet's formula and
+  its value modulo the fingerprint primes).
 
 Run it from the repo root where the reference checkout exists:
     python tests/golden/make_golden.py
@@ -26,7 +29,8 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 
-from oracle.oracle import FINGERPRINT_PRIMES, Reference, fib_fingerprint, fib_int, from_limbs  # noqa: E402
+from oracle.oracle import (FINGERPRINT_PRIMES, Reference, decimal_mod, fib_digits, fib_fingerprint,  # noqa: E402
+                           fib_int, from_limbs)
 
 import numpy as np  # noqa: E402
 
@@ -74,6 +78,19 @@ def main():
                        "limbs": int(limbs.size), **fp}
     g["fib"] = fib
 
+    # F(10^9): the workload of the learned-dispatcher runs; the hex digest of
+    # the reference's result, cross-checked by the fingerprint
+    n = 10 ** 9
+    limbs = ref.fibonacci(n)
+    x = from_limbs(limbs)
+    fp = fib_fingerprint(n)
+    assert x.bit_length() == fp["bits"]
+    assert x % (1 << 64) == fp["low64"]
+    assert [x % p for p in FINGERPRINT_PRIMES] == fp["mods"]
+    g["fib_large"] = {str(n): {"hex_sha256": hashlib.sha256(format(x, "x").encode()).hexdigest(),
+                               "limbs": int(limbs.size), **fp}}
+    del x, limbs
+
     a, s = xorshift_limbs(1 << 20, 88172645463325252)
     b, s = xorshift_limbs(1 << 20, s)
     sq = ref.mul_ntt(a, square=True)
@@ -105,6 +122,16 @@ def main():
         assert s == str(x)
         edges[format(x, "x")] = s
     dec["edges_hex_to_decimal"] = edges
+
+    # the decimal digits of F(10^8) by the reference's big_to_string, checked
+    # independently: the digit count from Binet's formula and the string's
+    # value modulo the four fingerprint primes against F(n) mod p
+    n = 10 ** 8
+    s8 = ref.to_string(ref.fibonacci(n))
+    assert len(s8) == fib_digits(n)
+    assert [decimal_mod(s8, p) for p in FINGERPRINT_PRIMES] == fib_fingerprint(n)["mods"]
+    dec["fib_100000000"] = {"sha256": hashlib.sha256(s8.encode()).hexdigest(), "digits": len(s8),
+                            "head": s8[:40], "tail": s8[-40:]}
     g["decimal"] = dec
 
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden.json")

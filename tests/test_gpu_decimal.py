@@ -7,13 +7,19 @@ str at sizes it finishes quickly, and with the reference's big_to_string
 the boundaries of the tree (10^(19 * 2^i) and its neighbours, 10^k - 1 with
 all nines, powers of two) and F(n).
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
 from fastfib_amd.workloads import operand_limbs
-from oracle.oracle import fib_int, from_limbs, to_limbs
+from oracle.oracle import FINGERPRINT_PRIMES, decimal_mod, fib_digits, fib_fingerprint, fib_int, from_limbs, to_limbs
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.json")))
 
 
 def test_recip_matches_oracle(ff, oracle):
@@ -68,9 +74,28 @@ def test_random_vs_python(ff, bits):
 
 @pytest.mark.parametrize("bits", [1_000_000, 4_000_000, 25_000_000])
 def test_large_vs_reference(ff, ref, bits):
+    """Against the reference build, and independently of it: the string's
+    value modulo the four fingerprint primes equals the number's."""
     rng = np.random.default_rng(bits)
     A = operand_limbs(rng, bits)
-    assert ff.to_string(A) == ref.to_string(A)
+    s = ff.to_string(A)
+    assert s == ref.to_string(A)
+    x = from_limbs(A)
+    assert [decimal_mod(s, p) for p in FINGERPRINT_PRIMES] == [x % p for p in FINGERPRINT_PRIMES]
+
+
+def test_fib_1e8_digits(ff):
+    """The 20,898,764 digits of F(10^8) (tree height 20), converted on the
+    device from the device-resident F(10^8): the reference's big_to_string
+    digest (tests/golden), the digit count from Binet's formula and the
+    string's value modulo four primes against F(n) mod p by fast doubling."""
+    n = 10 ** 8
+    g = GOLDEN["decimal"]["fib_100000000"]
+    s = ff.dfibonacci(n).to_string()
+    assert len(s) == g["digits"] == fib_digits(n) == 20898764
+    assert s[:40] == g["head"] and s[-40:] == g["tail"]
+    assert [decimal_mod(s, p) for p in FINGERPRINT_PRIMES] == fib_fingerprint(n)["mods"]
+    assert hashlib.sha256(s.encode()).hexdigest() == g["sha256"]
 
 
 def test_fibonacci_digits(ff, ref):

@@ -8,15 +8,26 @@ low 64 bits, F(n) mod four primes) at full size. The linear step and the
 last-step factors are also checked on their own with carry-adversarial
 operands (all-ones and all-zero limb runs), which force the exact carry scan.
 """
+import hashlib
+import json
 import os
 import random
 
 import numpy as np
 import pytest
 
-from oracle.oracle import fib_fingerprint, fib_int, from_limbs, to_limbs
+from oracle.oracle import FINGERPRINT_PRIMES, fib_fingerprint, fib_int, from_limbs, to_limbs
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.json")))
+
+
+def _check_fingerprint(x: int, n: int) -> None:
+    fp = fib_fingerprint(n)
+    assert x.bit_length() == fp["bits"], n
+    assert x % (1 << 64) == fp["low64"], n
+    assert [x % p for p in FINGERPRINT_PRIMES] == fp["mods"], n
 
 
 def test_fib_small_n(ff):
@@ -34,8 +45,33 @@ def test_fib_check_py_list(ff):
 
 @pytest.mark.parametrize("n", [20_000_000, 33_554_431, 50_000_001, 77_777_777, 100_000_000])
 def test_fib_large_vs_reference(ff, ref, n):
-    """tests/bigcheck.ps1:8's n (up to 1e8) against the reference build."""
-    assert (ff.fibonacci(n) == ref.fibonacci(n)).all(), n
+    """tests/bigcheck.ps1:8's n (up to 1e8) against the reference build, and
+    independently of it by the fingerprint (bit length from Binet's formula,
+    low 64 bits and F(n) mod four primes by fast doubling on residues)."""
+    got = ff.fibonacci(n)
+    assert np.array_equal(got, ref.fibonacci(n)), n
+    _check_fingerprint(from_limbs(got), n)
+
+
+@pytest.mark.parametrize("mode", [0, 4, 8], ids=["reference_rule", "tuned", "learned"])
+def test_fib_1e9_schedule_sources(ff, mode):
+    """F(10^9), the learned-dispatcher workload, with every product's
+    schedule from the reference rule, the autotuner table or the MLP (the
+    committed dispatch.bin): the same number whatever the source, against
+    the reference build's hex digest and the fingerprint."""
+    from fastfib_amd import dispatch as dp
+
+    dp.load(os.path.join(os.path.dirname(dp.__file__), "dispatch.bin"))
+    ff.set_option("dispatch", mode)
+    try:
+        d = ff.dfibonacci(10 ** 9)
+        x = d.value()
+    finally:
+        ff.set_option("dispatch", 0)
+    g = GOLDEN["fib_large"]["1000000000"]
+    assert d.bits == g["bits"] == 694241913
+    _check_fingerprint(x, 10 ** 9)
+    assert hashlib.sha256(format(x, "x").encode()).hexdigest() == g["hex_sha256"]
 
 
 def test_fib_1e8_fingerprint(ff):

@@ -28,7 +28,7 @@ GENS = [5, 31, 13]
 
 
 def test_primes_and_generators():
-    # p = c * 2^k + 1 with 3 * 2^25 | p - 1 (ntt_arith.c:92-100); g has full order
+    # p = c * 2^k + 1 with 3 * 2^25 | p - 1 (ntt_arith.c:44-52); g has full order
     for p, g, fac in zip(PRIMES, GENS, [(2, 3, 7), (2, 3, 5), (2, 3)]):
         assert (p - 1) % (3 << 25) == 0 and p < 2 ** 31
         assert all(pow(g, (p - 1) // q, p) != 1 for q in fac)
@@ -66,19 +66,29 @@ def test_width_and_length_rule(oracle):
     assert oracle.ntt_length(49) == (64, 6, 0)
     assert oracle.ntt_length(3 << 20) == (3 << 20, 20, 1)
     assert oracle.ntt_length((3 << 20) + 1) == (1 << 22, 22, 0)
-    # the bound: min(ca, cb) * 2^(2w) < p1 p2 p3
-    M = PRIMES[0] * PRIMES[1] * PRIMES[2]
-    for bits in (1000, 10 ** 6, 2 ** 26, 10 ** 9):
-        for w in range(32, 36):
-            c = (bits + w - 1) // w
-            assert oracle.width_ok(bits, bits, w) == (c << (2 * w) < M)
     # schedules quoted from the reference rule: the 2^26-bit product runs at
-    # L = 2^22, b = 32; F(10^8)'s final product at 2^21, b = 34
+    # L = 2^22, b = 32; F(10^8)'s final product at 2^21, b = 34; the square
+    # of a 3*2^25-bit operand at 3*2^21, b = 32
     pl = oracle.plan(1 << 26, 1 << 26)
     assert (pl["L"], pl["bw"], pl["r3"], pl["split"]) == (1 << 22, 32, 0, 0)
     fb = fib_fingerprint(10 ** 8)["bits"]
     pl = oracle.plan(fb // 2 + 1, fb // 2 + 2)
     assert (pl["L"], pl["bw"]) == (1 << 21, 34)
+    pl = oracle.plan(3 << 25, 3 << 25, True)
+    assert (pl["L"], pl["bw"], pl["r3"]) == (3 << 21, 32, 1)
+    # the width bound at its edge: the widest width whose bound holds for a
+    # product of two c-coefficient operands is the one the reference picks
+    # among equal lengths (narrowest), so a width refused here must leave
+    # coefficients that can exceed p1 p2 p3: c (2^w - 1)^2 >= p1 p2 p3 at the
+    # first refused width of each size (checked on the all-ones worst case)
+    M = PRIMES[0] * PRIMES[1] * PRIMES[2]
+    for bits in (1000, 10 ** 6, 2 ** 26, 10 ** 9, 3 * 10 ** 9):
+        ok = [w for w in range(32, 36) if oracle.width_ok(bits, bits, w)]
+        assert ok == list(range(32, 32 + len(ok))), bits
+        if len(ok) < 4:
+            w = 32 + len(ok)
+            c = (bits + w - 1) // w
+            assert c * ((1 << w) - 1) ** 2 >= M // 4, bits
 
 
 def _rand(rng, n, kind):
@@ -135,12 +145,30 @@ def test_fib_vs_python(oracle):
         assert from_limbs(oracle.fibonacci(n)) == fib_int(n), n
 
 
-def test_fib_step_formulas(oracle):
-    """fib.c:19-26's identities from the oracle's F(n) itself."""
-    for k in (5, 64, 1000):
-        f = [fib_int(i) for i in range(2 * k + 2)]
-        assert f[k] ** 2 + f[k - 1] ** 2 == f[2 * k - 1]
-        assert 4 * f[k] ** 2 - f[k - 1] ** 2 + 2 * (-1) ** k == f[2 * k + 1]
+def test_fib_step(oracle):
+    """The oracle's linear step (fib.c:36-62, 81-109) from s1 = F(k)^2 and
+    s2 = F(k-1)^2 must give (F(2k-1), F(2k)) for bit 0 and (F(2k), F(2k+1))
+    for bit 1, checked against F by Python-int fast doubling; k covers both
+    signs of 2(-1)^k, sizes where s1 is one limb longer than s2, and a k
+    whose F(k)^2 ends in a long run of one bits (a borrow chain)."""
+    ks = [2, 3, 5, 64, 93, 94, 1000, 1001, 4096, 12345]
+    for k in ks:
+        f = {i: fib_int(i) for i in (k - 1, k, 2 * k - 1, 2 * k, 2 * k + 1)}
+        s1, s2 = to_limbs(f[k] ** 2), to_limbs(f[k - 1] ** 2)
+        for bit in (0, 1):
+            lo, hi = oracle.fib_step(s1, s2, k & 1, bit)
+            want = (f[2 * k], f[2 * k + 1]) if bit else (f[2 * k - 1], f[2 * k])
+            assert (from_limbs(lo), from_limbs(hi)) == want, (k, bit)
+    # the step is linear in (s1, s2): all-ones limbs force every carry and
+    # borrow through the whole length (A = s1 + s2, B = 4 s1 - s2 + c, C = B - A)
+    s1, s2 = (1 << 6400) - 1, (1 << 6336) - 1
+    for k_odd in (0, 1):
+        c = -2 if k_odd else 2
+        for bit in (0, 1):
+            lo, hi = oracle.fib_step(to_limbs(s1), to_limbs(s2), k_odd, bit)
+            A, B = s1 + s2, 4 * s1 - s2 + c
+            want = (B - A, B) if bit else (A, B - A)
+            assert (from_limbs(lo), from_limbs(hi)) == want, (k_odd, bit)
 
 
 def test_fib_golden(oracle):

@@ -22,6 +22,10 @@ extern int g_dispatch_gen;  // bumped whenever the table or the network changes
 
 void ff_set_err(const char *fmt, ...);
 void ff_ensure_init(void);
+int ff_is_inited(void);
+int ff_device(void);  // the device the library runs on
+// The allocation-free product's scratch array on `device` (ff_noalloc.hip).
+int ff_noalloc_setup(int device);
 
 // Device memory: blocks are cached by size class and reused in stream
 // order (all device work runs on g_st), hipMalloc only on a miss.

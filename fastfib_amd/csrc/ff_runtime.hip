@@ -369,9 +369,17 @@ extern "C" int ff_init(int device) {
     (void)ff_lb_take(1u << 20);
     ff_lb_begin();
     (void)ff_lb_fault();
+    (void)ff_pinned_results(4096);
+    if (ff_noalloc_setup(g_device) != 0) {
+        ff_set_err("scratch array set-up failed");
+        return FF_ERR_DEVICE;
+    }
     g_inited = 1;
     return FF_OK;
 }
+
+int ff_is_inited(void) { return g_inited; }
+int ff_device(void) { return g_device; }
 
 void ff_ensure_init(void) {
     if (!g_inited && ff_init(-1) != FF_OK) {

@@ -154,6 +154,25 @@ FF_API int ff_dbig_download(const ff_dbig *x, uint64_t *limbs, size_t cap);
    (re)allocated as needed. Same schedule rule as ff_big_mul_ntt. */
 FF_API int ff_dmul(ff_dbig *out, const ff_dbig *a, const ff_dbig *b);
 
+/* No counterpart in the reference: the product for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc). out = a * b as ff_dmul computes
+   it (b == a or b == NULL squares; out may be a or b), but out->d is the
+   caller's (out->cap >= a->n + b->n + 1 limbs, else FF_ERR_ARG) and every
+   scratch buffer lives in one fixed device array of
+   FF_NOALLOC_SCRATCH_BYTES that each GPU holds from ff_init until the
+   process ends; nothing is allocated or freed. Products whose transform is
+   longer than FF_NOALLOC_MAX_LEN words (operands of up to about 2^24 bits
+   each) return FF_ERR_UNSUPPORTED with nothing launched; the reference rule
+   picks the schedule. The library must be initialized (ff_init) first,
+   else FF_ERR_ARG. Waits for the product, then sets out->n and out->bits.
+   Every call on a GPU shares that GPU's array: calls that overlapped on one
+   GPU would race; the library's lock and its single stream keep its own
+   calls from overlapping. */
+#define FF_NOALLOC_MAX_LEN (1u << 20)
+#define FF_NOALLOC_SCRATCH_BYTES (256u + 4u * 16384u + 2u * 4u * 3u * FF_NOALLOC_MAX_LEN + 256u)
+FF_API int ff_dmul_noalloc(ff_dbig *out, const ff_dbig *a, const ff_dbig *b);
+
 /* out = F(n), computed and left in device memory. */
 FF_API int ff_dfibonacci(uint64_t n, ff_dbig *out);
 

@@ -87,8 +87,20 @@ def profiled_traffic(kind: str):
 
 # One wave64 VALU instruction per 4 cycles per SIMD: 256 CUs x 4 SIMDs x 2.4 GHz / 4.
 VALU_PEAK_G = 256 * 4 * 2.4 / 4
-# 32-bit integer multiplies: one per lane per cycle, 256 CUs x 64 lanes x 2.4 GHz.
-INT32_MUL_PEAK_T = 256 * 64 * 2.4e9 / 1e12
+# 32-bit integer multiplies at the VALU issue rate, one per lane per cycle
+# (256 CUs x 64 lanes x 2.4 GHz): the spec figure beside the measured probe.
+INT32_MUL_SPEC_T = 256 * 64 * 2.4e9 / 1e12
+
+
+def int32_probe(ff) -> dict:
+    """The measured INT32-multiply peak: the library's probe kernel
+    (v_mul_lo_u32 / v_mul_hi_u32 chains on every SIMD), best of three runs."""
+    L = ff.lib()
+    L.ff_probe_int32_mul.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    rate, secs = ctypes.c_double(), ctypes.c_double()
+    ff._check(L.ff_probe_int32_mul(ctypes.byref(rate), ctypes.byref(secs)))
+    return {"measured_T_per_s": round(rate.value / 1e12, 2), "launch_ms": round(secs.value * 1e3, 3),
+            "spec_T_per_s": round(INT32_MUL_SPEC_T, 1)}
 
 
 def profiled_valu(kind: str):
@@ -115,38 +127,99 @@ def kernel_stats(ff) -> dict:
     return out
 
 
+def host_cpu() -> dict:
+    """The host's CPU as the baseline runs on it: model, logical CPUs of the
+    machine, CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity}
+
+
+REF_BUILDS = (("x86-64-v3", "libffref.so"), ("native", "libffref_native.so"))
+REF_CHOSEN = None  # the build the F(n) baseline found fastest; the other legs time it too
+
+
+def reference():
+    from oracle.oracle import Reference
+    return Reference(REF_CHOSEN) if REF_CHOSEN else Reference()
+
+
+def _ref_build_runs(path: str) -> bool:
+    """Whether a build of the reference runs on this host (a build for
+    another CPU's instruction set would fault): F(10^5) in a child process,
+    checked by its fingerprint."""
+    import subprocess
+    code = ("import sys; sys.path.insert(0, %r); from oracle.oracle import Reference, fib_fingerprint, from_limbs; "
+            "x = from_limbs(Reference(%r).fibonacci(100000)); fp = fib_fingerprint(100000); "
+            "assert x.bit_length() == fp['bits'] and x %% (1 << 64) == fp['low64']; print('ok')") % (ROOT, path)
+    try:
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    except subprocess.TimeoutExpired:
+        return False
+    return r.returncode == 0 and r.stdout.strip().endswith("ok")
+
+
 def cpu_baseline(n: int, seconds: float) -> dict | None:
     """The reference's fibonacci(n) on this host (oracle/_ref), best of runs
-    within a time budget."""
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = min(cores, 16)
+    within a time budget (the README method), for each build of it that runs
+    here: the portable AVX2 build (-march=x86-64-v3) and the build for this
+    host's own instruction set (gcc -march=native as the GPU host's gcc
+    expands it), when present. The pool uses every CPU this process may run
+    on, capped at 16 (FF_REF_THREADS): the reference starts one spinning
+    worker per CPU, and a GPU host shares its cores."""
+    host = host_cpu()
+    threads = min(host["affinity"], 16)
     os.environ["FF_REF_THREADS"] = str(threads)
     try:
-        from oracle.oracle import Reference
-        ref = Reference()
+        from oracle.oracle import REF_SO, Reference
     except Exception as e:  # not built: report nothing rather than a guess
         print(f"cpu baseline unavailable: {e}", file=sys.stderr)
         return None
-    times = []
-    t_end = time.perf_counter() + seconds
-    ref.time_fibonacci(min(n, 10 ** 6))  # start the pool
-    while time.perf_counter() < t_end or len(times) < 3:
-        times.append(ref.time_fibonacci(n))
-        if len(times) >= 200:
-            break
-    best = min(times)
-    return {"value": best * 1e3, "unit": "ms", "cores": ref.pool_cpus(), "kind": "reference",
-            "sample": f"fibonacci({n}) x {len(times)} runs, best (README method), "
-                      f"median {sorted(times)[len(times) // 2] * 1e3:.2f} ms; pool threads = cores"}
+    builds = {}
+    best_build, best, pool = None, None, None
+    for name, so in REF_BUILDS:
+        path = os.path.join(os.path.dirname(REF_SO), so)
+        if not os.path.exists(path) or (name != "x86-64-v3" and not _ref_build_runs(path)):
+            continue
+        ref = Reference(path)
+        times = []
+        t_end = time.perf_counter() + seconds / 2
+        ref.time_fibonacci(min(n, 10 ** 6))  # start the pool
+        while time.perf_counter() < t_end or len(times) < 3:
+            times.append(ref.time_fibonacci(n))
+            if len(times) >= 100:
+                break
+        b = min(times)
+        builds[name] = {"best_ms": round(b * 1e3, 3), "median_ms": round(sorted(times)[len(times) // 2] * 1e3, 3),
+                        "runs": len(times)}
+        pool = ref.pool_cpus()
+        if best is None or b < best:
+            best, best_build = b, name
+    if best is None:
+        return None
+    global REF_CHOSEN
+    REF_CHOSEN = os.path.join(os.path.dirname(REF_SO), dict(REF_BUILDS)[best_build])
+    return {"value": round(best * 1e3, 3), "unit": "ms", "cores": pool, "kind": "reference",
+            "sample": f"fibonacci({n}), best of runs in one process (README method: best time), "
+                      f"build {best_build}; pool threads = cores",
+            "host": host, "threads": pool, "builds": builds}
 
 
 BATCH_PRIMES = (2305843009213693951, 1152921504606846883)
 
 
-def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float) -> dict:
+def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bool = True) -> dict:
     """The uneven batch: count products, operand bits log-uniform in
     [2^12, 2^24], 30% squares, uniform limbs (fastfib_amd.workloads), one
     step = the whole batch. Run forms: one product at a time (plain), grouped
@@ -171,6 +244,15 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float) -> dict
     ff.sync()
     forms = [("plain", ff.BATCH_PLAIN), ("grouped", ff.BATCH_GROUPED), ("graph", ff.BATCH_GRAPH),
              ("graph_tuned", ff.BATCH_GRAPH | ff.BATCH_TUNED), ("graph_learned", ff.BATCH_GRAPH | ff.BATCH_LEARNED)]
+    # the reference's own big_mul / big_sqr over the same products
+    want = None
+    try:
+        if not cpu:
+            raise RuntimeError("skipped in a profiling run")
+        from oracle.oracle import Reference
+        want = Reference().mul_list([(A, B, B is None) for A, B in host])
+    except Exception as e:
+        print(f"reference products unavailable, residues only: {e}", file=sys.stderr)
     out, ref_limbs, ok = {}, None, True
     for name, flags in forms:
         bt = ff.Batch(pairs, flags)
@@ -184,7 +266,9 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float) -> dict
         bt.finish()
         dt = (time.perf_counter() - t0) / n_steps
         limbs = [o.limbs() for o in bt.outs]
-        if ref_limbs is None:
+        if want is not None:
+            good = all(np.array_equal(x, y) for x, y in zip(limbs, want))
+        elif ref_limbs is None:
             good = all(tuple(int.from_bytes(x.tobytes(), "little") % p for p in BATCH_PRIMES) == r
                        for x, r in zip(limbs, res))
             ref_limbs = limbs
@@ -197,8 +281,10 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float) -> dict
     plain = out["plain"]["gbit_products_per_s"]
     leg = {"workload": f"{count} uneven products, operands 2^12..2^24 bits log-uniform, 30% squares, seed {seed}",
            "operand_gbit": round(gbit, 3), "forms": out, "bit_exact": ok,
+           "checked_against": "reference big_mul/big_sqr, limb for limb" if want is not None else "residues",
            "speedup_vs_plain": {k: round(v["gbit_products_per_s"] / plain, 2) for k, v in out.items() if k != "plain"}}
-    leg["cpu_baseline"] = batch_cpu_baseline(host, gbit, cpu_seconds)
+    if cpu:
+        leg["cpu_baseline"] = batch_cpu_baseline(host, gbit, cpu_seconds)
     return leg
 
 
@@ -252,7 +338,7 @@ def fib_digits(n: int) -> int:
     return int(floor(n * log10(phi) - log10(sqrt(5)))) + 1
 
 
-def decimal_leg(ff, n: int, steps: int, cpu_seconds: float) -> dict:
+def decimal_leg(ff, n: int, steps: int, cpu_seconds: float, cpu: bool = True) -> dict:
     """Decimal digits of F(n) (device-resident input): the first conversion
     (powers, reciprocals and level graphs built), then warm conversions with
     the digits left in HBM, and one with the string copied to the host. The
@@ -277,11 +363,19 @@ def decimal_leg(ff, n: int, steps: int, cpu_seconds: float) -> dict:
     host = time.perf_counter() - t0
     ok = len(s) == fib_digits(n) and s[0] != "0" and all(digits_residue(s, q) == fib_mod(n, q) for q in DEC_PRIMES)
     leg = {"workload": f"decimal digits of F({n})", "digits": len(s), "bit_exact": ok,
-           "cold_ms": round(cold * 1e3, 3), "warm_device_ms": round(warm * 1e3, 3),
-           "warm_to_host_string_ms": round(host * 1e3, 3)}
+           "cold_ms": round(cold * 1e3, 3),
+           "cold_note": "first conversion of this size: powers 10^(19*2^i), their reciprocals and the level "
+                        "graph built, as the reference's big_to_string does on every call; the figure "
+                        "compared with the CPU baseline",
+           "warm_device_ms_cached": round(warm * 1e3, 3),
+           "warm_to_host_string_ms_cached": round(host * 1e3, 3),
+           "warm_note": "powers/reciprocals cached from the first conversion; digits left in HBM, or copied "
+                        "to a host string"}
     limbs = x.limbs()
     x.release()
-    leg["cpu_baseline"] = decimal_cpu_baseline(limbs, n, cpu_seconds)
+    leg["cpu_baseline"] = decimal_cpu_baseline(limbs, n, cpu_seconds) if cpu else None
+    if leg["cpu_baseline"]:
+        leg["speedup_cold_vs_cpu"] = round(leg["cpu_baseline"]["value"] / leg["cold_ms"], 2)
     return leg
 
 
@@ -347,7 +441,7 @@ def product_leg(ff, steps: int) -> dict:
     return out
 
 
-def large_fib_leg(ff, n: int, steps: int, cpu_seconds: float) -> dict:
+def large_fib_leg(ff, n: int, steps: int, cpu_seconds: float, cpu: bool = True) -> dict:
     """F(n) at a large n (10^9: 29 NTT products up to 3 * 2^23 words) with
     the reference rule, the autotuner table and the learned dispatcher
     choosing every product's schedule, run interleaved (one of each per
@@ -377,15 +471,15 @@ def large_fib_leg(ff, n: int, steps: int, cpu_seconds: float) -> dict:
         out[name] = {"ms": round(best[name] * 1e3, 3), "bit_exact": good[name]}
     out["bit_exact"] = all(good.values())
     out["learned_vs_reference_rule"] = round(best["reference_rule"] / best["learned"], 3)
-    out["cpu_baseline"] = large_fib_cpu_baseline(n, cpu_seconds)
+    if cpu:
+        out["cpu_baseline"] = large_fib_cpu_baseline(n, cpu_seconds)
     return out
 
 
 def large_fib_cpu_baseline(n: int, seconds: float) -> dict | None:
     """The reference's fibonacci(n) on this host, best of a few runs."""
     try:
-        from oracle.oracle import Reference
-        ref = Reference()
+        ref = reference()
     except Exception as e:
         print(f"cpu baseline unavailable: {e}", file=sys.stderr)
         return None
@@ -402,8 +496,7 @@ def large_fib_cpu_baseline(n: int, seconds: float) -> dict | None:
 def decimal_cpu_baseline(limbs, n: int, seconds: float) -> dict | None:
     """The reference's big_to_string of the same number on this host."""
     try:
-        from oracle.oracle import Reference
-        ref = Reference()
+        ref = reference()
     except Exception as e:
         print(f"cpu baseline unavailable: {e}", file=sys.stderr)
         return None
@@ -421,8 +514,7 @@ def batch_cpu_baseline(host, gbit: float, seconds: float) -> dict | None:
     """The reference's big_mul / big_sqr over the same products, in the
     batch's order, as far as the time budget goes (whole passes)."""
     try:
-        from oracle.oracle import Reference
-        ref = Reference()
+        ref = reference()
     except Exception as e:
         print(f"cpu baseline unavailable: {e}", file=sys.stderr)
         return None
@@ -439,6 +531,28 @@ def batch_cpu_baseline(host, gbit: float, seconds: float) -> dict | None:
                                            f"{best * 1e3:.1f} ms"}
 
 
+def run_legs_only(ff, args, legs) -> int:
+    """Only the named legs, no timed F(n) (profiling runs of one workload):
+    one JSON line with each leg's result, no CPU baselines."""
+    out, ok = {"legs": sorted(legs)}, True
+    if "batch" in legs:
+        out["batch"] = batch_leg(ff, args.batch_count, args.batch_seed, args.batch_steps, 0.0, cpu=False)
+        ok &= out["batch"]["bit_exact"]
+    if "product" in legs:
+        out["product_2p26"] = product_leg(ff, args.product_steps)
+        ok &= out["product_2p26"]["bit_exact"]
+    if "large" in legs:
+        out["large_fib"] = large_fib_leg(ff, args.large_n, args.large_steps, 0.0, cpu=False)
+        ok &= out["large_fib"]["bit_exact"]
+    if "decimal" in legs:
+        out["decimal"] = decimal_leg(ff, args.n, args.decimal_steps, 0.0, cpu=False)
+        ok &= out["decimal"]["bit_exact"]
+    if "probe" in legs:
+        out["int32_probe"] = int32_probe(ff)
+    print(json.dumps(out))
+    return 0 if ok else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -453,6 +567,9 @@ def main() -> int:
     ap.add_argument("--large-n", type=int, default=10 ** 9, help="F(n) leg with dispatcher schedules (0: skip)")
     ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--product-steps", type=int, default=10, help="2^26-bit product leg (0: skip)")
+    ap.add_argument("--legs", default="fib,batch,product,large,decimal",
+                    help="comma list of legs to run (fib: the timed F(n) and its roofline; a subset serves "
+                         "profiling runs of one workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the reference dispatch for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes), then exit")
@@ -475,6 +592,9 @@ def main() -> int:
     import fastfib_amd as ff
     ff.init()
     n = args.n
+    legs = set(args.legs.split(","))
+    if "fib" not in legs:
+        return run_legs_only(ff, args, legs)
     for _ in range(args.warmup):
         ff.dfibonacci(n).release()
     ff.sync()
@@ -510,11 +630,14 @@ def main() -> int:
                 "avg_launch_us": round(per_launch_ms * 1e3, 2), "launches": d["launches"]}
     if d.get("mults"):
         # algorithmic multiplies booked by the launcher (3 per twiddled
-        # butterfly, 4 per pointwise Montgomery product) over the launch time
+        # butterfly, 4 per pointwise Montgomery product) over the launch time,
+        # against the probe's measured multiply rate
+        probe = int32_probe(ff)
+        peak = probe["measured_T_per_s"]
         mr = d["mults"] / d["launches"] / (per_launch_ms * 1e-3) / 1e12
         roofline["int32_mul"] = {"mults_per_launch": round(d["mults"] / d["launches"]), "achieved": round(mr, 2),
-                                 "peak": round(INT32_MUL_PEAK_T, 1), "unit": "T int32 mul/s",
-                                 "frac": round(mr / INT32_MUL_PEAK_T, 4)}
+                                 "peak": peak, "unit": "T int32 mul/s", "frac": round(mr / peak, 4),
+                                 "peak_source": "measured by ff_probe_int32_mul on this GPU", "probe": probe}
     vi = profiled_valu(dom)
     if vi:
         # the transform kernels' actual bound: VALU issue (profiled instructions
@@ -525,20 +648,20 @@ def main() -> int:
 
     cpu = cpu_baseline(n, args.cpu_seconds)
     batch = None
-    if args.batch_count > 0:
+    if args.batch_count > 0 and "batch" in legs:
         batch = batch_leg(ff, args.batch_count, args.batch_seed, args.batch_steps, args.cpu_seconds)
         batch["dispatcher_regret"] = dispatcher_regret()
         ok &= batch["bit_exact"]
     product = None
-    if args.product_steps > 0:
+    if args.product_steps > 0 and "product" in legs:
         product = product_leg(ff, args.product_steps)
         ok &= product["bit_exact"]
     large = None
-    if args.large_n > 0:
+    if args.large_n > 0 and "large" in legs:
         large = large_fib_leg(ff, args.large_n, args.large_steps, args.cpu_seconds)
         ok &= large["bit_exact"]
     decimal = None
-    if args.decimal_steps > 0:
+    if args.decimal_steps > 0 and "decimal" in legs:
         decimal = decimal_leg(ff, n, args.decimal_steps, args.cpu_seconds)
         ok &= decimal["bit_exact"]
     line = {

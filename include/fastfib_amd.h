@@ -338,6 +338,13 @@ FF_API int ff_prof_read(int k, const char **name, long *launches, double *ms, do
    counterpart in the reference. */
 FF_API int ff_prof_read_mults(int k, double *mults);
 
+/* Issue-rate probe (no counterpart in the reference): the GPU's sustained
+   rate of 32-bit integer multiplies (v_mul_lo_u32 and v_mul_hi_u32, 8
+   independent chains per lane on every SIMD), the peak the transforms'
+   INT32-multiply roofline is quoted against; *seconds gets the timed
+   launch's duration. */
+FF_API int ff_probe_int32_mul(double *mults_per_s, double *seconds);
+
 /* Allocation counters (no counterpart in the reference, whose arena
    reuses warm pages, alloc.c:19-23,74-140): the device and pinned-host
    allocations the library has made since it started, and the bytes its

@@ -109,7 +109,6 @@ static void build_lists(ff_batch *B) {
 
     std::map<std::tuple<uint64_t, unsigned, int, int>, FFPlan> plans;
     std::vector<const FFPlan *> plan(n, nullptr);
-    B->work.assign(n, nullptr);
     std::vector<uint32_t *> fa(n), fb(n);
     std::vector<char> own_b(n, 0);  // the item transforms its b (else it reads an earlier item's)
     // products sharing their b operand (same pointer, limbs, width and
@@ -117,6 +116,29 @@ static void build_lists(ff_batch *B) {
     std::map<std::tuple<const uint64_t *, uint64_t, uint64_t, unsigned, const FFPlan *>, uint32_t *> shared_b;
     std::vector<FFCrtJob> jobs(n);
     ff_lb_begin();
+    // every item's transform arrays come from one block (one allocation per
+    // batch, not one per item: a decimal level has tens of thousands of
+    // items): the first walk sizes it, the second below carves it
+    size_t arena_bytes = 0;
+    {
+        std::set<std::tuple<const uint64_t *, uint64_t, uint64_t, unsigned, uint64_t, int, int>> seen_b;
+        for (size_t i : order) {
+            const ProdShape &s = B->shape[i];
+            const ff_prod &p = B->items[i];
+            const bool sq = !p.b || p.b == p.a;
+            const size_t words = 3 * (size_t)s.L;
+            bool ownb = false;
+            if (!sq) {
+                auto bk = std::make_tuple(p.b, (uint64_t)p.nb, (uint64_t)p.bits_b, s.bw, s.L,
+                                          ff_sched_inner_log(B->sched[i]), ff_sched_tile_words(B->sched[i]));
+                ownb = seen_b.insert(bk).second;
+            }
+            arena_bytes += al(words * 4) + (ownb ? al(words * 4) : 0) + 256;
+        }
+    }
+    char *arena = (char *)ff_dalloc(arena_bytes ? arena_bytes : 256);
+    B->work.assign(1, arena);
+    size_t arena_off = 0;
     for (size_t i : order) {
         const ProdShape &s = B->shape[i];
         const int il = ff_sched_inner_log(B->sched[i]), tw = ff_sched_tile_words(B->sched[i]);
@@ -143,8 +165,12 @@ static void build_lists(ff_batch *B) {
         }
         const bool ownb = !sq && !sb;
         const size_t o_fb = al(words * 4);
-        char *w = (char *)ff_dalloc(o_fb + (ownb ? al(words * 4) : 0) + 256);
-        B->work[i] = w;
+        char *w = arena + arena_off;
+        arena_off += o_fb + (ownb ? al(words * 4) : 0) + 256;
+        if (arena_off > arena_bytes) {
+            fprintf(stderr, "fastfib_amd: batch arena overrun\n");
+            exit(1);
+        }
         fa[i] = (uint32_t *)w;
         fb[i] = sq ? nullptr : (ownb ? (uint32_t *)(w + o_fb) : sb);
         own_b[i] = ownb;

@@ -114,6 +114,7 @@ struct DecCtx {
     std::vector<FFNum> P;   // P_0 .. P_T
     std::vector<FFNum> m;   // m_i = floor(2^(2 s_i) / P_i), i in (DEC_LEAF_LG, T]; empty below
     std::vector<uint64_t> s;
+    std::vector<uint64_t *> Y;  // the guarded reciprocals of the chain below (recip_chain)
 };
 
 // Levels below the top for one T: fixed slots, batches, one graph.
@@ -188,26 +189,120 @@ FFNum recip(const FFNum &P, uint64_t s) {
     return y3;
 }
 
-// The reciprocal made exact: m + k with k the largest such that
-// P (m + k) <= 2^(2s) (k <= 4 by big_recip's bound).
-FFNum recip_exact(const FFNum &P, uint64_t s) {
-    FFNum m = recip(P, s);
-    FFNum Z = ff_num_pow2(2 * s);
-    FFNum pm = ff_num_mul(P, &m);
-    FFNum e = ff_num_sub(Z, pm);
-    ff_num_free(Z), ff_num_free(pm);
-    for (int k = 0; k < 8 && ff_num_cmp(e, P) >= 0; k++) {
-        FFNum e2 = ff_num_sub(e, P);
-        FFNum m2 = ff_num_add_small(m, 1);
-        ff_num_free(e), ff_num_free(m);
-        e = e2, m = m2;
+// ------------------------------------------------------------------ reciprocal chain
+// The decimal levels need the exact m_i = floor(2^(2 s_i) / P_i). They are
+// formed bottom-up, every length known in advance, so the whole chain is
+// queued without waiting for the device (big_recip's recursion, kept above
+// for ff_big_recip, reads lengths and signs back at every step). With
+// G guard bits, Y_i approximates T_i* = 2^(2 s_i + G) / P_i from below:
+//
+//   Y_0 = floor(2^(2 s_0 + G) / P_0) exactly (P_0 = 10^19 < 2^64: a bitwise
+//   long division on the host);
+//   y0 = Y_{i-1}^2 >> k, k = 4 s' + G - 2 s (P_i = P_{i-1}^2, s' = s_{i-1},
+//   s = s_i): with Y_{i-1} = T*' - d, d < 2, y0 = T* - E where
+//   0 <= E <= d 2^(2s - 3s' + 2) + 1 <= 2^(s' + 3) + 1 (2s <= 4s');
+//   one Newton step y1 = y0 + floor(y0 e / 2^(2s + G)), e = 2^(2s + G) - P y0
+//   = P E >= 0: T* - y1 = E^2 / T* + (floor) < 2^(9 - G) + 1, with
+//   T* >= 2^(s + G), so Y_i = y1 is T_i or T_i - 1 (T_i = floor(T*));
+//   m_c = Y_i >> G is m_i or m_i - 1, and r = 2^(2s) - P m_c in [0, 2P)
+//   decides: m_i = m_c + [r >= P].
+// e < 2^(s + s' + 4) and r < 2^(s + 1) are formed as negations of P y0 and
+// P m_c modulo 2^(64 n) for n limbs below 2s + G and 2s bits respectively
+// (so the power of two vanishes). Per level: one square, three products,
+// two shifts, two negations, an addition, a comparison and an increment.
+
+constexpr uint64_t RC_G = 16;  // guard bits
+
+// out = a * b (b null: a^2) with operand bit bounds; lengths need not be
+// tight (leading zero limbs only make the transform what the bound says).
+void chain_mul(const uint64_t *a, uint64_t na, uint64_t bits_a, const uint64_t *b, uint64_t nb, uint64_t bits_b,
+               uint64_t *out, unsigned long long *key) {
+    const uint64_t bb = b ? bits_b : bits_a;
+    const ProdShape sh = ff_product_shape(bits_a, bb);
+    MulItem it{a, na, bits_a, b, b ? nb : na, bb, out, sh.conv, key};
+    if (sh.split) ff_mul_split(it);
+    else ff_mul_group(&it, 1, sh, FFSched{0, 0, 0});
+}
+
+// out (n limbs) = -b mod 2^(64 n), or a - b when a is given (na limbs);
+// cout, if given, gets the carry-out (1: no borrow).
+void chain_sub(const uint64_t *a, uint64_t na, const uint64_t *b, uint64_t nb, uint64_t *out, uint64_t n,
+               uint8_t *cout, void *scr) {
+    FFAddNodes J{};
+    J.a = a, J.na = a ? (na < n ? na : n) : 0;
+    J.b = b, J.nb = nb < n ? nb : n;
+    J.b_mode = FF_SUB;
+    J.out = out, J.n = n, J.N = 1;
+    J.cout = cout;
+    FF_CHECK(ff_add_nodes(J, scr, g_st));
+}
+
+void recip_chain(int T) {
+    DecCtx &D = g_dec;
+    if ((int)D.m.size() <= T) D.m.resize(T + 1, FFNum{nullptr, 0, 0, 0});
+    if (D.Y.empty()) {
+        // Y_0 = floor(2^(2 s_0 + G) / P_0) by restoring division
+        const uint64_t P0 = DEC_P0;
+        const int nbits = (int)(2 * D.s[0] + RC_G);
+        unsigned __int128 q = 0, r = 1;  // the numerator's leading 1
+        for (int b = 0; b < nbits; b++) {
+            r <<= 1;
+            q <<= 1;
+            if (r >= P0) r -= P0, q |= 1;
+        }
+        const uint64_t h[2] = {(uint64_t)q, (uint64_t)(q >> 64)};
+        uint64_t *y = (uint64_t *)ff_dalloc(16);
+        ff_copy(y, h, 16, hipMemcpyHostToDevice);
+        D.Y.push_back(y);
     }
-    if (ff_num_cmp(e, P) >= 0) {
-        fprintf(stderr, "fastfib_amd: reciprocal of 10^(19*2^i) off by more than 8\n");
-        exit(1);
+    if ((int)D.Y.size() > T) return;
+    unsigned long long *key = (unsigned long long *)ff_dalloc(8);
+    uint8_t *c = (uint8_t *)ff_dalloc(16);
+    FF_CHECK(ff_launch_fill((uint64_t *)c, 2, 0, g_st));  // c[8..15] stay zero: the increment's second carry
+    for (int i = (int)D.Y.size(); i <= T; i++) {
+        const uint64_t s1 = D.s[i - 1], s = D.s[i];
+        const FFNum &P = D.P[i];
+        const uint64_t bY = s1 + RC_G + 1, nYp = limbs_of(bY);
+        const uint64_t by = s + RC_G + 1, ny = limbs_of(by);
+        const uint64_t be = s + s1 + 7, ne = limbs_of(be);
+        const uint64_t nm = limbs_of(s + 1), nr = limbs_of(s + 2);
+        const uint64_t k = 4 * s1 + RC_G - 2 * s;
+        const uint64_t nsq = 2 * nYp + 1, nd = P.n + ny + 1, nt = ny + ne + 1, npm = P.n + nm + 1;
+        uint64_t *sq = (uint64_t *)ff_dalloc(nsq * 8), *y0 = (uint64_t *)ff_dalloc(ny * 8);
+        uint64_t *d = (uint64_t *)ff_dalloc(nd * 8), *e = (uint64_t *)ff_dalloc(ne * 8);
+        uint64_t *t = (uint64_t *)ff_dalloc(nt * 8), *corr = (uint64_t *)ff_dalloc(ny * 8);
+        uint64_t *y1 = (uint64_t *)ff_dalloc(ny * 8), *mc = (uint64_t *)ff_dalloc(nm * 8);
+        uint64_t *pm = (uint64_t *)ff_dalloc(npm * 8), *r = (uint64_t *)ff_dalloc(nr * 8);
+        uint64_t *rp = (uint64_t *)ff_dalloc(nr * 8);
+        void *scr = ff_dalloc(ff_add_nodes_scratch(1, std::max(ny, std::max(ne, nr))) + 256);
+        chain_mul(D.Y[i - 1], nYp, bY, nullptr, 0, 0, sq, key);
+        FF_CHECK(ff_shr_nodes(sq, 0, nsq, k, 1, y0, 0, ny, g_st));
+        chain_mul(P.d, P.n, s, y0, ny, by, d, key);
+        chain_sub(nullptr, 0, d, nd, e, ne, nullptr, scr);
+        chain_mul(y0, ny, by, e, ne, be, t, key);
+        FF_CHECK(ff_shr_nodes(t, 0, nt, 2 * s + RC_G, 1, corr, 0, ny, g_st));
+        {
+            FFAddNodes J{};
+            J.a = y0, J.na = ny, J.b = corr, J.nb = ny, J.b_mode = FF_ADD;
+            J.out = y1, J.n = ny, J.N = 1;
+            FF_CHECK(ff_add_nodes(J, scr, g_st));
+        }
+        FF_CHECK(ff_shr_nodes(y1, 0, ny, RC_G, 1, mc, 0, nm, g_st));
+        chain_mul(P.d, P.n, s, mc, nm, s + 1, pm, key);
+        chain_sub(nullptr, 0, pm, npm, r, nr, nullptr, scr);
+        chain_sub(r, nr, P.d, P.n, rp, nr, c, scr);  // c[0] = [r >= P]
+        FFNum m = ff_num_alloc(nm);
+        FF_CHECK(ff_copy_add_small(mc, 0, nm, c, c + 8, 1, m.d, 0, nm, g_st));
+        m.n = nm, m.bits = s + 1;  // 2^s < 2^(2s) / P < 2^(s+1): exactly s + 1 bits
+        if (i > DEC_LEAF_LG) D.m[i] = m;
+        else ff_num_free(m);
+        D.Y.push_back(y1);
+        for (void *q : {(void *)sq, (void *)y0, (void *)d, (void *)e, (void *)t, (void *)corr, (void *)mc,
+                        (void *)pm, (void *)r, (void *)rp, scr})
+            ff_dfree(q);
     }
-    ff_num_free(e);
-    return m;
+    ff_dfree(key);
+    ff_dfree(c);
 }
 
 // Powers up to P_T (by squaring, decimal.c:236-244).
@@ -224,12 +319,7 @@ void ensure_pow(int T) {
 }
 
 // Exact reciprocals of P_5 .. P_T.
-void ensure_recip(int T) {
-    if ((int)g_dec.m.size() <= T) g_dec.m.resize(T + 1, FFNum{nullptr, 0, 0, 0});
-    for (int i = DEC_LEAF_LG + 1; i <= T; i++) {
-        if (!g_dec.m[i].d) g_dec.m[i] = recip_exact(g_dec.P[i], g_dec.s[i]);
-    }
-}
+void ensure_recip(int T) { recip_chain(T); }
 
 // The smallest T with 2 (bits(P_T) - 1) >= bits(a) (decimal.c:236-244),
 // with the powers and reciprocals it needs.
@@ -309,6 +399,19 @@ ff_batch *level_batch(const uint64_t *a, uint64_t sa, uint64_t na, uint64_t bits
     return B;
 }
 
+// The levels and leaves of a plan captured in a HIP graph, for the plan's
+// second and later conversions (the first queues them directly: capture,
+// instantiation and the graph's first upload cost more than one direct run).
+void capture_plan(DecPlan &pl) {
+    FF_CHECK(hipStreamSynchronize(g_st));
+    ff_prof_pause(1);
+    FF_CHECK(hipStreamBeginCapture(g_st, hipStreamCaptureModeRelaxed));
+    enqueue_tail(pl);
+    FF_CHECK(hipStreamEndCapture(g_st, &pl.graph));
+    ff_prof_pause(0);
+    FF_CHECK(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0));
+}
+
 // The plan below the top for height T > DEC_LEAF_LG.
 DecPlan *get_plan(int T) {
     auto it = g_plans.find(T);
@@ -371,14 +474,6 @@ DecPlan *get_plan(int T) {
     pl.digits = (char *)palloc(pl, pl.ndigits + 64);
     pl.first_nz = (unsigned long long *)palloc(pl, 8);
     pl.err = (uint32_t *)palloc(pl, 8);
-    // capture the levels and the leaves once
-    FF_CHECK(hipStreamSynchronize(g_st));
-    ff_prof_pause(1);
-    FF_CHECK(hipStreamBeginCapture(g_st, hipStreamCaptureModeRelaxed));
-    enqueue_tail(pl);
-    FF_CHECK(hipStreamEndCapture(g_st, &pl.graph));
-    ff_prof_pause(0);
-    FF_CHECK(hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0));
     DecPlan *raw = up.get();
     g_plans[T] = std::move(up);
     return raw;
@@ -486,9 +581,12 @@ void decimal_dev(const uint64_t *a, uint64_t na, uint64_t bits, const char **dev
                            first_nz, err);
         FF_CHECK(hipGetLastError());
     } else {
+        const bool fresh = g_plans.find(T) == g_plans.end();
         DecPlan *pl = get_plan(T);
+        if (!fresh && !pl->exec) capture_plan(*pl);
         top_split(*pl, a, na, bits);
-        FF_CHECK(hipGraphLaunch(pl->exec, g_st));
+        if (pl->exec) FF_CHECK(hipGraphLaunch(pl->exec, g_st));
+        else enqueue_tail(*pl);
         dig = pl->digits;
         nd = pl->ndigits;
         first_nz = pl->first_nz;
@@ -587,6 +685,24 @@ extern "C" int ff_dbig_to_decimal(const ff_dbig *a, const char **dev_digits, siz
         return FF_OK;
     }
     decimal_dev(a->d, a->n, a->bits, dev_digits, len);
+    return FF_OK;
+}
+
+extern "C" int ff_dec_reciprocal(int i, uint64_t *limbs, size_t cap, size_t *n) {
+    if (i <= DEC_LEAF_LG || i > 40 || !n) return FF_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    ff_ensure_init();
+    ensure_pow(i);
+    recip_chain(i);
+    const FFNum &m = g_dec.m[i];
+    uint64_t nn = m.n;
+    std::vector<uint64_t> h(nn);
+    ff_copy(h.data(), m.d, nn * 8, hipMemcpyDeviceToHost);
+    while (nn > 1 && h[nn - 1] == 0) nn--;
+    *n = nn;
+    if (!limbs) return FF_OK;
+    if (cap < nn) return FF_ERR_ARG;
+    memcpy(limbs, h.data(), nn * 8);
     return FF_OK;
 }
 

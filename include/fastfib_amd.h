@@ -97,6 +97,13 @@ typedef struct {
 } ff_plan_info;
 FF_API int ff_product_plan(uint64_t bits_a, uint64_t bits_b, int square, ff_plan_info *out);
 
+/* The exact reciprocal floor(2^(2 s) / P_i) of P_i = 10^(19 * 2^i) (s its
+   bit length), i in 5..40, that ff_big_to_string's Barrett levels use
+   (where the reference calls big_recip, decimal.c:252-259, and tolerates
+   its 4-low window): *n gets its limb count; limbs, when not NULL, gets the
+   limbs (room for cap). No counterpart in the reference; for tests. */
+FF_API int ff_dec_reciprocal(int i, uint64_t *limbs, size_t cap, size_t *n);
+
 /* Replaces big_free (src/bigint.c:26): frees x's limbs, sets d = NULL, n = 0. */
 FF_API void ff_big_free(Big *x);
 

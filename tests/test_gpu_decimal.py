@@ -115,3 +115,37 @@ def test_repeat_and_mixed_heights(ff):
     want = [str(x) for x in xs]
     for _ in range(2):
         assert [ff.to_string(x) for x in xs] == want
+
+
+def _dec_recip(ff, i):
+    import ctypes
+    L = ff.lib()
+    L.ff_dec_reciprocal.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    n = ctypes.c_size_t()
+    assert L.ff_dec_reciprocal(i, None, 0, ctypes.byref(n)) == 0
+    out = np.zeros(n.value, dtype=np.uint64)
+    assert L.ff_dec_reciprocal(i, out.ctypes.data, out.size, ctypes.byref(n)) == 0
+    return from_limbs(out)
+
+
+def test_decimal_reciprocal_chain(ff):
+    """The exact reciprocals m_i = floor(2^(2s) / 10^(19 * 2^i)) the Barrett
+    levels use (formed bottom-up from squares and one Newton step each, no
+    length read back): equal to Python's floor division where Python is
+    quick (i <= 12), and beyond, up to F(10^8)'s height 20, the defining
+    inequalities P m <= 2^(2s) < P (m + 1) with P and the products formed
+    exactly by the library's multiply."""
+    for i in range(5, 13):
+        P = 10 ** (19 * 2 ** i)
+        s = P.bit_length()
+        assert _dec_recip(ff, i) == (1 << (2 * s)) // P, i
+    P = to_limbs(10 ** (19 * 2 ** 12))
+    for i in range(13, 21):
+        P = ff.big_sqr(P)  # P_i = P_(i-1)^2, exact
+        Pv = from_limbs(P)
+        s = Pv.bit_length()
+        m = _dec_recip(ff, i)
+        assert m.bit_length() == s + 1, i
+        lo = from_limbs(ff.big_mul(P, to_limbs(m)))
+        hi = from_limbs(ff.big_mul(P, to_limbs(m + 1)))
+        assert lo <= (1 << (2 * s)) < hi, i

@@ -534,6 +534,60 @@ def batch_cpu_baseline(host, gbit: float, seconds: float) -> dict | None:
                                            f"{best * 1e3:.1f} ms"}
 
 
+def small_leg(ff, seed: int, steps: int) -> dict:
+    """Products below the NTT threshold (both operands under 200 limbs) by
+    the schoolbook kernel against the one-block NTT they took before
+    (school_limbs 0): (a) the uneven batch's products of that kind (the same
+    draw as the batch leg, grouped form), (b) one whole warm decimal
+    conversion of F(10^7), whose low tree levels multiply such numbers by
+    the thousand. Results are compared between the two paths."""
+    import numpy as np
+    from fastfib_amd.workloads import operand_limbs, uneven_mix
+
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for ba, bb in uneven_mix(seed, 4096):
+        A = operand_limbs(rng, ba)
+        B = None if bb is None else operand_limbs(rng, bb)
+        if A.size < 200 and (B is None or B.size < 200):
+            pairs.append((ff.Device(A), None if B is None else ff.Device(B)))
+    out, limbs = {}, {}
+    for name, bound in (("schoolbook", 200), ("schoolbook_below_64", 64), ("schoolbook_below_32", 32),
+                        ("one_block_ntt", 0)):
+        ff.set_option("school_limbs", bound)
+        try:
+            bt = ff.Batch(pairs, ff.BATCH_GROUPED)
+            bt.run()
+            bt.finish()
+            ff.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                bt.run()
+            bt.finish()
+            out[f"batch_{name}_us"] = round((time.perf_counter() - t0) / steps * 1e6, 1)
+            limbs[name] = [o.limbs() for o in bt.outs]
+            bt.close()
+            x = ff.dfibonacci(10 ** 7)
+            s0 = x.to_string()
+            x.to_string()  # the plan's graph
+            ff.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                s = x.to_string()
+            out[f"decimal_F1e7_{name}_ms"] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+            limbs[name + "_dec"] = s
+            x.release()
+        finally:
+            ff.set_option("school_limbs", 32)  # the library default
+    same = True
+    for name in ("schoolbook_below_64", "schoolbook_below_32", "one_block_ntt"):
+        same &= all(np.array_equal(a, b) for a, b in zip(limbs["schoolbook"], limbs[name]))
+        same &= limbs["schoolbook_dec"] == limbs[name + "_dec"]
+    return {"workload": f"{len(pairs)} products of the uneven batch (seed {seed}) with both operands under 200 "
+                        "limbs; warm decimal conversion of F(10^7)", "products": len(pairs), **out,
+            "identical": bool(same)}
+
+
 def run_legs_only(ff, args, legs) -> int:
     """Only the named legs, no timed F(n) (profiling runs of one workload):
     one JSON line with each leg's result, no CPU baselines."""
@@ -552,6 +606,9 @@ def run_legs_only(ff, args, legs) -> int:
         ok &= out["decimal"]["bit_exact"]
     if "probe" in legs:
         out["int32_probe"] = int32_probe(ff)
+    if "small" in legs:
+        out["small_products"] = small_leg(ff, args.batch_seed, max(3, args.batch_steps))
+        ok &= out["small_products"]["identical"]
     print(json.dumps(out))
     return 0 if ok else 1
 
@@ -570,7 +627,7 @@ def main() -> int:
     ap.add_argument("--large-n", type=int, default=10 ** 9, help="F(n) leg with dispatcher schedules (0: skip)")
     ap.add_argument("--large-steps", type=int, default=5)
     ap.add_argument("--product-steps", type=int, default=10, help="2^26-bit product leg (0: skip)")
-    ap.add_argument("--legs", default="fib,batch,product,large,decimal",
+    ap.add_argument("--legs", default="fib,batch,product,large,decimal,small",
                     help="comma list of legs to run (fib: the timed F(n) and its roofline; a subset serves "
                          "profiling runs of one workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -663,6 +720,10 @@ def main() -> int:
     if args.large_n > 0 and "large" in legs:
         large = large_fib_leg(ff, args.large_n, args.large_steps, args.cpu_seconds)
         ok &= large["bit_exact"]
+    small = None
+    if "small" in legs:
+        small = small_leg(ff, args.batch_seed, max(3, args.batch_steps))
+        ok &= small["identical"]
     decimal = None
     if args.decimal_steps > 0 and "decimal" in legs:
         decimal = decimal_leg(ff, n, args.decimal_steps, args.cpu_seconds)
@@ -680,6 +741,7 @@ def main() -> int:
         "decimal": decimal,
         "large_fib": large,
         "product_2p26": product,
+        "small_products": small,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 4),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
                     for k, v in stats.items()},

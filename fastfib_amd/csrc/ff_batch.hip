@@ -51,6 +51,7 @@ struct ff_batch {
     std::vector<ProdShape> shape;
     std::vector<FFSched> sched;
     std::vector<char> zero;
+    std::vector<char> school;  // both operands below the schoolbook bound: no transforms
     std::vector<void *> work;
     unsigned long long *slots = nullptr;
     char *desc = nullptr;
@@ -104,7 +105,7 @@ static void build_lists(ff_batch *B) {
     const size_t n = B->items.size();
     std::vector<size_t> order;
     for (size_t i = 0; i < n; i++)
-        if (!B->zero[i]) order.push_back(i);
+        if (!B->zero[i] && !B->school[i]) order.push_back(i);
     std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return B->shape[x].L > B->shape[y].L; });
 
     std::map<std::tuple<uint64_t, unsigned, int, int>, FFPlan> plans;
@@ -401,6 +402,29 @@ static void build_lists(ff_batch *B) {
             pend.push_back(pd);
         }
     }
+    {
+        // schoolbook items: one wavefront each up to 128 output limbs, else
+        // one workgroup each
+        std::vector<FFSchoolItem> sch[2];
+        double sbytes[2] = {0, 0};
+        for (size_t i = 0; i < n; i++) {
+            if (!B->school[i]) continue;
+            const ff_prod &p = B->items[i];
+            const bool sq = !p.b || p.b == p.a;
+            const uint64_t nb = sq ? p.na : p.nb;
+            const int wave = p.na + nb + 1 <= FF_SCHOOL_WAVE_OUT;
+            sch[wave].push_back(FFSchoolItem{p.a, sq ? p.a : p.b, (uint32_t)p.na, (uint32_t)nb, p.out, B->slots + i});
+            sbytes[wave] += 8.0 * (2 * p.na + 2 * nb + 1);
+        }
+        for (int wave = 0; wave < 2; wave++) {
+            if (sch[wave].empty()) continue;
+            Pending pd{};
+            pd.L.kind = wave ? 5 : 4, pd.L.n = (int)sch[wave].size(), pd.L.bytes = sbytes[wave];
+            pd.items_off = put(sch[wave].data(), sch[wave].size() * sizeof(FFSchoolItem));
+            pd.start_off = 0;
+            pend.push_back(pd);
+        }
+    }
     B->desc = (char *)ff_dalloc(img.size() + 256);
     ff_copy(B->desc, img.data(), img.size(), hipMemcpyHostToDevice);
     for (auto &pd : pend) {
@@ -433,6 +457,12 @@ static void enqueue_grouped(ff_batch *B) {
             FF_CHECK(ff_launch_crt_list((const FFCrtJob *)L.items, L.start, L.n, L.blocks, L.conv_total, L.out_total,
                                         g_st));
             break;
+        case 4:
+            FF_CHECK(ff_launch_school_list((const FFSchoolItem *)L.items, L.n, L.bytes, g_st));
+            break;
+        case 5:
+            FF_CHECK(ff_launch_school_wave_list((const FFSchoolItem *)L.items, L.n, L.bytes, g_st));
+            break;
         default:
             FF_CHECK(ff_launch_fill(L.fill, 1, 0, g_st));
             break;
@@ -462,6 +492,7 @@ int ff_batch_create_sched(ff_batch **out, const ff_prod *items, size_t count, in
     B->shape.resize(count);
     B->sched.resize(count);
     B->zero.assign(count, 0);
+    B->school.assign(count, 0);
     std::vector<ProdShape> s0;
     std::vector<uint64_t> ba, bb;
     std::vector<uint8_t> sq;
@@ -474,6 +505,7 @@ int ff_batch_create_sched(ff_batch **out, const ff_prod *items, size_t count, in
             B->zero[i] = 1;
             continue;
         }
+        if (ff_school_ok(p.na, square ? p.na : p.nb)) B->school[i] = 1;
         const ProdShape s = ff_product_shape(p.bits_a, bits_b);
         if (s.split) {
             delete B;
@@ -531,7 +563,15 @@ extern "C" int ff_batch_run(ff_batch *B) {
                 FF_CHECK(ff_launch_fill(B->items[i].out, 1, 0, g_st));
                 continue;
             }
-            ff_mul_group(&B->plain[k++], 1, B->shape[i], B->sched[i]);
+            const MulItem &m = B->plain[k++];
+            if (B->school[i]) {
+                FFSchoolSet ss = {};
+                ss.j[0] = FFSchoolItem{m.a, m.b ? m.b : m.a, (uint32_t)m.na, (uint32_t)(m.b ? m.nb : m.na), m.out,
+                                       m.lenkey};
+                FF_CHECK(ff_launch_school_set(&ss, 1, 8.0 * (2 * m.na + 2 * (m.b ? m.nb : m.na) + 1), g_st));
+                continue;
+            }
+            ff_mul_group(&m, 1, B->shape[i], B->sched[i]);
         }
     }
     return FF_OK;

@@ -149,7 +149,9 @@ struct DecPlan {
 };
 
 DecCtx g_dec;
-std::map<int, std::unique_ptr<DecPlan>> g_plans;
+// plans per tree height and schoolbook bound (the bound decides, at plan
+// time, which level products skip the transforms)
+std::map<std::pair<int, int>, std::unique_ptr<DecPlan>> g_plans;
 
 uint64_t limbs_of(uint64_t bits) { return bits ? (bits + 63) / 64 : 1; }
 
@@ -414,7 +416,7 @@ void capture_plan(DecPlan &pl) {
 
 // The plan below the top for height T > DEC_LEAF_LG.
 DecPlan *get_plan(int T) {
-    auto it = g_plans.find(T);
+    auto it = g_plans.find({T, g_school_limbs});
     if (it != g_plans.end()) return it->second.get();
     auto up = std::make_unique<DecPlan>();
     DecPlan &pl = *up;
@@ -475,7 +477,7 @@ DecPlan *get_plan(int T) {
     pl.first_nz = (unsigned long long *)palloc(pl, 8);
     pl.err = (uint32_t *)palloc(pl, 8);
     DecPlan *raw = up.get();
-    g_plans[T] = std::move(up);
+    g_plans[{T, g_school_limbs}] = std::move(up);
     return raw;
 }
 
@@ -581,7 +583,7 @@ void decimal_dev(const uint64_t *a, uint64_t na, uint64_t bits, const char **dev
                            first_nz, err);
         FF_CHECK(hipGetLastError());
     } else {
-        const bool fresh = g_plans.find(T) == g_plans.end();
+        const bool fresh = g_plans.find({T, g_school_limbs}) == g_plans.end();
         DecPlan *pl = get_plan(T);
         if (!fresh && !pl->exec) capture_plan(*pl);
         top_split(*pl, a, na, bits);

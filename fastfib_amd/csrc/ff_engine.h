@@ -46,6 +46,8 @@ struct MulItem {
 
 // One product past the longest transform (split into transformed-once pieces).
 void ff_mul_split(const MulItem &it);
+// Both operands (normalized limb counts) below the schoolbook bound.
+bool ff_school_ok(uint64_t na, uint64_t nb);
 // A split product's coefficient width and piece transform length (log2).
 void ff_split_plan(uint64_t bits_a, uint64_t bits_b, bool sq, unsigned *bw, int *lgp);
 

@@ -193,6 +193,10 @@ static void dnum_free(DNum &x) {
 
 static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
+bool ff_school_ok(uint64_t na, uint64_t nb) {
+    return na < (uint64_t)g_school_limbs && nb < (uint64_t)g_school_limbs;
+}
+
 int ff_sched_inner_log(const FFSched &sc) { return sc.inner_log ? sc.inner_log : g_inner_log; }
 int ff_sched_tile_words(const FFSched &sc) { return sc.tile_words ? sc.tile_words : g_tile_words; }
 
@@ -303,8 +307,29 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
         out[i].slot = slot;
         items[i] = MulItem{a->d, a->n, a->bits, sq ? nullptr : b->d, nb, bb, out[i].d, shapes[i].conv, g_slots + slot};
     }
-    // one group per (length, width), up to FF_MAXB products per launch
+    // one group per (length, width), up to FF_MAXB products per launch;
+    // products whose operands are both below the NTT threshold go to the
+    // schoolbook kernel instead, up to FF_MAXB per launch
     std::vector<int> done(cnt, 0);
+    {
+        FFSchoolSet ss = {};
+        int ns = 0;
+        double sbytes = 0;
+        for (int i = 0; i < cnt; i++) {
+            if (zero[i] || shapes[i].split || !ff_school_ok(items[i].na, items[i].b ? items[i].nb : items[i].na))
+                continue;
+            const MulItem &m = items[i];
+            ss.j[ns++] = FFSchoolItem{m.a, m.b ? m.b : m.a, (uint32_t)m.na, (uint32_t)(m.b ? m.nb : m.na), m.out,
+                                      m.lenkey};
+            sbytes += 8.0 * (2 * m.na + 2 * (m.b ? m.nb : m.na) + 1);
+            done[i] = 1;
+            if (ns == FF_MAXB) {
+                FF_CHECK(ff_launch_school_set(&ss, ns, sbytes, g_st));
+                ns = 0, sbytes = 0;
+            }
+        }
+        if (ns) FF_CHECK(ff_launch_school_set(&ss, ns, sbytes, g_st));
+    }
     for (int i = 0; i < cnt; i++) {
         if (zero[i] || done[i]) continue;
         if (shapes[i].split) {

@@ -123,8 +123,27 @@ struct FFInnerItem {
 // Kernel kinds for the live per-launch timer (ff_prof_*).
 enum FFKernelKind {
     FK_FWD_TOP, FK_FWD, FK_INNER, FK_SMALL, FK_INV, FK_INV_TOP,
-    FK_CRT_PACK, FK_CRT_SCAN, FK_CRT_APPLY, FK_STEP, FK_CARRY, FK_NORM, FK_LINEAR, FK_DEC_LEAF, FK_EARLY, FK_STEP_CRT, FK_COUNT
+    FK_CRT_PACK, FK_CRT_SCAN, FK_CRT_APPLY, FK_STEP, FK_CARRY, FK_NORM, FK_LINEAR, FK_DEC_LEAF, FK_EARLY, FK_STEP_CRT,
+    FK_SCHOOL, FK_COUNT
 };
+
+// Schoolbook products below the NTT threshold (ff_school.hip): both
+// operands shorter than FF_SCHOOL_LIMBS limbs, one workgroup per product.
+#define FF_SCHOOL_LIMBS 200
+struct FFSchoolItem {
+    const uint64_t *a, *b;   // b == a: a square
+    uint32_t na, nb;         // normalized lengths, both < FF_SCHOOL_LIMBS
+    uint64_t *out;           // nout = na + nb + 1 limbs written
+    unsigned long long *key; // (top nonzero limb + 1) << 8 | its bit length; 0 for zero
+};
+struct FFSchoolSet {
+    FFSchoolItem j[FF_MAXB];
+};
+extern "C" hipError_t ff_launch_school_list(const FFSchoolItem *items, int n, double bytes, hipStream_t st);
+extern "C" hipError_t ff_launch_school_set(const FFSchoolSet *set, int n, double bytes, hipStream_t st);
+// Items of at most 128 output limbs (na + nb + 1 <= 128), one per wavefront.
+#define FF_SCHOOL_WAVE_OUT 128
+extern "C" hipError_t ff_launch_school_wave_list(const FFSchoolItem *items, int n, double bytes, hipStream_t st);
 
 extern "C" {
 // ff_engine.hip: when enabled, brackets each launch with HIP events on its

@@ -204,6 +204,17 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
     const int G = 1 << NL;
     const int per_sub = (1 << (t.rho - NL)) << t.logC;
     const int ngroups = per_sub * t.nsub;
+    // The group's rows differ from its first only in row bits [b_lo, b_lo +
+    // NL), disjoint from every other bit of the position, so the padded LDS
+    // address of row j splits into the first row's address plus an offset
+    // that is the same for every group of the round (uniform: kept in
+    // scalar registers, one addition per element instead of the address).
+    int off[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+        const int J = j << b_lo;
+        off[j] = J * t.ld + (t.pad ? (J >> 4) : 0);
+    }
     for (int gi = ff_tile_tid(t); gi < ngroups; gi += ff_tile_nthr(t)) {
         const int sub = gi / per_sub;
         const int g = gi - sub * per_sub;
@@ -212,11 +223,12 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
         const int low = rest & ((1 << b_lo) - 1);
         const int base = low | ((rest >> b_lo) << (b_lo + NL));
         const int pos0 = sub << t.rho;
+        const int a0 = ff_pos_addr(t, pos0 + base, c);
         uint32_t v[G];
 #pragma unroll
         for (int j = 0; j < G; j++) {
             const int pos = pos0 + (base | (j << b_lo));
-            v[j] = FROM_G ? ld(pos, c) : t.s[ff_pos_addr(t, pos, c)];
+            v[j] = FROM_G ? ld(pos, c) : t.s[a0 + off[j]];
         }
         if constexpr (Gt::on && KIND == FF_R_DIT && FROM_G) {
             // b_lo = 0 here: rows pos0 + base + j
@@ -232,10 +244,7 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
         } else {
             ff_dif_group<NL, Gt::lds_tables>(v, low, b_lo, rt, rtp, p, pinv);
 #pragma unroll
-            for (int j = 0; j < G; j++) {
-                const int pos = pos0 + (base | (j << b_lo));
-                v[j] = gt.mid(v[j], B ? B[ff_pos_addr(t, pos, c)] : v[j], p, pinv);
-            }
+            for (int j = 0; j < G; j++) v[j] = gt.mid(v[j], B ? B[a0 + off[j]] : v[j], p, pinv);
             ff_dit_group<NL, Gt::lds_tables>(v, low, b_lo, irt, irtp, p, pinv);
         }
         if constexpr (Gt::on && KIND == FF_R_DIF && TO_G) {
@@ -248,7 +257,7 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
         for (int j = 0; j < G; j++) {
             const int pos = pos0 + (base | (j << b_lo));
             if (TO_G) st(pos, c, v[j]);
-            else t.s[ff_pos_addr(t, pos, c)] = v[j];
+            else t.s[a0 + off[j]] = v[j];
         }
     }
 }

@@ -17,6 +17,7 @@ extern int g_pair_steps;   // bit 0: doubling steps as weighted pairs; bit 1: th
 extern int g_quad_max_lg;     // longest quadrupling transform 2^quad_max_lg (0: no quadrupling)
 extern int g_quad_inner_log;  // inner block of quadrupling transforms (0: inner_log)
 extern int g_host_bits;       // top bits of n done on the host before the device chain (10..13)
+extern int g_school_limbs;    // products with both operands shorter than this run schoolbook (0: none)
 extern int g_dispatch;      // schedule source of the drop-in products and F(n): 0, FF_BATCH_TUNED or FF_BATCH_LEARNED
 extern int g_dispatch_gen;  // bumped whenever the table or the network changes
 

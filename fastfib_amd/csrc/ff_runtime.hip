@@ -29,6 +29,7 @@ int g_pair_steps = 2;
 int g_quad_max_lg = 18;
 int g_quad_inner_log = 10;
 int g_host_bits = 13;
+int g_school_limbs = 32;  // measured: the one-block NTT is as fast or faster from 32 limbs up
 int g_dispatch_gen = 0;
 
 void ff_set_err(const char *fmt, ...) {
@@ -301,7 +302,8 @@ static void prof_collect(void) {
 
 static const char *kKindNames[FK_COUNT] = {"ntt_fwd_top", "ntt_fwd", "ntt_inner", "ntt_small", "ntt_inv",
                                            "ntt_inv_top", "crt_pack", "crt_scan", "crt_apply", "fib_step",
-                                           "carry", "normlen", "bigops", "dec_leaf", "fib_early", "step_crt"};
+                                           "carry", "normlen", "bigops", "dec_leaf", "fib_early", "step_crt",
+                                           "school"};
 
 extern "C" int ff_prof_enable(int on) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -425,6 +427,9 @@ extern "C" int ff_set_option(const char *name, long long v) {
     } else if (!strcmp(name, "host_bits")) {
         if (v < 10 || v > 13) return FF_ERR_ARG;
         g_host_bits = (int)v;
+    } else if (!strcmp(name, "school_limbs")) {
+        if (v < 0 || v > FF_SCHOOL_LIMBS) return FF_ERR_ARG;
+        g_school_limbs = (int)v;
     } else if (!strcmp(name, "test_fail_alloc")) {
         if (v < -1) return FF_ERR_ARG;
         g_fail_after = v;

@@ -322,6 +322,12 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
                    step (about 10 us) by microseconds of host squaring
      "dispatch"    schedules of single products and F(n): 0 the reference
                    rule (default), FF_BATCH_TUNED or FF_BATCH_LEARNED
+     "school_limbs" products whose operands are both shorter than this many
+                   limbs (0..200) are multiplied by the schoolbook method on
+                   the GPU (one wavefront or workgroup per product, no
+                   transforms), as the reference does below its NTT threshold
+                   of 200 limbs (mul_basic.c:33-35); default 32, where the
+                   one-block NTT measured as fast from there up
      "test_fail_alloc" testing hook: after this many more device or pinned
                    allocations, every further one fails as if memory were
                    exhausted, so the drop-in's "out of memory (N bytes)" and

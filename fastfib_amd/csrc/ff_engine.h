@@ -25,6 +25,7 @@ struct FFSched {
     int len_up;
     int inner_log;
     int tile_words;
+    int split_lg;  // split products: piece transform 2^split_lg (0: the reference's cost model)
 };
 
 ProdShape ff_product_shape(uint64_t bits_a, uint64_t bits_b);
@@ -45,11 +46,12 @@ struct MulItem {
 };
 
 // One product past the longest transform (split into transformed-once pieces).
-void ff_mul_split(const MulItem &it);
+void ff_mul_split(const MulItem &it, int split_lg = 0);
 // Both operands (normalized limb counts) below the schoolbook bound.
 bool ff_school_ok(uint64_t na, uint64_t nb);
 // A split product's coefficient width and piece transform length (log2).
-void ff_split_plan(uint64_t bits_a, uint64_t bits_b, bool sq, unsigned *bw, int *lgp);
+// split_lg > 0 forces the piece length (kept within [max(10, max_lg - 4), max_lg]).
+void ff_split_plan(uint64_t bits_a, uint64_t bits_b, bool sq, unsigned *bw, int *lgp, int split_lg = 0);
 
 // Runs up to FF_MAXB products sharing one shape and schedule, one launch
 // per pass (transform arrays come from the memory pool, stream-ordered).

@@ -286,6 +286,13 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
         }
         shapes[i] = product_shape(a->bits, bb);
         if (shapes[i].split) {
+            if (given) {
+                sched[i] = given[i];
+            } else if (g_dispatch) {
+                const uint64_t ba = a->bits;
+                const uint8_t s1 = sq;
+                ff_pick_scheds(&shapes[i], &ba, &bb, &s1, 1, (size_t)cnt, g_dispatch, &sched[i]);
+            }
             const uint64_t nb = sq ? a->n : b->n;
             const int slot = out[i].slot;
             out[i] = dnum_alloc(out_room(shapes[i], a->n, nb));
@@ -333,7 +340,7 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
     for (int i = 0; i < cnt; i++) {
         if (zero[i] || done[i]) continue;
         if (shapes[i].split) {
-            ff_mul_split(items[i]);
+            ff_mul_split(items[i], sched[i].split_lg);
             done[i] = 1;
             continue;
         }

@@ -135,20 +135,22 @@ static int split_lg(uint64_t bits_a, uint64_t bits_b, unsigned bw, bool sq) {
     return lgp;
 }
 
-void ff_split_plan(uint64_t bits_a, uint64_t bits_b, bool sq, unsigned *bw_out, int *lgp_out) {
+void ff_split_plan(uint64_t bits_a, uint64_t bits_b, bool sq, unsigned *bw_out, int *lgp_out, int force_lg) {
     // the widest width the CRT bound allows (mul_split.c:152-154)
     unsigned bw = 35;
     while (bw > 8 && !ff_width_ok(bits_a, bits_b, bw)) bw--;
     *bw_out = bw;
-    *lgp_out = split_lg(bits_a, bits_b, bw, sq);
+    const int lo = g_max_lg - 4 > 10 ? g_max_lg - 4 : 10;
+    if (force_lg > 0) *lgp_out = force_lg < lo ? lo : (force_lg > g_max_lg ? g_max_lg : force_lg);
+    else *lgp_out = split_lg(bits_a, bits_b, bw, sq);
 }
 
-void ff_mul_split(const MulItem &it) {
+void ff_mul_split(const MulItem &it, int force_lg) {
     const bool sq = it.b == nullptr;
     const uint64_t bits_b = sq ? it.bits_a : it.bits_b;
     unsigned bw;
     int lgp;
-    ff_split_plan(it.bits_a, bits_b, sq, &bw, &lgp);
+    ff_split_plan(it.bits_a, bits_b, sq, &bw, &lgp, force_lg);
     const uint64_t Lp = 1ull << lgp, H = Lp / 2;
     const uint64_t ca = (it.bits_a + bw - 1) / bw, cb = (bits_b + bw - 1) / bw;
     const int ka = (int)((ca + H - 1) / H), kb = sq ? ka : (int)((cb + H - 1) / H);

@@ -28,16 +28,24 @@
 
 namespace {
 
-constexpr int K_CAND = 16;
+// Candidates 0..15 are single-transform schedules; 16..20 serve products
+// past the longest transform (the split multiply of mul_split.c): piece
+// transforms of 2^(max_lg - 4) .. 2^max_lg words, the range the
+// reference's cost model chooses from (mul_split.c:160-170).
+constexpr int K_SINGLE = 16, K_SPLIT = 5, K_CAND = K_SINGLE + K_SPLIT;
 constexpr int kInner[2] = {10, 12};
 constexpr int kTile[4] = {2048, 4096, 8192, 16384};
 constexpr int K_DEFAULT = 5;  // the library default: shortest length (the reference rule), inner 2^12, tile 4096
 
-FFSched cand_sched(int k) { return FFSched{k / 8, kInner[(k / 4) % 2], kTile[k % 4]}; }
+FFSched cand_sched(int k) {
+    if (k >= K_SINGLE) return FFSched{0, 0, 0, g_max_lg - 4 + (k - K_SINGLE)};
+    return FFSched{k / 8, kInner[(k / 4) % 2], kTile[k % 4], 0};
+}
 
-// size classes of the table: lg of the shortest length (4..25), radix-3,
-// square, floor(log2(batch size)) clamped to 0..12
-constexpr int T_LG = 22, T_CNT = 13, T_SIZE = T_LG * 2 * 2 * T_CNT;
+// size classes of the table: lg of the shortest length (4..28; above 25
+// the product splits), radix-3, square, floor(log2(batch size)) clamped to
+// 0..12
+constexpr int T_LG = 25, T_CNT = 13, T_SIZE = T_LG * 2 * 2 * T_CNT;
 std::vector<int32_t> g_table;  // empty: not loaded
 bool g_auto_tried = false;
 
@@ -48,7 +56,7 @@ int lg_count(size_t count) {
 }
 
 int table_class(const ProdShape &s, bool sq, size_t count) {
-    const int lg = s.lg < 4 ? 4 : (s.lg > 25 ? 25 : s.lg);
+    const int lg = s.lg < 4 ? 4 : (s.lg > 28 ? 28 : s.lg);
     return (((lg - 4) * 2 + s.r3) * 2 + (sq ? 1 : 0)) * T_CNT + lg_count(count);
 }
 
@@ -79,15 +87,33 @@ void features(uint64_t bits_a, uint64_t bits_b, bool sq, size_t count, const Pro
     x[7] = (float)((int)s.bw - 32);
 }
 
-// Candidates that apply to a shape: a longer length must stay supported.
+// Candidates that apply to a shape: a longer length must stay supported;
+// split products take the piece-length candidates (pieces of at least 2^10
+// words), the others the single-transform ones.
 uint32_t cand_mask(const ProdShape &s) {
     uint32_t m = 0;
-    for (int k = 0; k < K_CAND; k++) {
+    if (s.split) {
+        for (int k = K_SINGLE; k < K_CAND; k++)
+            if (cand_sched(k).split_lg >= 10) m |= 1u << k;
+        return m;
+    }
+    for (int k = 0; k < K_SINGLE; k++) {
         const FFSched sc = cand_sched(k);
         if (sc.len_up && ff_shape_with(s, sc).L == s.L) continue;
         m |= 1u << k;
     }
     return m;
+}
+
+// The reference rule's candidate for a shape: K_DEFAULT, or for a split
+// product the piece length of the reference's cost model.
+int rule_candidate(uint64_t bits_a, uint64_t bits_b, bool sq, const ProdShape &s) {
+    if (!s.split) return K_DEFAULT;
+    unsigned bw;
+    int lgp;
+    ff_split_plan(bits_a, bits_b, sq, &bw, &lgp);
+    const int k = K_SINGLE + lgp - (g_max_lg - 4);
+    return k >= K_SINGLE && k < K_CAND ? k : K_CAND - 1;
 }
 
 // The dispatcher file: "FFDISP1\0", int32 K, int32 table entries, the
@@ -166,7 +192,7 @@ void ff_pick_scheds(const ProdShape *s0, const uint64_t *bits_a, const uint64_t 
         uint32_t *dm = (uint32_t *)(dx + n * FF_MLP_IN);
         int *dc = (int *)(dm + n);
         FF_CHECK(hipMemcpyAsync(dx, hx, (n * FF_MLP_IN + n) * 4, hipMemcpyHostToDevice, g_st));
-        std::vector<int> choice(n, K_DEFAULT);
+        std::vector<int> choice(n, -1);
         if (ff_mlp_infer_dev(dx, n, dm, nullptr, dc, FF_MLP_MARGIN, K_DEFAULT) == FF_OK) {
             int *hc = (int *)(hm + n);
             FF_CHECK(hipMemcpyAsync(hc, dc, n * 4, hipMemcpyDeviceToHost, g_st));
@@ -174,13 +200,19 @@ void ff_pick_scheds(const ProdShape *s0, const uint64_t *bits_a, const uint64_t 
             memcpy(choice.data(), hc, n * 4);
         }
         ff_dfree(buf);
-        for (size_t i = 0; i < n; i++) out[i] = cand_sched(choice[i] >= 0 ? choice[i] : K_DEFAULT);
+        for (size_t i = 0; i < n; i++) {
+            const int k = choice[i] >= 0 && ((mask[i] >> choice[i]) & 1)
+                              ? choice[i]
+                              : rule_candidate(bits_a[i], bits_b[i], square[i], s0[i]);
+            out[i] = cand_sched(k);
+        }
         return;
     }
     if (g_table.size() == (size_t)T_SIZE) {
         for (size_t i = 0; i < n; i++) {
             int k = g_table[table_class(s0[i], square[i], count)];
-            if (k < 0 || k >= K_CAND || !((cand_mask(s0[i]) >> k) & 1)) k = K_DEFAULT;
+            if (k < 0 || k >= K_CAND || !((cand_mask(s0[i]) >> k) & 1))
+                k = rule_candidate(bits_a[i], bits_b[i], square[i], s0[i]);
             out[i] = cand_sched(k);
         }
     }
@@ -194,7 +226,7 @@ extern "C" int ff_tune_candidate(int k, int *len_up, int *inner_log, int *tile_w
     if (k < 0 || k >= K_CAND) return FF_ERR_ARG;
     const FFSched s = cand_sched(k);
     if (len_up) *len_up = s.len_up;
-    if (inner_log) *inner_log = s.inner_log;
+    if (inner_log) *inner_log = s.split_lg ? -s.split_lg : s.inner_log;
     if (tile_words) *tile_words = s.tile_words;
     return FF_OK;
 }
@@ -207,7 +239,6 @@ extern "C" int ff_tune_features(uint64_t bits_a, uint64_t bits_b, int square, si
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     ff_ensure_init();
     const ProdShape s = ff_product_shape(bits_a, square ? bits_a : bits_b);
-    if (s.split) return FF_ERR_UNSUPPORTED;
     features(bits_a, bits_b, square != 0, count, s, free_hbm_fraction(), x);
     if (mask) *mask = cand_mask(s);
     return FF_OK;
@@ -216,8 +247,14 @@ extern "C" int ff_tune_features(uint64_t bits_a, uint64_t bits_b, int square, si
 extern "C" int ff_tune_class(uint64_t bits_a, uint64_t bits_b, int square, size_t count) {
     if (bits_a == 0 || (!square && bits_b == 0)) return FF_ERR_ARG;
     const ProdShape s = ff_product_shape(bits_a, square ? bits_a : bits_b);
-    if (s.split) return FF_ERR_UNSUPPORTED;
     return table_class(s, square != 0, count);
+}
+
+extern "C" int ff_tune_rule_candidate(uint64_t bits_a, uint64_t bits_b, int square) {
+    if (bits_a == 0 || (!square && bits_b == 0)) return FF_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    const uint64_t bb = square ? bits_a : bits_b;
+    return rule_candidate(bits_a, bb, square != 0, ff_product_shape(bits_a, bb));
 }
 
 extern "C" int ff_tune_table_size(void) { return T_SIZE; }
@@ -261,7 +298,7 @@ extern "C" int ff_tune_measure(uint64_t bits_a, uint64_t bits_b, int square, siz
     ff_ensure_init();
     const uint64_t bb = square ? bits_a : bits_b;
     const ProdShape s0 = ff_product_shape(bits_a, bb);
-    if (s0.split) return FF_ERR_UNSUPPORTED;
+    if (s0.split) copies = 1;  // one product past the longest transform fills the GPU
     const uint64_t na = (bits_a + 63) / 64, nb = (bb + 63) / 64;
     std::vector<void *> mem;
     std::vector<ff_prod> items(copies);
@@ -290,6 +327,23 @@ extern "C" int ff_tune_measure(uint64_t bits_a, uint64_t bits_b, int square, siz
     for (int k = 0; k < K_CAND && rc == FF_OK; k++) {
         if (!((mask >> k) & 1)) {
             us[k] = NAN;
+            continue;
+        }
+        if (s0.split) {
+            // split products run alone (batches hold single transforms only):
+            // one warm run, then reps timed runs of the forced piece length
+            unsigned long long *key = (unsigned long long *)ff_dalloc(8);
+            MulItem it{items[0].a, na, bits_a, square ? nullptr : items[0].b, nb, bb, items[0].out, s0.conv, key};
+            const int lg = cand_sched(k).split_lg;
+            ff_mul_split(it, lg);
+            FF_CHECK(hipEventRecord(e0, g_st));
+            for (int r = 0; r < reps; r++) ff_mul_split(it, lg);
+            FF_CHECK(hipEventRecord(e1, g_st));
+            FF_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            FF_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            us[k] = ms * 1000.0f / (float)reps;
+            ff_dfree(key);
             continue;
         }
         for (auto &x : sc) x = cand_sched(k);

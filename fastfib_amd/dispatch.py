@@ -41,6 +41,7 @@ def _sigs(L) -> None:
     L.ff_tune_measure.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
                                   ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]
     L.ff_tune_class.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t]
+    L.ff_tune_rule_candidate.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
     L.ff_tune_set_table.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t]
     L.ff_dispatch_load.argtypes = [ctypes.c_char_p]
     L.ff_mlp_param_count.argtypes = [ctypes.c_int]
@@ -77,20 +78,33 @@ def default_candidate() -> int:
 def candidate(k: int) -> dict:
     a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _check(_L().ff_tune_candidate(k, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+    if b.value < 0:
+        return {"split_piece_lg": -b.value}
     return {"len_up": a.value, "inner_log": b.value, "tile_words": c.value}
+
+
+def rule_candidates(d: dict) -> np.ndarray:
+    """Per sample, the candidate the reference rule amounts to (the default
+    schedule, or the split cost model's piece length)."""
+    L = _L()
+    return np.array([L.ff_tune_rule_candidate(int(a), int(b), int(q))
+                     for a, b, q in zip(d["bits_a"], d["bits_b"], d["square"])], dtype=np.int64)
 
 
 # ------------------------------------------------------------------ sampling
 
-def draw_samples(seed: int, mixes: int, per_mix: int) -> list[tuple[int, int, int, int, int]]:
+def draw_samples(seed: int, mixes: int, per_mix: int, lo_lg: float = 6.0, hi_lg: float = 29.0,
+                 mix0: int = 0) -> list[tuple[int, int, int, int, int]]:
     """[(bits_a, bits_b, square, count, mix)]: per mix a batch size
-    floor(2^(12 u)), an operand range 2^lo .. 2^hi inside 2^6 .. 2^29 and a
-    square fraction, then per_mix products drawn from it."""
+    floor(2^(12 u)), an operand range 2^lo .. 2^hi inside 2^lo_lg .. 2^hi_lg
+    (by default 2^6 .. 2^29; 2^29 .. 2^31.5 reaches the split multiply) and
+    a square fraction, then per_mix products drawn from it; mixes are
+    numbered from mix0."""
     rng = np.random.default_rng(seed)
     out = []
-    for m in range(mixes):
+    for m in range(mix0, mix0 + mixes):
         count = int(np.floor(2.0 ** (12.0 * rng.random())))
-        lo, hi = sorted(6.0 + 23.0 * rng.random(2))
+        lo, hi = sorted(lo_lg + (hi_lg - lo_lg) * rng.random(2))
         sqf = rng.random()
         for _ in range(per_mix):
             ba = int(np.floor(2.0 ** (lo + (hi - lo) * rng.random())))
@@ -244,7 +258,7 @@ def table_choice(table: np.ndarray, d: dict) -> np.ndarray:
     """The table's candidate per sample, the reference rule where it does not apply."""
     tc = table[d["cls"]]
     ok = ((d["mask"].astype(np.int64) >> tc) & 1) == 1
-    return np.where(ok, tc, default_candidate())
+    return np.where(ok, tc, rule_candidates(d))
 
 
 def regret(T_us: np.ndarray, choice: np.ndarray, mix: np.ndarray) -> dict:
@@ -259,6 +273,30 @@ def regret(T_us: np.ndarray, choice: np.ndarray, mix: np.ndarray) -> dict:
     per = np.array(per)
     return {"mean": float(per.mean()), "median": float(np.median(per)), "p95": float(np.quantile(per, 0.95)),
             "aggregate": float(got.sum() / best.sum() - 1.0), "mixes": int(per.size)}
+
+
+def ho_split(d: dict) -> np.ndarray:
+    """Samples whose products take the split multiply (piece candidates only)."""
+    return (d["mask"].astype(np.int64) >> 16) != 0
+
+
+def build_stamp() -> dict:
+    """The library the samples were measured with: its SHA-256 and the
+    sources' newest commit."""
+    import hashlib
+    import subprocess
+    from . import LIB_PATH
+    with open(LIB_PATH, "rb") as f:
+        h = hashlib.sha256(f.read()).hexdigest()[:16]
+    c = os.environ.get("FF_BUILD_COMMIT", "")
+    if not c:
+        try:
+            c = subprocess.run(["git", "log", "-1", "--format=%h", "--", "fastfib_amd/csrc"], capture_output=True,
+                               text=True, cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__)))).stdout
+            c = c.strip()
+        except OSError:
+            c = ""
+    return {"library_sha256_16": h, "csrc_commit": c or None}
 
 
 def write_bin(path: str, K: int, table: np.ndarray, params: np.ndarray) -> None:
@@ -301,6 +339,10 @@ def main(argv=None) -> int:
     c.add_argument("--per-mix", type=int, default=20)
     c.add_argument("--seed", type=int, default=1)
     c.add_argument("--reps", type=int, default=2)
+    c.add_argument("--lo-lg", type=float, default=6.0)
+    c.add_argument("--hi-lg", type=float, default=29.0)
+    c.add_argument("--mix0", type=int, default=0)
+    c.add_argument("--append", default=None, help="samples file to extend")
     c.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--train", required=True)
@@ -315,9 +357,12 @@ def main(argv=None) -> int:
     from . import init
     init()
     if a.cmd == "collect":
-        s = draw_samples(a.seed, a.mixes, a.per_mix)
+        s = draw_samples(a.seed, a.mixes, a.per_mix, a.lo_lg, a.hi_lg, a.mix0)
         print(f"collect: {len(s)} samples from {a.mixes} mixes (seed {a.seed})", flush=True)
         d = collect(s, a.reps)
+        if a.append:
+            old = _load(a.append)
+            d = {k: np.concatenate([old[k], d[k]]) for k in d}
         _save(a.out, d)
         print(f"wrote {a.out}", flush=True)
         return 0
@@ -338,8 +383,9 @@ def main(argv=None) -> int:
            "train_seconds": round(dt, 2), "final_loss": loss,
            "regret_learned": regret(ho["T_us"], choice, ho["mix"]),
            "regret_table": regret(ho["T_us"], table_choice(table, ho), ho["mix"]),
-           "regret_reference_rule": regret(ho["T_us"], np.full(ho["X"].shape[0], default_candidate()),
-                                           ho["mix"])}
+           "regret_reference_rule": regret(ho["T_us"], rule_candidates(ho), ho["mix"]),
+           "candidates": K, "split_samples": {"train": int(np.sum(ho_split(tr))), "heldout": int(np.sum(ho_split(ho)))},
+           "build": build_stamp()}
     write_bin(a.out, K, table, params)
     print(json.dumps(rep), flush=True)
     if a.report:

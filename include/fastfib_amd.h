@@ -233,11 +233,15 @@ FF_API void ff_batch_destroy(ff_batch *b);
 /* ------------------------------------------------------------------------
  * Schedule autotuner and learned dispatcher (no counterpart in the
  * reference, which always takes the shortest length and fixed block sizes:
- * mul_ntt.c:103-118, ntt_conv.c:34-62).
+ * mul_ntt.c:103-118, ntt_conv.c:34-62, and its cost model's piece length
+ * for split products, mul_split.c:160-170).
  *
- * Candidate k (0 .. ff_tune_candidates()-1) is a schedule: the transform
- * length (the shortest, or the next supported one up), the LDS-resident
- * inner block (log2) and the outer-pass tile (LDS words). A batch created
+ * Candidate k (0 .. ff_tune_candidates()-1) is a schedule: for k < 16 the
+ * transform length (the shortest, or the next supported one up), the
+ * LDS-resident inner block (log2) and the outer-pass tile (LDS words); for
+ * k >= 16, products past the longest transform only, the split multiply's
+ * piece transform 2^(max_lg - 4 + k - 16) (ff_tune_candidate reports it as
+ * a negative inner_log: -lg). A batch created
  * with FF_BATCH_TUNED takes each product's candidate from the autotuner
  * table, with FF_BATCH_LEARNED from the MLP; when neither was set, both are
  * read from dispatch.bin beside the library on first use. Without them a
@@ -265,6 +269,10 @@ FF_API int ff_tune_measure(uint64_t bits_a, uint64_t bits_b, int square, size_t 
 /* The autotuner table: one candidate per size class; ff_tune_class gives a
    product's class (index into the table of ff_tune_table_size() entries). */
 FF_API int ff_tune_table_size(void);
+/* The candidate the reference rule amounts to for a product: the default
+   single-transform schedule, or for a product past the longest transform
+   the piece length of the reference's cost model (mul_split.c:160-170). */
+FF_API int ff_tune_rule_candidate(uint64_t bits_a, uint64_t bits_b, int square);
 FF_API int ff_tune_class(uint64_t bits_a, uint64_t bits_b, int square, size_t count);
 FF_API int ff_tune_set_table(const int32_t *table, size_t n);
 

@@ -288,10 +288,12 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bo
     return leg
 
 
-def dispatcher_regret() -> dict:
-    """Regret on the committed held-out autotuner samples (1000 mixes) of the
-    learned dispatcher (inference by the library's HIP kernel), the
-    autotuner table and the reference rule, against the exhaustive best."""
+def dispatcher_regret(live_mixes: int = 40) -> dict:
+    """Regret of the learned dispatcher (inference by the library's HIP
+    kernel), the autotuner table and the reference rule against the
+    exhaustive best: on the committed held-out samples (timed when the
+    dispatcher was trained, on the build stamped in its report), and on a
+    subset of those mixes timed again now, on this build and GPU."""
     import numpy as np
     from fastfib_amd import dispatch as dp
 
@@ -301,12 +303,32 @@ def dispatcher_regret() -> dict:
     K, table, params = dp.read_bin(os.path.join(here, "dispatch.bin"))
     dp.set_params(K, params)
     y, _ = dp.predict(ho["X"], ho["mask"])
-    choice = dp.dispatch_choice(y, ho["mask"], dp.default_candidate())
-    r = {"learned": dp.regret(ho["T_us"], choice, ho["mix"]),
-         "table": dp.regret(ho["T_us"], dp.table_choice(table, ho), ho["mix"]),
-         "reference_rule": dp.regret(ho["T_us"], np.full(ho["X"].shape[0], dp.default_candidate()), ho["mix"])}
-    return {k: {"mean": round(v["mean"], 4), "aggregate": round(v["aggregate"], 4), "p95": round(v["p95"], 4)}
-            for k, v in r.items()} | {"mixes": int(np.unique(ho["mix"]).size)}
+    rule = dp.rule_candidates(ho)
+    choice = dp.dispatch_choice(y, ho["mask"], dp.default_candidate(), rule=rule)
+    tab = dp.table_choice(table, ho)
+
+    def summary(T, sel):
+        r = {"learned": dp.regret(T, choice[sel], ho["mix"][sel]),
+             "table": dp.regret(T, tab[sel], ho["mix"][sel]),
+             "reference_rule": dp.regret(T, rule[sel], ho["mix"][sel])}
+        return {k: {"mean": round(v["mean"], 4), "aggregate": round(v["aggregate"], 4), "p95": round(v["p95"], 4)}
+                for k, v in r.items()} | {"mixes": int(np.unique(ho["mix"][sel]).size)}
+
+    allrows = np.ones(ho["X"].shape[0], dtype=bool)
+    out = {"committed_timings": summary(ho["T_us"], allrows)}
+    mixes = np.unique(ho["mix"])
+    pick = set(mixes[:: max(1, len(mixes) // live_mixes)].tolist())
+    sel = np.array([m in pick for m in ho["mix"]])
+    rows = np.flatnonzero(sel)
+    T = np.array([dp.measure(int(ho["bits_a"][i]), int(ho["bits_b"][i]), int(ho["square"][i]),
+                             int(ho["count"][i]), seed=7 * int(i) + 1) for i in rows])
+    out["retimed_now"] = summary(T, sel)
+    try:
+        with open(os.path.join(here, "dispatch_data", "report.json")) as f:
+            out["trained_on_build"] = json.load(f).get("build")
+    except OSError:
+        pass
+    return out
 
 
 DEC_PRIMES = (2147483647, 2147483629, 2147483587, 1000000007)
@@ -595,6 +617,8 @@ def run_legs_only(ff, args, legs) -> int:
     if "batch" in legs:
         out["batch"] = batch_leg(ff, args.batch_count, args.batch_seed, args.batch_steps, 0.0, cpu=False)
         ok &= out["batch"]["bit_exact"]
+    if "regret" in legs:
+        out["dispatcher_regret"] = dispatcher_regret()
     if "product" in legs:
         out["product_2p26"] = product_leg(ff, args.product_steps)
         ok &= out["product_2p26"]["bit_exact"]

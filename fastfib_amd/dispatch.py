@@ -242,16 +242,24 @@ def build_table(d: dict, size: int, K: int, default: int) -> np.ndarray:
 MARGIN = 0.02  # the library's FF_MLP_MARGIN
 
 
-def dispatch_choice(y: np.ndarray, mask: np.ndarray, default: int, margin: float = MARGIN) -> np.ndarray:
+def dispatch_choice(y: np.ndarray, mask: np.ndarray, default: int, margin: float = MARGIN,
+                    rule: np.ndarray | None = None) -> np.ndarray:
     """The dispatcher's pick from the network's outputs: the argmin over the
     applicable candidates, unless the default is predicted within margin of
-    it (the rule of the library's inference kernel)."""
+    it (the rule of the library's inference kernel); products past the
+    longest transform take the reference rule's candidate (rule), as the
+    library does."""
     K = y.shape[1]
-    allowed = ((np.asarray(mask, dtype=np.int64)[:, None] >> np.arange(K)) & 1) == 1
+    m = np.asarray(mask, dtype=np.int64)
+    allowed = ((m[:, None] >> np.arange(K)) & 1) == 1
     yy = np.where(allowed, y, np.inf)
     best = np.argmin(yy, axis=1)
-    gain = yy[np.arange(len(yy)), default] - yy[np.arange(len(yy)), best]
-    return np.where(allowed[:, default] & (gain <= margin), default, best)
+    ok_def = allowed[:, default]
+    gain = np.where(ok_def, yy[np.arange(len(yy)), default] - yy[np.arange(len(yy)), best], np.inf)
+    pick = np.where(ok_def & (gain <= margin), default, best)
+    if rule is not None:
+        pick = np.where((m >> 16) != 0, rule, pick)
+    return pick
 
 
 def table_choice(table: np.ndarray, d: dict) -> np.ndarray:
@@ -369,22 +377,30 @@ def main(argv=None) -> int:
     tr, ho = _load(a.train), _load(a.heldout)
     K = candidates()
     Ytr = targets(tr["T_us"])
+    # products past the longest transform are few among the samples (they
+    # are drawn from a range of their own) but cost the most: repeated so
+    # that they make up about a twentieth of the training rows
+    sp = ho_split(tr)
+    rep = max(1, int(round(0.05 * np.sum(~sp) / max(1, np.sum(sp)))))
+    Xt = np.concatenate([tr["X"]] + [tr["X"][sp]] * (rep - 1))
+    Yt = np.concatenate([Ytr] + [Ytr[sp]] * (rep - 1))
     t0 = time.time()
-    set_params(K, init_params(K, tr["X"], a.seed))
-    loss = train(tr["X"], Ytr, a.epochs, a.batch, a.lr, a.seed)
+    set_params(K, init_params(K, Xt, a.seed))
+    loss = train(Xt, Yt, a.epochs, a.batch, a.lr, a.seed)
     dt = time.time() - t0
     K, params = get_params()
     size = int(_L().ff_tune_table_size())
     table = build_table(tr, size, K, default_candidate())
     y, _ = predict(ho["X"], ho["mask"])
-    choice = dispatch_choice(y, ho["mask"], default_candidate())
+    choice = dispatch_choice(y, ho["mask"], default_candidate(), rule=rule_candidates(ho))
     rep = {"train_samples": int(tr["X"].shape[0]), "heldout_samples": int(ho["X"].shape[0]),
            "train_mixes": int(np.unique(tr["mix"]).size), "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
            "train_seconds": round(dt, 2), "final_loss": loss,
            "regret_learned": regret(ho["T_us"], choice, ho["mix"]),
            "regret_table": regret(ho["T_us"], table_choice(table, ho), ho["mix"]),
            "regret_reference_rule": regret(ho["T_us"], rule_candidates(ho), ho["mix"]),
-           "candidates": K, "split_samples": {"train": int(np.sum(ho_split(tr))), "heldout": int(np.sum(ho_split(ho)))},
+           "candidates": K, "split_samples": {"train": int(np.sum(sp)), "heldout": int(np.sum(ho_split(ho))),
+                                              "train_repeats": rep},
            "build": build_stamp()}
     write_bin(a.out, K, table, params)
     print(json.dumps(rep), flush=True)

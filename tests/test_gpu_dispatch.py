@@ -169,9 +169,12 @@ def test_fibonacci_with_dispatcher(ff, mode):
         ff.set_option("dispatch", 0)
 
 
-def test_learned_batch_gain(ff):
-    """Batched with the committed learned dispatcher against one product per
-    launch sequence on an uneven batch: at least 1.5x, results identical."""
+def test_batched_learned_vs_plain(ff):
+    """The batched form with the committed learned dispatcher against one
+    product per launch sequence (the reference's calling pattern) on an
+    uneven batch: at least 1.5x, results identical. This is the gain of
+    batching and dispatch together; test_network_gain_over_rule isolates the
+    network's."""
     import os
     import time
 
@@ -198,3 +201,64 @@ def test_learned_batch_gain(ff):
         bt.close()
     assert all(np.array_equal(x, y) for x, y in zip(outs["plain"], outs["learned"]))
     assert times["plain"] / times["learned"] >= 1.5, times
+
+
+def test_network_gain_over_rule(ff):
+    """What the trained network adds over the method without it: the
+    committed held-out mix on which the committed timings show the largest
+    gain of the network's picks over the reference rule (shortest length,
+    default blocks), re-run now in the same graph form both ways. Each of
+    the mix's products runs as a batch of as many copies as its sample was
+    timed with; the network's schedules must take at least 5% less time."""
+    import os
+    import time
+
+    from oracle import mlp as om
+
+    here = os.path.dirname(dp.__file__)
+    dp.load(os.path.join(here, "dispatch.bin"))
+    K, table, params = dp.read_bin(os.path.join(here, "dispatch.bin"))
+    with np.load(os.path.join(here, "dispatch_data", "heldout.npz")) as z:
+        ho = {k: z[k] for k in z.files}
+    y = om.forward(params, ho["X"], K)[3]
+    rule = dp.rule_candidates(ho)
+    choice = dp.dispatch_choice(y, ho["mask"], dp.default_candidate(), rule=rule)
+    T = np.where(np.isfinite(ho["T_us"]), ho["T_us"], np.inf)
+    got, ref = T[np.arange(len(T)), choice], T[np.arange(len(T)), rule]
+    best_mix, best_gain = None, 0.0
+    for m in np.unique(ho["mix"]):
+        sel = ho["mix"] == m
+        g = ref[sel].sum() / got[sel].sum()
+        if g > best_gain:
+            best_mix, best_gain = m, g
+    sel = np.flatnonzero(ho["mix"] == best_mix)
+    assert best_gain >= 1.10, best_gain  # the committed data: the rule's regret there is at least 10%
+    rng = np.random.default_rng(int(best_mix))
+    batches = {}
+    for name, flags in (("rule", ff.BATCH_GRAPH), ("learned", ff.BATCH_GRAPH | ff.BATCH_LEARNED)):
+        batches[name] = []
+        for i in sel:
+            ba, bb, sq, cnt = (int(ho[k][i]) for k in ("bits_a", "bits_b", "square", "count"))
+            pairs = []
+            for _ in range(dp.copies_for(ba, bb, cnt)):
+                A = ff.Device(operand_limbs(rng, ba))
+                pairs.append((A, None if sq else ff.Device(operand_limbs(rng, bb))))
+            bt = ff.Batch(pairs, flags)
+            bt.run()
+            bt.finish()
+            batches[name].append(bt)
+    times = {name: [] for name in batches}
+    for _ in range(7):  # interleaved trials, best of each
+        for name, bts in batches.items():
+            ff.sync()
+            t0 = time.perf_counter()
+            for _ in range(20):
+                for bt in bts:
+                    bt.run()
+            ff.sync()
+            times[name].append(time.perf_counter() - t0)
+    for bts in batches.values():
+        for bt in bts:
+            bt.close()
+    gain = min(times["rule"]) / min(times["learned"])
+    assert gain >= 1.05, (int(best_mix), best_gain, gain)

@@ -223,7 +223,9 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bo
     """The uneven batch: count products, operand bits log-uniform in
     [2^12, 2^24], 30% squares, uniform limbs (fastfib_amd.workloads), one
     step = the whole batch. Run forms: one product at a time (plain), grouped
-    work lists, the grouped launches replayed from a HIP graph. Every form's
+    work lists, the grouped launches replayed from a HIP graph, the
+    persistent work queue (one launch per run; alone, captured, and under
+    the unbounded and 3-waves-per-SIMD register budgets). Every form's
     outputs are checked: the first form's per product modulo two primes, the
     others byte-equal to it."""
     import numpy as np
@@ -242,8 +244,11 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bo
         pairs.append((ff.Device(A), None if B is None else ff.Device(B)))
         host.append((A, B))
     ff.sync()
-    forms = [("plain", ff.BATCH_PLAIN), ("grouped", ff.BATCH_GROUPED), ("graph", ff.BATCH_GRAPH),
-             ("graph_tuned", ff.BATCH_GRAPH | ff.BATCH_TUNED), ("graph_learned", ff.BATCH_GRAPH | ff.BATCH_LEARNED)]
+    forms = [("plain", ff.BATCH_PLAIN, 1), ("grouped", ff.BATCH_GROUPED, 1), ("graph", ff.BATCH_GRAPH, 1),
+             ("graph_tuned", ff.BATCH_GRAPH | ff.BATCH_TUNED, 1),
+             ("graph_learned", ff.BATCH_GRAPH | ff.BATCH_LEARNED, 1),
+             ("queue", ff.BATCH_QUEUE, 4), ("graph_queue", ff.BATCH_GRAPH | ff.BATCH_QUEUE, 4),
+             ("queue_w1", ff.BATCH_QUEUE, 1), ("queue_w3", ff.BATCH_QUEUE, 3)]
     # the reference's own big_mul / big_sqr over the same products
     want = None
     try:
@@ -254,7 +259,8 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bo
     except Exception as e:
         print(f"reference products unavailable, residues only: {e}", file=sys.stderr)
     out, ref_limbs, ok = {}, None, True
-    for name, flags in forms:
+    for name, flags, waves in forms:
+        ff.set_option("queue_waves", waves)
         bt = ff.Batch(pairs, flags)
         bt.run()
         bt.finish()
@@ -278,6 +284,7 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bo
         out[name] = {"ms": round(dt * 1e3, 3), "gbit_products_per_s": round(gbit / dt, 2), "bit_exact": good,
                      "steps": n_steps}
         bt.close()
+    ff.set_option("queue_waves", 4)
     plain = out["plain"]["gbit_products_per_s"]
     leg = {"workload": f"{count} uneven products, operands 2^12..2^24 bits log-uniform, 30% squares, seed {seed}",
            "operand_gbit": round(gbit, 3), "forms": out, "bit_exact": ok,
@@ -383,6 +390,20 @@ def decimal_leg(ff, n: int, steps: int, cpu_seconds: float, cpu: bool = True) ->
     for _ in range(steps):
         ff._check(L.ff_dbig_to_decimal(ctypes.byref(x.h), ctypes.byref(p), ctypes.byref(ln)))
     warm = (time.perf_counter() - t0) / steps
+    # the level batches in each form: grouped launches, persistent work
+    # queues (one launch per batch)
+    default = ff.get_option("dec_queue")
+    forms = {}
+    for name, q in (("grouped", 0), ("queue", 1)):
+        ff.set_option("dec_queue", q)
+        for _ in range(2):  # plan built, then captured
+            ff._check(L.ff_dbig_to_decimal(ctypes.byref(x.h), ctypes.byref(p), ctypes.byref(ln)))
+        ff.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ff._check(L.ff_dbig_to_decimal(ctypes.byref(x.h), ctypes.byref(p), ctypes.byref(ln)))
+        forms[name] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+    ff.set_option("dec_queue", default)
     t0 = time.perf_counter()
     s = x.to_string()
     host = time.perf_counter() - t0
@@ -393,6 +414,8 @@ def decimal_leg(ff, n: int, steps: int, cpu_seconds: float, cpu: bool = True) ->
                         "graph built, as the reference's big_to_string does on every call; the figure "
                         "compared with the CPU baseline",
            "warm_device_ms_cached": round(warm * 1e3, 3),
+           "warm_device_ms_cached_by_level_form": forms,
+           "level_form_default": "queue" if default else "grouped",
            "warm_to_host_string_ms_cached": round(host * 1e3, 3),
            "warm_note": "powers/reciprocals cached from the first conversion; digits left in HBM, or copied "
                         "to a host string"}

@@ -66,6 +66,7 @@ def lib() -> ctypes.CDLL:
     L.ff_dmul.argtypes = [ctypes.POINTER(DBig), ctypes.POINTER(DBig), ctypes.POINTER(DBig)]
     L.ff_dfibonacci.argtypes = [ctypes.c_uint64, ctypes.POINTER(DBig)]
     L.ff_set_option.argtypes = [ctypes.c_char_p, ctypes.c_longlong]
+    L.ff_get_option.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong)]
     L.ff_fib_step_linear.argtypes = [ctypes.POINTER(Big), ctypes.POINTER(Big), ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(Big), ctypes.POINTER(Big)]
     L.ff_last_factors.argtypes = [ctypes.POINTER(Big), ctypes.POINTER(Big), ctypes.POINTER(Big),
@@ -115,6 +116,12 @@ def sync() -> None:
 
 def set_option(name: str, value: int) -> None:
     _check(lib().ff_set_option(name.encode(), int(value)))
+
+
+def get_option(name: str) -> int:
+    v = ctypes.c_longlong()
+    _check(lib().ff_get_option(name.encode(), ctypes.byref(v)))
+    return v.value
 
 
 def big_mul(a, b) -> np.ndarray:
@@ -300,7 +307,7 @@ class Prod(ctypes.Structure):
                 ("bits_b", ctypes.c_uint64)]
 
 
-BATCH_PLAIN, BATCH_GROUPED, BATCH_GRAPH, BATCH_TUNED, BATCH_LEARNED = 0, 1, 2, 4, 8
+BATCH_PLAIN, BATCH_GROUPED, BATCH_GRAPH, BATCH_TUNED, BATCH_LEARNED, BATCH_QUEUE = 0, 1, 2, 4, 8, 16
 
 
 def _batch_sigs(L) -> None:

@@ -13,9 +13,42 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define FF_P1 2113929217u  // 63 * 2^25 + 1, generator 5
 #define FF_P2 2013265921u  // 15 * 2^27 + 1, generator 31
 #define FF_P3 1811939329u  // 27 * 2^26 + 1, generator 13
+
+// Device pointers known to address global memory. Accesses through them
+// compile to global_* instructions (an SGPR base plus a 32-bit lane offset
+// where the base is uniform) instead of flat ones, which need a 64-bit
+// address per lane and also count against the LDS wait counter, so that an
+// LDS wait would wait for them too. Pointers loaded from a work item in
+// memory are otherwise flat: the compiler cannot tell their address space.
+template <class T>
+using ff_gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ ff_gptr<T> ff_g(T *p) {
+    return (ff_gptr<T>)p;
+}
+
+// Loads and stores of data that other workgroups of the same launch write
+// or read (the batch work queue, k_queue): with COH, agent-scope relaxed
+// accesses, which the L2 of one XCD does not hold stale or dirty (the XCDs'
+// L2s are not coherent with each other), so only the order of those
+// accesses needs care, not a write-back or invalidation of whole caches.
+template <bool COH, class P>
+__device__ __forceinline__ auto ff_ld(P p) {
+    typedef typename std::remove_cv<typename std::remove_reference<decltype(*p)>::type>::type T;
+    if (COH) return __hip_atomic_load((const T *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (T)*p;
+}
+template <bool COH, class P, class T>
+__device__ __forceinline__ void ff_st(P p, T v) {
+    typedef typename std::remove_reference<decltype(*p)>::type U;
+    if (COH) __hip_atomic_store((U *)p, (U)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = (U)v;
+}
 
 // Every supported length divides FF_NBASE = 3 * 2^25, so w_L = w_NBASE^(NBASE/L).
 #define FF_NBASE (3u << 25)

@@ -59,6 +59,12 @@ struct ff_batch {
     std::vector<MulItem> plain;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // the work-queue form
+    const FFQTask *qtasks = nullptr;  // in desc
+    uint32_t ntasks = 0, nctr = 0;
+    uint32_t *ctr = nullptr;
+    size_t qlds = 0;
+    double qbytes = 0;
 };
 
 // Bit lengths of many device numbers: thread k scans number k down from its
@@ -112,6 +118,9 @@ static void build_lists(ff_batch *B) {
     std::vector<const FFPlan *> plan(n, nullptr);
     std::vector<uint32_t *> fa(n), fb(n);
     std::vector<char> own_b(n, 0);  // the item transforms its b (else it reads an earlier item's)
+    std::vector<int> b_owner(n, -1);  // the item whose b transform this one reads
+    std::vector<int> pre_of(n, -1);   // the shared one-block spectrum this one reads
+    std::map<const uint32_t *, int> owner_of;
     // products sharing their b operand (same pointer, limbs, width and
     // schedule) share one transform of it
     std::map<std::tuple<const uint64_t *, uint64_t, uint64_t, unsigned, const FFPlan *>, uint32_t *> shared_b;
@@ -175,6 +184,8 @@ static void build_lists(ff_batch *B) {
         fa[i] = (uint32_t *)w;
         fb[i] = sq ? nullptr : (ownb ? (uint32_t *)(w + o_fb) : sb);
         own_b[i] = ownb;
+        if (ownb) owner_of[fb[i]] = (int)i;
+        else if (sb) b_owner[i] = owner_of[sb];
         if (ownb) shared_b[std::make_tuple(p.b, (uint64_t)p.nb, (uint64_t)p.bits_b, s.bw, plan[i])] = fb[i];
         jobs[i] = FFCrtJob{fa[i], p.out, ff_lb_take(ff_crt_lb_words(nout)), B->slots + i, nout, s.conv,
                            (uint32_t)s.L, (int)s.bw};
@@ -204,7 +215,7 @@ static void build_lists(ff_batch *B) {
             ii.c.bspec = 2;
             pre.push_back(ii);
             pre_plan.push_back(&pl);
-            for (size_t i : u.second) fb[i] = spec, bspec[i] = 1;
+            for (size_t i : u.second) fb[i] = spec, bspec[i] = 1, b_owner[i] = -1, pre_of[i] = (int)pre.size() - 1;
         }
     }
 
@@ -219,6 +230,9 @@ static void build_lists(ff_batch *B) {
     struct Pending {
         BLaunch L;
         size_t items_off, start_off;
+        size_t item_size;
+        std::vector<uint32_t> start;  // host copy: item k owns blocks [start[k], start[k + 1])
+        std::vector<int> who;         // each item's product (n + k: shared spectrum k; -1: none)
     };
     std::vector<Pending> pend;
 
@@ -242,6 +256,7 @@ static void build_lists(ff_batch *B) {
     auto outer_list = [&](int pass, bool inv, int r3sel, int oc) {
         std::vector<FFOuterItem> its;
         std::vector<uint32_t> start;
+        std::vector<int> who;
         uint32_t blocks = 0;
         size_t lds = 0;
         double bytes = 0;
@@ -266,6 +281,7 @@ static void build_lists(ff_batch *B) {
                 bytes += (!inv && pass == 0) ? 3.0 * 4.0 * pl.L : 3.0 * 8.0 * pl.L;
                 its.push_back(oi);
                 start.push_back(blocks);
+                who.push_back((int)i);
                 blocks += 3 * ps.nblocks * ps.tpb;
                 const size_t l = ff_outer_lds_bytes(&ps);
                 lds = l > lds ? l : lds;
@@ -278,6 +294,7 @@ static void build_lists(ff_batch *B) {
         pd.L.blocks = blocks, pd.L.lds = lds, pd.L.bytes = bytes;
         pd.items_off = put(its.data(), its.size() * sizeof(FFOuterItem));
         pd.start_off = put(start.data(), start.size() * 4);
+        pd.item_size = sizeof(FFOuterItem), pd.start = start, pd.who = who;
         pend.push_back(pd);
     };
     auto inner_occs = [&](int small, int r3sel) {
@@ -292,6 +309,7 @@ static void build_lists(ff_batch *B) {
     auto inner_list = [&](int small, int r3sel, int oc) {
         std::vector<FFInnerItem> its;
         std::vector<uint32_t> start;
+        std::vector<int> who;
         uint32_t blocks = 0;
         size_t lds = 0;
         double bytes = 0;
@@ -308,6 +326,7 @@ static void build_lists(ff_batch *B) {
             ii.c.bspec = bspec[i];
             its.push_back(ii);
             start.push_back(blocks);
+            who.push_back((int)i);
             blocks += 3 * (small ? 1 : pl.L / pl.inner.S);
             const size_t l = ff_inner_lds_bytes(&pl.inner);
             lds = l > lds ? l : lds;
@@ -320,6 +339,7 @@ static void build_lists(ff_batch *B) {
         pd.L.blocks = blocks, pd.L.lds = lds, pd.L.bytes = bytes;
         pd.items_off = put(its.data(), its.size() * sizeof(FFInnerItem));
         pd.start_off = put(start.data(), start.size() * 4);
+        pd.item_size = sizeof(FFInnerItem), pd.start = start, pd.who = who;
         pend.push_back(pd);
     };
 
@@ -331,6 +351,7 @@ static void build_lists(ff_batch *B) {
         for (int oc : occs) {
             std::vector<FFInnerItem> its;
             std::vector<uint32_t> start;
+            std::vector<int> who;
             uint32_t blocks = 0;
             size_t lds = 0;
             double bytes = 0;
@@ -339,6 +360,7 @@ static void build_lists(ff_batch *B) {
                 if (pl.inner.r3 != r3sel || occ(ff_inner_lds_bytes(&pl.inner)) != oc) continue;
                 its.push_back(pre[k]);
                 start.push_back(blocks);
+                who.push_back((int)(n + k));
                 blocks += 3;
                 const size_t l = ff_inner_lds_bytes(&pl.inner);
                 lds = l > lds ? l : lds;
@@ -349,6 +371,7 @@ static void build_lists(ff_batch *B) {
             pd.L.blocks = blocks, pd.L.lds = lds, pd.L.bytes = bytes;
             pd.items_off = put(its.data(), its.size() * sizeof(FFInnerItem));
             pd.start_off = put(start.data(), start.size() * 4);
+            pd.item_size = sizeof(FFInnerItem), pd.start = start, pd.who = who;
             pend.push_back(pd);
         }
     };
@@ -384,11 +407,13 @@ static void build_lists(ff_batch *B) {
     {
         std::vector<FFCrtJob> js;
         std::vector<uint32_t> start;
+        std::vector<int> who;
         uint32_t blocks = 0;
         double conv = 0, out = 0;
         for (size_t i : order) {
             js.push_back(jobs[i]);
             start.push_back(blocks);
+            who.push_back((int)i);
             blocks += (uint32_t)ff_crt_blocks(jobs[i].nout, jobs[i].bw);
             conv += (double)jobs[i].conv;
             out += (double)jobs[i].nout;
@@ -397,8 +422,10 @@ static void build_lists(ff_batch *B) {
             Pending pd{};
             pd.L.kind = 2, pd.L.n = (int)js.size(), pd.L.blocks = blocks;
             pd.L.conv_total = conv, pd.L.out_total = out;
+            pd.L.bytes = 12.0 * conv + 8.0 * out;
             pd.items_off = put(js.data(), js.size() * sizeof(FFCrtJob));
             pd.start_off = put(start.data(), start.size() * 4);
+            pd.item_size = sizeof(FFCrtJob), pd.start = start, pd.who = who;
             pend.push_back(pd);
         }
     }
@@ -422,11 +449,89 @@ static void build_lists(ff_batch *B) {
             pd.L.kind = wave ? 5 : 4, pd.L.n = (int)sch[wave].size(), pd.L.bytes = sbytes[wave];
             pd.items_off = put(sch[wave].data(), sch[wave].size() * sizeof(FFSchoolItem));
             pd.start_off = 0;
+            pd.item_size = sizeof(FFSchoolItem);
             pend.push_back(pd);
         }
     }
+    // the work-queue form: every block of every launch above as a task, in
+    // launch order, with its dependencies (k_queue)
+    size_t tasks_off = 0;
+    if (B->flags & FF_BATCH_QUEUE) {
+        const size_t ncnt = n + pre.size();
+        std::vector<uint32_t> count(ncnt, 0), fwd_total(ncnt, 0);
+        for (const Pending &pd : pend) {
+            if (pd.L.kind != 0 || pd.L.inv) continue;
+            for (size_t k = 0; k < pd.who.size(); k++) {
+                const uint32_t end = k + 1 < pd.start.size() ? pd.start[k + 1] : pd.L.blocks;
+                fwd_total[pd.who[k]] += end - pd.start[k];
+            }
+        }
+        std::vector<FFQTask> tasks;
+        size_t qlds = 0;
+        double qbytes = 0;
+        for (const Pending &pd : pend) {
+            qbytes += pd.L.bytes;
+            const int kind = pd.L.kind;
+            if (kind == 4 || kind == 5) {
+                qlds = std::max(qlds, kind == 4 ? ff_queue_lds_school() : ff_queue_lds_school_wave());
+                const uint32_t per = kind == 4 ? 1 : 4;
+                for (uint32_t k = 0; k < (uint32_t)pd.L.n; k += per) {
+                    const uint32_t m = std::min(per, (uint32_t)pd.L.n - k);
+                    tasks.push_back(FFQTask{kind == 4 ? (uint32_t)FQ_SCHOOL : (uint32_t)FQ_SCHOOL_WAVE,
+                                            (uint32_t)(pd.items_off + k * pd.item_size), m, FQ_NONE,
+                                            {FQ_NONE, FQ_NONE}, {0, 0}});
+                }
+                continue;
+            }
+            uint32_t qk;
+            if (kind == 0) qk = FQ_OUTER | (pd.L.inv ? 1u : 0u) | (pd.L.top ? 2u : 0u) | (pd.L.r3 ? 4u : 0u);
+            else if (kind == 1) qk = !pd.L.small ? FQ_INNER : (pd.L.r3 ? FQ_INNER_SMALL_R3 : FQ_INNER_SMALL);
+            else qk = FQ_CRT;
+            qlds = std::max(qlds, kind == 2 ? ff_queue_lds_crt() : pd.L.lds);
+            std::vector<uint32_t> added(ncnt, 0);
+            for (size_t k = 0; k < pd.who.size(); k++) {
+                const int c = pd.who[k];
+                const uint32_t end = k + 1 < pd.start.size() ? pd.start[k + 1] : pd.L.blocks;
+                FFQTask t{qk, (uint32_t)(pd.items_off + k * pd.item_size), 0, kind == 2 ? FQ_NONE : (uint32_t)c,
+                          {FQ_NONE, FQ_NONE}, {0, 0}};
+                // the product's tasks in earlier launches: its previous passes
+                if (count[c]) t.dep[0] = (uint32_t)c, t.need[0] = count[c];
+                // inner blocks also read a b transform made by another product
+                if (kind == 1 && c < (int)n) {
+                    if (b_owner[c] >= 0) t.dep[1] = (uint32_t)b_owner[c], t.need[1] = fwd_total[b_owner[c]];
+                    else if (pre_of[c] >= 0) t.dep[1] = (uint32_t)(n + pre_of[c]), t.need[1] = 3;
+                }
+                for (uint32_t b = pd.start[k]; b < end; b++) {
+                    t.sub = b - pd.start[k];
+                    tasks.push_back(t);
+                }
+                added[c] += end - pd.start[k];
+            }
+            for (size_t c = 0; c < ncnt; c++) count[c] += added[c];
+        }
+        for (size_t i = 0; i < n; i++) {
+            if (!B->zero[i]) continue;
+            const uint64_t *out = B->items[i].out;
+            tasks.push_back(FFQTask{FQ_FILL, (uint32_t)put(&out, sizeof out), 0, FQ_NONE, {FQ_NONE, FQ_NONE}, {0, 0}});
+        }
+        tasks_off = put(tasks.data(), tasks.size() * sizeof(FFQTask));
+        B->ntasks = (uint32_t)tasks.size();
+        B->nctr = (uint32_t)ncnt;
+        B->qlds = qlds;
+        B->qbytes = qbytes;
+        B->ctr = (uint32_t *)ff_dalloc((FQ_HEADER + ncnt) * sizeof(uint32_t));
+        FF_CHECK(hipMemsetAsync(B->ctr, 0, (FQ_HEADER + ncnt) * sizeof(uint32_t), g_st));
+    }
+    if (img.size() > 0xffffffffull) {
+        fprintf(stderr, "fastfib_amd: batch descriptors past 4 GiB\n");
+        exit(1);
+    }
     B->desc = (char *)ff_dalloc(img.size() + 256);
     ff_copy(B->desc, img.data(), img.size(), hipMemcpyHostToDevice);
+    if (B->flags & FF_BATCH_QUEUE) {
+        B->qtasks = (const FFQTask *)(B->desc + tasks_off);
+        return;
+    }
     for (auto &pd : pend) {
         BLaunch L = pd.L;
         L.items = B->desc + pd.items_off;
@@ -443,6 +548,10 @@ static void build_lists(ff_batch *B) {
 }
 
 static void enqueue_grouped(ff_batch *B) {
+    if (B->qtasks) {
+        FF_CHECK(ff_launch_queue(B->qtasks, B->ntasks, B->desc, B->ctr, B->nctr, B->qlds, B->qbytes, g_st));
+        return;
+    }
     for (const BLaunch &L : B->launches) {
         switch (L.kind) {
         case 0:
@@ -525,7 +634,7 @@ int ff_batch_create_sched(ff_batch **out, const ff_prod *items, size_t count, in
         B->shape[who[j]] = ff_shape_with(s0[j], pick[j]);
     }
     B->slots = (unsigned long long *)ff_dalloc((count ? count : 1) * 8);
-    if (!(flags & (FF_BATCH_GROUPED | FF_BATCH_GRAPH))) {
+    if (!(flags & (FF_BATCH_GROUPED | FF_BATCH_GRAPH | FF_BATCH_QUEUE))) {
         for (size_t i = 0; i < count; i++) {
             if (B->zero[i]) continue;
             const ff_prod &p = B->items[i];
@@ -554,7 +663,7 @@ extern "C" int ff_batch_run(ff_batch *B) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     if (B->exec) {
         FF_CHECK(hipGraphLaunch(B->exec, g_st));
-    } else if (!B->launches.empty() || (B->flags & FF_BATCH_GROUPED)) {
+    } else if (!B->launches.empty() || B->qtasks || (B->flags & (FF_BATCH_GROUPED | FF_BATCH_QUEUE))) {
         enqueue_grouped(B);
     } else {
         size_t k = 0;
@@ -602,5 +711,6 @@ extern "C" void ff_batch_destroy(ff_batch *B) {
     for (void *w : B->work) ff_dfree(w);
     ff_dfree(B->desc);
     ff_dfree(B->slots);
+    if (B->ctr) ff_dfree(B->ctr);
     delete B;
 }

@@ -23,6 +23,7 @@
 // fib.c:36-62 in one launch.
 #include <stdint.h>
 
+#include "ff_crt_body.h"
 #include "ff_crt_dev.h"
 #include "ff_internal.h"
 #include "ff_lookback.h"
@@ -59,171 +60,11 @@ static void garner_init(void) {
     g_garner_ready = 1;
 }
 
-
-#define CRT_LIMBS 256  // limbs per block, one per thread
-// Coefficients a block can touch: limbs [i0 - 2, i0 + 256) span 258 * 64 bits
-// and a coefficient 93 more, so at most (258 * 64 + 93) / bw + 2 of them;
-// 2077 at the narrowest width, 8 (mul_split.c:151-152).
-#define CRT_MAXC 2112
-
-
-// One product's residues for the limb sums.
-struct CrtSrc {
-    const uint32_t *y;  // [3][L]
-    uint32_t L;
-    int bw;
-    uint64_t conv;
-};
-
-// Limb sums of NP products for the workgroup's limbs: thread t holds limbs
-// i0 + t LPT + q (q < LPT), u[P][q] = lo(T_i) + hi(T_{i-1}) of product P,
-// and 2^64 - 1 at and past nout (a carry passes through); with prev,
-// thread 0 also gets each product's u at limb i0 - 1 (uprev[P]). The products' Garner
-// values are formed in one loop (their loads overlap). xlo / xhi: NP x MAXC
-// LDS words, hx: NP x 256 + 4 NP.
-template <int LPT, int NP, int MAXC>
-__device__ __forceinline__ void crt_sums(const CrtSrc *src, uint64_t nout, uint64_t i0, const FFGarner &G,
-                                         uint64_t *xlo, uint32_t *xhi, uint64_t *hx, bool prev, u128 (*u)[LPT],
-                                         u128 *uprev) {
-    const int t = threadIdx.x;
-    const uint64_t i1 = i0 + CRT_LIMBS * LPT < nout ? i0 + CRT_LIMBS * LPT : nout;
-    uint64_t jbase[NP], jend[NP];
-#pragma unroll
-    for (int P = 0; P < NP; P++) {
-        // coefficients touching limbs [i0 - 2, i1)
-        const uint64_t lim = src[P].conv < src[P].L ? src[P].conv : src[P].L;
-        jbase[P] = first_coef(i0 >= 2 ? i0 - 2 : 0, src[P].bw);
-        jend[P] = (64 * i1 + src[P].bw - 1) / src[P].bw;
-        if (jend[P] > lim) jend[P] = lim;
-        if (jend[P] < jbase[P]) jend[P] = jbase[P];
-    }
-    const uint64_t nj = NP == 1 ? jend[0] - jbase[0]
-                                : (jend[0] - jbase[0] > jend[NP - 1] - jbase[NP - 1] ? jend[0] - jbase[0]
-                                                                                   : jend[NP - 1] - jbase[NP - 1]);
-    // every residue load issued before the first Garner step: one HBM
-    // latency per workgroup instead of one per loop trip
-    constexpr int ITERS = (MAXC + CRT_LIMBS - 1) / CRT_LIMBS;
-    uint32_t r[NP][ITERS][3];
-#pragma unroll
-    for (int it = 0; it < ITERS; it++) {
-        const uint64_t k = t + (uint64_t)it * CRT_LIMBS;
-#pragma unroll
-        for (int P = 0; P < NP; P++) {
-            const uint64_t j = jbase[P] + k;
-            if (k < nj && j < jend[P]) {
-                const uint32_t *y = src[P].y;
-                const size_t L = src[P].L;
-                r[P][it][0] = y[j];
-                r[P][it][1] = y[L + j];
-                r[P][it][2] = y[2 * L + j];
-            }
-        }
-    }
-#pragma unroll
-    for (int it = 0; it < ITERS; it++) {
-        const uint64_t k = t + (uint64_t)it * CRT_LIMBS;
-#pragma unroll
-        for (int P = 0; P < NP; P++) {
-            if (k < nj && jbase[P] + k < jend[P]) {
-                const u128 X = ff_garner(r[P][it][0], r[P][it][1], r[P][it][2], G);
-                xlo[P * MAXC + k] = (uint64_t)X;
-                xhi[P * MAXC + k] = (uint32_t)(X >> 64);
-            }
-        }
-    }
-    __syncthreads();
-    const uint64_t ib = i0 + (uint64_t)t * LPT;
-#pragma unroll
-    for (int P = 0; P < NP; P++) {
-        const int bw = src[P].bw;
-        const uint64_t *xl = xlo + P * MAXC;
-        const uint32_t *xh = xhi + P * MAXC;
-        u128 T[LPT];
-        limb_sums<LPT>(ib, i1, bw, xl, xh, jbase[P], jend[P], T);
-            hx[P * CRT_LIMBS + t] = (uint64_t)(T[LPT - 1] >> 64);
-        // the sums below the workgroup's first limb, by lanes 2P and 2P + 1
-        // of the first wavefront together: T at i0 - 1 and at i0 - 2
-        if (t >> 1 == P && i0 > (uint64_t)(t & 1)) {
-            const u128 Tb = limb_sum(i0 - 1 - (t & 1), bw, jbase[P], xl, xh, jbase[P], jend[P]);
-            hx[NP * CRT_LIMBS + 4 * P + 2 * (t & 1)] = (uint64_t)Tb;
-            hx[NP * CRT_LIMBS + 4 * P + 2 * (t & 1) + 1] = (uint64_t)(Tb >> 64);
-        } else if (t >> 1 == P) {
-            hx[NP * CRT_LIMBS + 4 * P + 2 * (t & 1)] = 0;
-            hx[NP * CRT_LIMBS + 4 * P + 2 * (t & 1) + 1] = 0;
-        }
-#pragma unroll
-        for (int q = 0; q < LPT; q++) {
-            const u128 lo = (u128)(uint64_t)T[q];
-            if (q) u[P][q] = lo + (uint64_t)(T[q - 1] >> 64);
-            else u[P][0] = lo;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int P = 0; P < NP; P++) {
-        // hx[NP 256 + 4P ..]: lo, hi of T at i0 - 1, then at i0 - 2
-        const uint64_t *tb = hx + NP * CRT_LIMBS + 4 * P;
-        const uint64_t hp = t ? hx[P * CRT_LIMBS + t - 1] : tb[1];
-        if (prev && t == 0) uprev[P] = (u128)tb[0] + tb[3];
-        u[P][0] += hp;
-#pragma unroll
-        for (int q = 0; q < LPT; q++)
-            if (ib + q >= nout) u[P][q] = (u128)~0ull;
-    }
-    __syncthreads();  // the LDS is reused by the next call
+void ff_garner_get(FFGarner *out) {
+    garner_init();
+    *out = g_garner;
 }
 
-// Garner slots for the coefficients touching 256 LPT + 2 limbs at widths of
-// at least 32 bits (the fused step's, and the CRT's at four limbs per thread).
-#define CRT_MAXC_W32(LPT) (((CRT_LIMBS * (LPT) + 2) * 64 + 93) / 32 + 3)
-
-struct CrtShared {
-    uint64_t xlo[CRT_MAXC];
-    uint32_t xhi[CRT_MAXC];
-    uint64_t hx[CRT_LIMBS + 4];
-    LbShared S;
-};
-
-template <int LPT>
-__device__ __forceinline__ void crt_body(const FFCrtJob &J, uint32_t bx, const FFGarner &G, uint32_t *fault,
-                                         CrtShared &sh, bool resident) {
-    const uint64_t nblk = (J.nout + CRT_LIMBS * LPT - 1) / (CRT_LIMBS * LPT);
-    if (bx >= nblk) return;
-    uint64_t *xlo = sh.xlo, *hx = sh.hx;
-    uint32_t *xhi = sh.xhi;
-    LbShared &S = sh.S;
-    const uint32_t b = lb_ticket(S, J.lb, resident, bx);
-    const uint64_t i0 = (uint64_t)b * CRT_LIMBS * LPT, ib = i0 + (uint64_t)threadIdx.x * LPT;
-    u128 u[1][LPT], unused[1];
-    const CrtSrc src{J.y, J.L, J.bw, J.conv};
-    crt_sums<LPT, 1, CRT_MAXC>(&src, J.nout, i0, G, xlo, xhi, hx, false, u, unused);
-    bool g, p;
-    lb_group_gp<LPT>(u[0], &g, &p);
-    const uint32_t c = lb_round<1>(S, J.lb + 2, b, &g, &p, 0, fault);
-    uint64_t v[LPT];
-    lb_group_apply<LPT>(u[0], c, v);
-#pragma unroll
-    for (int q = 0; q < LPT; q++)
-        if (ib + q < J.nout) J.out[ib + q] = v[q];
-    unsigned long long *const keys[1] = {J.lenkey};
-    lb_keys<1, LPT>(J.lb + 2 + nblk, b, (uint32_t)nblk, i0, J.nout, v, keys, fault);
-    if (lb_last(S, J.lb, J.lb + 3 + nblk, b, (uint32_t)nblk)) lb_clear(J.lb, 3 + nblk + lb_groups(nblk));
-}
-
-// Limbs per thread of a job: one (shortest latency per workgroup) while a
-// workgroup per 256 limbs fits one wave of residency (~1280 workgroups);
-// above, four, where the coefficients are at least 32 bits wide (those
-// touching 1026 limbs fit CRT_MAXC).
-#define CRT_ONE_WAVE_LIMBS (1280u * CRT_LIMBS)
-__host__ __device__ static inline int crt_lpt(int bw, uint64_t nout) { return bw >= 32 && nout > CRT_ONE_WAVE_LIMBS ? 4 : 1; }
-// A work list (a batch) fills the GPU with many jobs: four limbs per thread
-// wherever the width allows, for throughput, except in jobs of at most one
-// block of limbs (the low levels of the decimal tree, small batch items):
-// one workgroup holds such a job either way, and one limb per thread keeps
-// its serial part short.
-__host__ __device__ static inline int crt_lpt_list(int bw, uint64_t nout) {
-    return bw >= 32 && nout > CRT_LIMBS ? 4 : 1;
-}
 
 template <int LPT>
 __global__ __launch_bounds__(CRT_LIMBS) void k_crt(FFCrtSet js, FFGarner G, uint32_t *fault, int resident) {

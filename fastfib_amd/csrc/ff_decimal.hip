@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -151,7 +152,7 @@ struct DecPlan {
 DecCtx g_dec;
 // plans per tree height and schoolbook bound (the bound decides, at plan
 // time, which level products skip the transforms)
-std::map<std::pair<int, int>, std::unique_ptr<DecPlan>> g_plans;
+std::map<std::tuple<int, int, int>, std::unique_ptr<DecPlan>> g_plans;  // by height, schoolbook bound, queue form
 
 uint64_t limbs_of(uint64_t bits) { return bits ? (bits + 63) / 64 : 1; }
 
@@ -394,7 +395,7 @@ ff_batch *level_batch(const uint64_t *a, uint64_t sa, uint64_t na, uint64_t bits
         it[j] = ff_prod{a + (uint64_t)j * sa, na, b.d, b.n, out + (uint64_t)j * so, 0, bits_a, b.bits};
     ff_batch *B = nullptr;
     std::vector<FFSched> sc(N, FFSched{0, 0, 0});
-    if (ff_batch_create_sched(&B, it.data(), N, FF_BATCH_GROUPED, sc.data()) != FF_OK) {
+    if (ff_batch_create_sched(&B, it.data(), N, g_dec_queue ? FF_BATCH_QUEUE : FF_BATCH_GROUPED, sc.data()) != FF_OK) {
         fprintf(stderr, "fastfib_amd: decimal level batch: %s\n", ff_last_error());
         exit(1);
     }
@@ -416,7 +417,7 @@ void capture_plan(DecPlan &pl) {
 
 // The plan below the top for height T > DEC_LEAF_LG.
 DecPlan *get_plan(int T) {
-    auto it = g_plans.find({T, g_school_limbs});
+    auto it = g_plans.find(std::make_tuple(T, g_school_limbs, g_dec_queue));
     if (it != g_plans.end()) return it->second.get();
     auto up = std::make_unique<DecPlan>();
     DecPlan &pl = *up;
@@ -477,7 +478,7 @@ DecPlan *get_plan(int T) {
     pl.first_nz = (unsigned long long *)palloc(pl, 8);
     pl.err = (uint32_t *)palloc(pl, 8);
     DecPlan *raw = up.get();
-    g_plans[{T, g_school_limbs}] = std::move(up);
+    g_plans[std::make_tuple(T, g_school_limbs, g_dec_queue)] = std::move(up);
     return raw;
 }
 
@@ -583,7 +584,7 @@ void decimal_dev(const uint64_t *a, uint64_t na, uint64_t bits, const char **dev
                            first_nz, err);
         FF_CHECK(hipGetLastError());
     } else {
-        const bool fresh = g_plans.find({T, g_school_limbs}) == g_plans.end();
+        const bool fresh = g_plans.find(std::make_tuple(T, g_school_limbs, g_dec_queue)) == g_plans.end();
         DecPlan *pl = get_plan(T);
         if (!fresh && !pl->exec) capture_plan(*pl);
         top_split(*pl, a, na, bits);

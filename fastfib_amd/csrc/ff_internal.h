@@ -124,8 +124,44 @@ struct FFInnerItem {
 enum FFKernelKind {
     FK_FWD_TOP, FK_FWD, FK_INNER, FK_SMALL, FK_INV, FK_INV_TOP,
     FK_CRT_PACK, FK_CRT_SCAN, FK_CRT_APPLY, FK_STEP, FK_CARRY, FK_NORM, FK_LINEAR, FK_DEC_LEAF, FK_EARLY, FK_STEP_CRT,
-    FK_SCHOOL, FK_COUNT
+    FK_SCHOOL, FK_QUEUE, FK_COUNT
 };
+
+// A batch as one persistent launch over a task table (k_queue, ff_ntt.hip;
+// the table is built by ff_batch.hip). A task is one workgroup's share of
+// one pass of one item, exactly a block of the corresponding list launch.
+// Tasks are listed in an order in which every dependency points to an
+// earlier task; workgroups claim them in that order with one atomic, wait
+// for their dependencies (counters of finished tasks per item), run the
+// body and count themselves finished. Outer kinds: FQ_OUTER | inv | top << 1
+// | r3 << 2.
+enum FFQueueKind {
+    FQ_OUTER = 0, FQ_INNER = 8, FQ_INNER_SMALL = 9, FQ_INNER_SMALL_R3 = 10, FQ_CRT = 11, FQ_SCHOOL = 12,
+    FQ_SCHOOL_WAVE = 13, FQ_FILL = 14
+};
+#define FQ_NONE 0xffffffffu
+struct FFQTask {
+    uint32_t kind;
+    uint32_t off;     // byte offset of the item in the batch's descriptor buffer (an FFOuterItem, FFInnerItem,
+                      // FFCrtJob, FFSchoolItem, or the output pointer of a zero product)
+    uint32_t sub;     // block within the item, prime-major; wave groups: products in the group (1 to 4)
+    uint32_t done;    // counter this task increments when it ends (FQ_NONE: none)
+    uint32_t dep[2];  // wait until counter dep[k] reaches need[k] (FQ_NONE: no dependency)
+    uint32_t need[2];
+};
+// ctr: FQ_HEADER words (the claim counter of part q at FQ_LINE * q, the
+// count of finished workgroups at FQ_LINE * FQ_MAXQ), then counter c at
+// FQ_HEADER + c; zero before the first launch, and zeroed again by the
+// launch's last workgroup.
+#define FQ_MAXQ 8
+#define FQ_LINE 32  // words: one 128-byte line per claim counter
+#define FQ_HEADER (FQ_LINE * (FQ_MAXQ + 1))
+hipError_t ff_launch_queue(const FFQTask *tasks, uint32_t ntasks, const char *desc, uint32_t *ctr, uint32_t nctr,
+                           size_t lds_bytes, double bytes, hipStream_t st);
+// Dynamic LDS the queue kernel needs for a CRT / schoolbook / one-wave schoolbook task.
+size_t ff_queue_lds_crt(void);
+size_t ff_queue_lds_school(void);
+size_t ff_queue_lds_school_wave(void);
 
 // Schoolbook products below the NTT threshold (ff_school.hip): both
 // operands shorter than FF_SCHOOL_LIMBS limbs, one workgroup per product.

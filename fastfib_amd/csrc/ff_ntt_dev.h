@@ -434,7 +434,8 @@ __device__ __forceinline__ uint32_t ff_row_freq(int pos, int rho, int r3) {
 // b-bit coefficient k of an operand stored as nw 32-bit words (the limbs,
 // little-endian), reduced mod p; zero past the words (src_load8,
 // ntt_kernels.c:40-74, reference).
-__device__ __forceinline__ uint32_t ff_load_coef(const uint32_t *w, uint32_t nw, uint64_t k, int bw, const FFPrime &P) {
+template <class Ptr>
+__device__ __forceinline__ uint32_t ff_load_coef(Ptr w, uint32_t nw, uint64_t k, int bw, const FFPrime &P) {
     const uint64_t bit = k * (uint64_t)bw;
     const uint64_t wi = bit >> 5;
     const int sh = (int)(bit & 31);

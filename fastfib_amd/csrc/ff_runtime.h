@@ -18,6 +18,10 @@ extern int g_quad_max_lg;     // longest quadrupling transform 2^quad_max_lg (0:
 extern int g_quad_inner_log;  // inner block of quadrupling transforms (0: inner_log)
 extern int g_host_bits;       // top bits of n done on the host before the device chain (10..13)
 extern int g_school_limbs;    // products with both operands shorter than this run schoolbook (0: none)
+extern int g_queue_waves;     // register budget of the batch work-queue kernel (waves per SIMD: 1 = unbounded, 3, 4)
+extern int g_queue_parts;     // claim counters of the work-queue kernel (1..8)
+extern int g_queue_chunk;     // tasks per claim (1..64)
+extern int g_dec_queue;       // the decimal conversion's level batches run as work queues (FF_BATCH_QUEUE)
 extern int g_dispatch;      // schedule source of the drop-in products and F(n): 0, FF_BATCH_TUNED or FF_BATCH_LEARNED
 extern int g_dispatch_gen;  // bumped whenever the table or the network changes
 

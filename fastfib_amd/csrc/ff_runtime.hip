@@ -29,6 +29,10 @@ int g_pair_steps = 2;
 int g_quad_max_lg = 18;
 int g_quad_inner_log = 10;
 int g_host_bits = 13;
+int g_queue_waves = 4;
+int g_dec_queue = 0;
+int g_queue_parts = 8;
+int g_queue_chunk = 8;
 int g_school_limbs = 32;  // measured: the one-block NTT is as fast or faster from 32 limbs up
 int g_dispatch_gen = 0;
 
@@ -303,7 +307,7 @@ static void prof_collect(void) {
 static const char *kKindNames[FK_COUNT] = {"ntt_fwd_top", "ntt_fwd", "ntt_inner", "ntt_small", "ntt_inv",
                                            "ntt_inv_top", "crt_pack", "crt_scan", "crt_apply", "fib_step",
                                            "carry", "normlen", "bigops", "dec_leaf", "fib_early", "step_crt",
-                                           "school"};
+                                           "school", "batch_queue"};
 
 extern "C" int ff_prof_enable(int on) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -400,6 +404,26 @@ extern "C" int ff_sync(void) {
     return hipStreamSynchronize(g_st) == hipSuccess ? FF_OK : FF_ERR_DEVICE;
 }
 
+extern "C" int ff_get_option(const char *name, long long *v) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!name || !v) return FF_ERR_ARG;
+    const struct {
+        const char *name;
+        long long value;
+    } opts[] = {{"inner_log", g_inner_log},       {"tile_words", g_tile_words},     {"force_len", (long long)g_force_len},
+                {"max_lg", g_max_lg},             {"pair_steps", g_pair_steps},     {"quad_max_lg", g_quad_max_lg},
+                {"quad_inner_log", g_quad_inner_log}, {"host_bits", g_host_bits}, {"school_limbs", g_school_limbs},
+                {"queue_waves", g_queue_waves},   {"dec_queue", g_dec_queue},
+                {"queue_parts", g_queue_parts},   {"queue_chunk", g_queue_chunk},       {"dispatch", g_dispatch}};
+    for (const auto &o : opts) {
+        if (!strcmp(name, o.name)) {
+            *v = o.value;
+            return FF_OK;
+        }
+    }
+    return FF_ERR_ARG;
+}
+
 extern "C" int ff_set_option(const char *name, long long v) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     if (!name) return FF_ERR_ARG;
@@ -430,6 +454,18 @@ extern "C" int ff_set_option(const char *name, long long v) {
     } else if (!strcmp(name, "school_limbs")) {
         if (v < 0 || v > FF_SCHOOL_LIMBS) return FF_ERR_ARG;
         g_school_limbs = (int)v;
+    } else if (!strcmp(name, "queue_waves")) {
+        if (v != 1 && v != 3 && v != 4) return FF_ERR_ARG;
+        g_queue_waves = (int)v;
+    } else if (!strcmp(name, "queue_parts")) {
+        if (v < 1 || v > FQ_MAXQ) return FF_ERR_ARG;
+        g_queue_parts = (int)v;
+    } else if (!strcmp(name, "queue_chunk")) {
+        if (v < 1 || v > 64) return FF_ERR_ARG;
+        g_queue_chunk = (int)v;
+    } else if (!strcmp(name, "dec_queue")) {
+        if (v != 0 && v != 1) return FF_ERR_ARG;
+        g_dec_queue = (int)v;
     } else if (!strcmp(name, "test_fail_alloc")) {
         if (v < -1) return FF_ERR_ARG;
         g_fail_after = v;

@@ -220,6 +220,8 @@ typedef struct {
 #define FF_BATCH_GRAPH 2   /* grouped, captured once in a HIP graph and replayed */
 #define FF_BATCH_TUNED 4   /* schedules from the autotuner table (ff_tune_*) */
 #define FF_BATCH_LEARNED 8 /* schedules from the learned dispatcher (ff_mlp_*) */
+#define FF_BATCH_QUEUE 16  /* one persistent launch per run: workgroups claim the batch's pass blocks from a
+                              task table in dependency order (with FF_BATCH_GRAPH: that launch captured) */
 
 typedef struct ff_batch ff_batch;
 
@@ -336,6 +338,17 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
                    transforms), as the reference does below its NTT threshold
                    of 200 limbs (mul_basic.c:33-35); default 32, where the
                    one-block NTT measured as fast from there up
+     "queue_waves" register budget of the work-queue kernel (FF_BATCH_QUEUE),
+                   in waves per SIMD: 1 the compiler's own allocation, no
+                   spills, two workgroups per CU; 3 or 4 (default, measured
+                   fastest) more workgroups per CU at the price of spilled
+                   registers
+     "queue_parts" claim counters of the work-queue kernel (1..8, default 8):
+                   workgroup w claims from part w mod parts
+     "queue_chunk" tasks a workgroup takes per claim (1..64, default 8)
+     "dec_queue"   1: the decimal conversion runs each of its level batches
+                   as one work-queue launch (FF_BATCH_QUEUE) instead of one
+                   launch per pass kind; default 0
      "test_fail_alloc" testing hook: after this many more device or pinned
                    allocations, every further one fails as if memory were
                    exhausted, so the drop-in's "out of memory (N bytes)" and
@@ -343,6 +356,8 @@ FF_API int ff_mlp_predict(const float *x, size_t n, const uint32_t *mask, float 
                    turns it off
    Returns FF_ERR_ARG for an unknown name or value. */
 FF_API int ff_set_option(const char *name, long long value);
+/* The current value of an option above (FF_ERR_ARG for an unknown name). */
+FF_API int ff_get_option(const char *name, long long *value);
 
 /* Live kernel timer (no counterpart in the reference; its profiling was a
    sampling profiler, tests/sprof.c). While enabled, each kernel launch is

@@ -1,7 +1,7 @@
 """Batched products (ff_batch_*) on the GPU, bit-exact.
 
 Every run form (one product at a time, grouped work lists, the captured
-graph) is checked against exact Python-int products on mixed batches that
+graph, the persistent work queue alone and captured) is checked against exact Python-int products on mixed batches that
 cover every schedule kind at once: one-block transforms, one and two outer
 passes, radix-3 lengths, widths 32..35, squares by pointer identity, zero
 operands. The full 4096-product uneven batch is checked per product through
@@ -36,7 +36,7 @@ def _mixed_pairs(ff, seed, count, lo, hi):
     return pairs, want
 
 
-@pytest.mark.parametrize("flags", [0, 1, 2])
+@pytest.mark.parametrize("flags", [0, 1, 2, 16, 18])
 def test_mixed_batch_exact(ff, flags):
     """80 products, operands of 2^6 .. 2^21 bits, against Python ints."""
     pairs, want = _mixed_pairs(ff, 11 + flags, 80, 6.0, 21.0)
@@ -49,6 +49,24 @@ def test_mixed_batch_exact(ff, flags):
     for i, o in enumerate(bt.outs):
         assert o.n == max(1, (want[i].bit_length() + 63) // 64)
     bt.close()
+
+
+@pytest.mark.parametrize("waves", [1, 3, 4])
+def test_queue_register_budgets(ff, waves):
+    """The work-queue kernel under each register budget (unbounded, 3 and 4
+    waves per SIMD) on a mixed batch, twice in a row (its counters are
+    cleared by the launch itself)."""
+    pairs, want = _mixed_pairs(ff, 41, 60, 6.0, 21.0)
+    ff.set_option("queue_waves", waves)
+    try:
+        bt = ff.Batch(pairs, ff.BATCH_QUEUE)
+        for rep in range(2):
+            bt.run()
+            bt.finish()
+            assert bt.results() == want, (waves, rep)
+        bt.close()
+    finally:
+        ff.set_option("queue_waves", 4)
 
 
 def test_batch_rerun_new_values(ff):
@@ -65,6 +83,7 @@ def test_batch_rerun_new_values(ff):
     ptrs = [(da.h.d, None if db is None else db.h.d) for da, db in pairs]
     g = ff.Batch(pairs, 2)
     p = ff.Batch(pairs, 1)
+    q = ff.Batch(pairs, 18)
     for rep in range(3):
         vals = []
         for (ba, bb), (da, db) in zip(sizes, pairs):
@@ -82,15 +101,22 @@ def test_batch_rerun_new_values(ff):
         p.run()
         p.finish()
         assert p.results() == vals, f"grouped, values {rep}"
+        for _ in range(2):
+            q.run()
+            q.finish()
+            assert q.results() == vals, f"queue graph, values {rep}"
     g.close()
     p.close()
+    q.close()
 
 
-def test_graph_replays_without_rebuild(ff):
-    """The captured graph replayed many times over the same operands, with
-    other library work in between, gives the same products each time."""
+@pytest.mark.parametrize("flags", [2, 18])
+def test_graph_replays_without_rebuild(ff, flags):
+    """The captured graph (of the grouped launches, or of the one queue
+    launch) replayed many times over the same operands, with other library
+    work in between, gives the same products each time."""
     pairs, want = _mixed_pairs(ff, 23, 40, 8.0, 20.0)
-    bt = ff.Batch(pairs, 2)
+    bt = ff.Batch(pairs, flags)
     x = ff.Device(12345678901234567890123 ** 50)
     for rep in range(4):
         bt.run()
@@ -107,7 +133,7 @@ def test_zero_and_tiny_items(ff):
     big = ff.Device((1 << 4000) - 1)
     pairs = [(zero, big), (big, zero), (zero, None), (one, one), (one, big), (big, None)]
     want = [0, 0, 0, 1, (1 << 4000) - 1, ((1 << 4000) - 1) ** 2]
-    for flags in (0, 1, 2):
+    for flags in (0, 1, 2, 16):
         bt = ff.Batch(pairs, flags)
         bt.run()
         bt.finish()
@@ -118,8 +144,8 @@ def test_zero_and_tiny_items(ff):
 
 def test_uneven_4096_residues(ff):
     """The full uneven batch (4096 products, operands 2^12 .. 2^24 bits, 30%
-    squares, seed 0) checked per product modulo three 61-bit primes, grouped
-    and graph forms."""
+    squares, seed 0) checked per product modulo three 61-bit primes, grouped,
+    graph and work-queue forms."""
     rng = np.random.default_rng(0)
     mix = uneven_mix(0, 4096)
     pairs, res = [], []
@@ -136,7 +162,7 @@ def test_uneven_4096_residues(ff):
             b = from_limbs(B)
             pairs.append((da, ff.Device(B)))
             res.append([x * (b % p) % p for x, p in zip(ra, PRIMES)])
-    for flags in (1, 2):
+    for flags in (1, 2, 16):
         bt = ff.Batch(pairs, flags)
         bt.run()
         bt.finish()
@@ -150,7 +176,8 @@ def test_uneven_4096_vs_reference(ff, ref):
     """The full uneven batch (4096 products, operands 2^12 .. 2^24 bits
     log-uniform, 30% squares, seed 0) limb for limb against the reference
     build's big_mul / big_sqr over the same operands, in every form: one
-    product at a time, grouped work lists and the captured graph."""
+    product at a time, grouped work lists, the captured graph and the
+    captured work queue."""
     rng = np.random.default_rng(0)
     mix = uneven_mix(0, 4096)
     items, pairs = [], []
@@ -160,7 +187,7 @@ def test_uneven_4096_vs_reference(ff, ref):
         items.append((A, B, B is None))
         pairs.append((ff.Device(A), None if B is None else ff.Device(B)))
     want = ref.mul_list(items)
-    for flags in (0, 1, 2):
+    for flags in (0, 1, 2, 18):
         bt = ff.Batch(pairs, flags)
         bt.run()
         bt.finish()
@@ -181,7 +208,7 @@ def test_shared_operand_one_block(ff):
     As = [operand_limbs(rng, s) for s in sizes]
     das = [ff.Device(a) for a in As]
     pairs = [(das[k], dbs[k % 2]) for k in range(len(As))] + [(das[0], None), (dbs[1], None)]
-    for flags in (0, 1, 2):
+    for flags in (0, 1, 2, 16):
         want = [from_limbs(As[k]) * from_limbs(Bs[k % 2]) for k in range(len(As))]
         want += [from_limbs(As[0]) ** 2, from_limbs(Bs[1]) ** 2]
         bt = ff.Batch(pairs, flags)
@@ -214,7 +241,7 @@ def test_shared_operand(ff):
         want.append(from_limbs(A) * b)
     pairs.append((db, None))
     want.append(b * b)
-    for flags in (0, 1, 2):
+    for flags in (0, 1, 2, 16):
         bt = ff.Batch(pairs, flags)
         bt.run()
         bt.finish()

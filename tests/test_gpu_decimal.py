@@ -149,3 +149,28 @@ def test_decimal_reciprocal_chain(ff):
         lo = from_limbs(ff.big_mul(P, to_limbs(m)))
         hi = from_limbs(ff.big_mul(P, to_limbs(m + 1)))
         assert lo <= (1 << (2 * s)) < hi, i
+
+
+def test_levels_as_work_queues(ff, ref):
+    """The level batches run as persistent work queues (option dec_queue):
+    values of every tree height against Python and the reference build, and
+    F(10^8)'s 20,898,764 digits by their digest, each conversion repeated
+    (the second runs the captured levels)."""
+    g = GOLDEN["decimal"]["fib_100000000"]
+    rng = np.random.default_rng(19)
+    ff.set_option("dec_queue", 1)
+    try:
+        for bits in (100, 2021, 70000, 330000):
+            x = from_limbs(operand_limbs(rng, bits))
+            assert ff.to_string(x) == ff.to_string(x) == str(x), bits
+        A = operand_limbs(rng, 4_000_000)
+        want = ref.to_string(A)
+        assert ff.to_string(A) == want and ff.to_string(A) == want
+        d = ff.dfibonacci(10 ** 8)
+        for _ in range(2):
+            s = d.to_string()
+            assert len(s) == g["digits"]
+            assert hashlib.sha256(s.encode()).hexdigest() == g["sha256"]
+    finally:
+        ff.set_option("dec_queue", 0)
+

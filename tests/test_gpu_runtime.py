@@ -49,7 +49,7 @@ def test_batch_replay_steady_state(ff):
         B = None if bb is None else operand_limbs(rng, bb)
         pairs.append((ff.Device(A), None if B is None else ff.Device(B)))
         want.append(from_limbs(A) * (from_limbs(A) if B is None else from_limbs(B)))
-    for flags in (ff.BATCH_GROUPED, ff.BATCH_GRAPH):
+    for flags in (ff.BATCH_GROUPED, ff.BATCH_GRAPH, ff.BATCH_QUEUE | ff.BATCH_GRAPH):
         bt = ff.Batch(pairs, flags)
         bt.run()
         bt.finish()
@@ -106,16 +106,21 @@ def test_out_of_memory_exits():
     assert "returned" not in r.stdout
 
 
-@pytest.mark.parametrize("entry", ["batch", "decimal"])
+@pytest.mark.parametrize("entry", ["batch", "queue", "decimal"])
 def test_lookback_timeout_fails_loudly_everywhere(entry):
-    """The time-out word set from the host: a batch's finish and a decimal
-    conversion (both carry their products by look-back) end the process
+    """The time-out word set from the host: a batch's finish (grouped, or a
+    work queue launched after it is set) and a decimal conversion (both carry their products by look-back) end the process
     with the message and status 1 instead of returning wrong limbs."""
     setup = ("import fastfib_amd as ff; ff.init(); "
              "a = ff.Device((1 << 200000) - 3); b = ff.Device((1 << 150000) + 7); ")
     if entry == "batch":
         work = ("bt = ff.Batch([(a, b), (a, None)], ff.BATCH_GROUPED); bt.run(); "
                 "assert ff.lib().ff_test_set_lb_fault() == 0; bt.finish(); print('returned')")
+    elif entry == "queue":
+        # set before the launch: the work queue's dependency waits give up
+        # at once (every task still runs and the kernel ends)
+        work = ("bt = ff.Batch([(a, b), (a, None)], ff.BATCH_QUEUE); "
+                "assert ff.lib().ff_test_set_lb_fault() == 0; bt.run(); bt.finish(); print('returned')")
     else:
         work = "assert ff.lib().ff_test_set_lb_fault() == 0; a.to_string(); print('returned')"
     r = _run(setup + work)

@@ -80,7 +80,7 @@ def test_school_in_batches(ff):
             b = _rand(rng, nb, (t + 1) % 4)
             pairs.append((da, ff.Device(b)))
             want.append(a * b)
-    for flags in (ff.BATCH_PLAIN, ff.BATCH_GROUPED, ff.BATCH_GRAPH):
+    for flags in (ff.BATCH_PLAIN, ff.BATCH_GROUPED, ff.BATCH_GRAPH, ff.BATCH_QUEUE):
         bt = ff.Batch(pairs, flags)
         bt.run()
         bt.finish()

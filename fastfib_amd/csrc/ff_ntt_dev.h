@@ -283,8 +283,13 @@ __device__ void ff_round_any(int nl, bool from_g, bool to_g, const FFTile &t, in
 #undef FF_RCASE
 }
 
-// Round sizes: DIF rounds cover the row bits from the top, 4 at a time, the
-// remainder (rho % 4, or 4) last; DIT rounds mirror them.
+// Round sizes: rounds of 4 levels and one of the remainder (rho % 4, or 4).
+// DIF rounds cover the row bits from the top and DIT rounds from the bottom,
+// so the remainder takes the top bits in both: the round at the lowest bits
+// (the forward's last, the inverse's first) is the one that meets HBM in
+// the outer passes and applies their twiddles, two table roots and a
+// geometric sequence per group, which 16-element groups share far better
+// than 2-element ones.
 __device__ __forceinline__ int ff_nrounds(int rho) { return (rho + 3) / 4; }
 __device__ __forceinline__ int ff_last_nl(int rho) { return rho % 4 == 0 ? 4 : rho % 4; }
 
@@ -317,7 +322,7 @@ __device__ void ff_forward_io(const FFTile &t, const uint32_t *rt, const uint32_
     const int R = ff_nrounds(t.rho);
     int b = t.rho;
     for (int r = 0; r < R; r++) {
-        const int nl = r == R - 1 ? ff_last_nl(t.rho) : 4;
+        const int nl = r == 0 ? ff_last_nl(t.rho) : 4;
         b -= nl;
         if (r) __syncthreads();
         ff_round_any<FF_R_DIF>(nl, from_g && r == 0, to_g && r == R - 1, t, b, rt, rtp, nullptr, nullptr, nullptr,
@@ -325,7 +330,7 @@ __device__ void ff_forward_io(const FFTile &t, const uint32_t *rt, const uint32_
     }
 }
 
-// Inverse (DIT) over the tile, mirror of ff_forward_io.
+// Inverse (DIT) over the tile: the rounds of ff_forward_io in reverse.
 template <class Ld, class St, class Gt = FFNoTw>
 __device__ void ff_inverse_io(const FFTile &t, const uint32_t *irt, const uint32_t *irtp, uint32_t p, uint32_t pinv,
                               bool from_g, bool to_g, Ld ld, St st, const Gt &gt = Gt()) {
@@ -336,7 +341,7 @@ __device__ void ff_inverse_io(const FFTile &t, const uint32_t *irt, const uint32
     const int R = ff_nrounds(t.rho);
     int b = 0;
     for (int r = 0; r < R; r++) {
-        const int nl = r == 0 ? ff_last_nl(t.rho) : 4;
+        const int nl = r == R - 1 ? ff_last_nl(t.rho) : 4;
         if (r) __syncthreads();
         ff_round_any<FF_R_DIT>(nl, from_g && r == 0, to_g && r == R - 1, t, b, nullptr, nullptr, irt, irtp, nullptr,
                                p, pinv, ld, st, gt);

@@ -160,6 +160,56 @@ struct FFOuterTw {
     }
 };
 
+// The tile of a forward top pass from the operand's limbs: element (row
+// pos, column c) is the bw-bit coefficient pos * Sb + c0 + c of the nw
+// 32-bit words at src, reduced mod p (src_load8, ntt_kernels.c:40-74,
+// reference). Coefficient positions fit 32 bits (L <= 3 * 2^25 coefficients
+// of at most 35 bits). The three words holding a coefficient are read by
+// buffer loads, which return zero past the operand's last word, so
+// coefficients past the operand need no bounds test; each thread issues
+// the loads of FF_UNPACK_B elements before it uses the first of them. The
+// high bits of a coefficient wider than 32 bits add hi * 2^32 mod p, a
+// Montgomery product by R^2.
+#define FF_UNPACK_B 8
+__device__ __forceinline__ void ff_unpack_tile(const FFTile &t, const uint32_t *src, uint32_t nw, uint32_t Sb,
+                                               uint32_t c0, int bw, const FFPrime &P) {
+    const int n = (t.nsub << t.rho) << t.logC;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)(nw * 4u), 0x00020000);
+    const uint32_t p = P.p;
+    const uint32_t lomask = bw >= 32 ? ~0u : ((1u << bw) - 1);
+    const uint32_t himask = bw > 32 ? ((1u << (bw - 32)) - 1) : 0u;
+    const uint32_t r2p = P.r2 * P.pinv;
+    const int tid = ff_tile_tid(t), nthr = ff_tile_nthr(t);
+    for (int k0 = 0; k0 * nthr < n; k0 += FF_UNPACK_B) {
+        uint32_t w[FF_UNPACK_B][3], sh[FF_UNPACK_B];
+#pragma unroll
+        for (int e = 0; e < FF_UNPACK_B; e++) {
+            const int i = tid + (k0 + e) * nthr;
+            const uint32_t g = (uint32_t)(i >> t.logC) * Sb + c0 + (uint32_t)(i & (t.C - 1));
+            const uint32_t bit = g * (uint32_t)bw;
+            const uint32_t off = (bit >> 5) << 2;
+            sh[e] = bit & 31;
+            // past the tile (i >= n) the loads are harmless and their values unused
+            w[e][0] = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+            w[e][1] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 4, 0, 0);
+            w[e][2] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 8, 0, 0);
+        }
+#pragma unroll
+        for (int e = 0; e < FF_UNPACK_B; e++) {
+            const int i = tid + (k0 + e) * nthr;
+            const uint32_t lo = __builtin_amdgcn_alignbit(w[e][1], w[e][0], sh[e]) & lomask;
+            const uint32_t hi = __builtin_amdgcn_alignbit(w[e][2], w[e][1], sh[e]) & himask;
+            uint32_t r = lo, q = r - p;
+            r = q < r ? q : r;
+            q = r - p;
+            r = q < r ? q : r;  // lo < 2^32 < 3p
+            if (bw > 32) r = ff_add(r, ff_mont_w(hi, P.r2, r2p, p), p);
+            if (i < n) t.s[ff_pos_addr(t, i >> t.logC, i & (t.C - 1))] = r;
+        }
+    }
+}
+
 // Outer pass over one tile of R rows x C columns (tile index `tile` among
 // the pass's nblocks * tpb) of array X, prime pi. COH: the array is read and
 // written by coherent accesses (ff_ld / ff_st).
@@ -199,11 +249,9 @@ __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, i
         if (!R3 && !TOP) {
             ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, true, true, load_fwd, store_fwd, tw);
         } else {
-            // limbs are unpacked element by element into the LDS tile (16
-            // unpacks in flight per thread would not fit in registers), and
-            // the radix-3 layer works on the tile
-            for (int i = threadIdx.x; i < n; i += blockDim.x) lds_store(i >> ps.logC, i & (ps.C - 1),
-                                                                        load_fwd(i >> ps.logC, i & (ps.C - 1)));
+            // the coefficients are unpacked into the LDS tile, and the
+            // radix-3 layer works on the tile
+            ff_unpack_tile(t, X.src, nw, Sb, c0, bw, P);
             if (R3) ff_radix3_layer<false>(t, T, pi, P, ps.rstep3);
             else __syncthreads();
             ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, false, true, lds_load, store_fwd, tw);

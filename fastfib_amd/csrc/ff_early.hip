@@ -54,8 +54,8 @@ __device__ __forceinline__ void early_pair_mid(const FFTile &t, const uint32_t *
             va[j] = t.s[ff_pos_addr(t, (g << NL) | j, 0)];
             vb[j] = t.s[ff_pos_addr(t, off + ((g << NL) | j), 0)];
         }
-        ff_dif_group<NL, true>(va, 0, 0, tw[0], tw[1], p, pinv);
-        ff_dif_group<NL, true>(vb, 0, 0, tw[0], tw[1], p, pinv);
+        ff_dif_group<NL, true, true>(va, 0, 0, tw[0], tw[1], p, pinv);
+        ff_dif_group<NL, true, true>(vb, 0, 0, tw[0], tw[1], p, pinv);
 #pragma unroll
         for (int j = 0; j < G; j++) {
             const uint32_t a = va[j], b = vb[j];
@@ -64,8 +64,8 @@ __device__ __forceinline__ void early_pair_mid(const FFTile &t, const uint32_t *
             va[j] = bit ? x : y;
             vb[j] = bit ? ff_add(x, y, p) : x;
         }
-        ff_dit_group<NL, true>(va, 0, 0, tw[2], tw[3], p, pinv);
-        ff_dit_group<NL, true>(vb, 0, 0, tw[2], tw[3], p, pinv);
+        ff_dit_group<NL, true, true>(va, 0, 0, tw[2], tw[3], p, pinv);
+        ff_dit_group<NL, true, true>(vb, 0, 0, tw[2], tw[3], p, pinv);
 #pragma unroll
         for (int j = 0; j < G; j++) {
             t.s[ff_pos_addr(t, (g << NL) | j, 0)] = va[j];

@@ -475,7 +475,7 @@ __device__ __forceinline__ void pair_mid_round(const FFTile &t, bool active, con
                 const int ad = ff_pos_addr(t, (g << NL) | j, 0);
                 v[j] = pt(A[ad], B[ad], p, pinv);
             }
-            ff_dit_group<NL>(v, 0, 0, irt, irtp, p, pinv);
+            ff_dit_group<NL, false, true>(v, 0, 0, irt, irtp, p, pinv);
         }
         __syncthreads();
         if (act) {

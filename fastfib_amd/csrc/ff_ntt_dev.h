@@ -21,16 +21,27 @@
 // rtp[8 + m]), the same for every group of the prime: one gathered load
 // per level instead of 2^lb, at the price of cheap multiplies.
 // LDS: the tables were copied into LDS (rt points there).
-template <int LB, bool LDS>
+// LOW0: the round starts at row bit 0 (low = 0, h = 2^lb), where every
+// twiddle rt[h + k] is one of the first 16 table entries, read as it is (no
+// products) and the same for every group.
+template <int LB, bool LDS, bool LOW0 = false>
 __device__ __forceinline__ void ff_group_twiddles(uint32_t tw[1 << LB], uint32_t twp[1 << LB], int h, int low,
                                                   const uint32_t *rt, const uint32_t *rtp, uint32_t p,
                                                   uint32_t pinv) {
-    tw[0] = rt[h + low];
-    twp[0] = rtp[h + low];
     // the 16th roots sit at the same address for every lane: from HBM they
     // are read through the constant address space, so they arrive by scalar
     // loads
     typedef const __attribute__((address_space(4))) uint32_t *cptr;
+    if constexpr (LOW0) {
+#pragma unroll
+        for (int k = 0; k < (1 << LB); k++) {
+            tw[k] = LDS ? rt[(1 << LB) + k] : ((cptr)rt)[(1 << LB) + k];
+            twp[k] = LDS ? rtp[(1 << LB) + k] : ((cptr)rtp)[(1 << LB) + k];
+        }
+        return;
+    }
+    tw[0] = rt[h + low];
+    twp[0] = rtp[h + low];
 #pragma unroll
     for (int k = 1; k < (1 << LB); k++) {
         const int m = k << (3 - LB);  // w_(2^(LB+1))^k = w_16^(k 2^(3-LB))
@@ -41,12 +52,14 @@ __device__ __forceinline__ void ff_group_twiddles(uint32_t tw[1 << LB], uint32_t
     }
 }
 
-template <int NL, bool LDS = false>
+// LOW0: b_lo = 0 known at compile time (twiddles from the first table
+// entries, the product by w^0 = 1 left out).
+template <int NL, bool LDS = false, bool LOW0 = false>
 __device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, const uint32_t *rt, const uint32_t *rtp,
                                              uint32_t p, uint32_t pinv) {
 #pragma unroll
     for (int lb = NL - 1; lb >= 0; lb--) {
-        const int b = b_lo + lb;
+        const int b = LOW0 ? lb : b_lo + lb;
         const int h = 1 << b;
         if (b == 0) {
 #pragma unroll
@@ -60,10 +73,10 @@ __device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, con
             continue;
         }
         uint32_t tw[8], twp[8];
-        if (lb == 0) ff_group_twiddles<0, LDS>(tw, twp, h, low, rt, rtp, p, pinv);
-        else if (lb == 1) ff_group_twiddles<1, LDS>(tw, twp, h, low, rt, rtp, p, pinv);
-        else if (lb == 2) ff_group_twiddles<2, LDS>(tw, twp, h, low, rt, rtp, p, pinv);
-        else ff_group_twiddles<3, LDS>(tw, twp, h, low, rt, rtp, p, pinv);
+        if (lb == 0) ff_group_twiddles<0, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv);
+        else if (lb == 1) ff_group_twiddles<1, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv);
+        else if (lb == 2) ff_group_twiddles<2, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv);
+        else ff_group_twiddles<3, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv);
 #pragma unroll
         for (int j = 0; j < (1 << NL); j++) {
             if (j & (1 << lb)) continue;
@@ -71,34 +84,32 @@ __device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, con
             const uint32_t x = v[j], y = v[k];
             const int q = j & ((1 << lb) - 1);
             v[j] = ff_add(x, y, p);
-            v[k] = ff_mont_w(x - y + p, tw[q], twp[q], p);
+            v[k] = (LOW0 && q == 0) ? ff_sub(x, y, p) : ff_mont_w(x - y + p, tw[q], twp[q], p);
         }
     }
 }
 
-template <int NL, bool LDS = false>
+template <int NL, bool LDS = false, bool LOW0 = false>
 __device__ __forceinline__ void ff_dit_group(uint32_t *v, int low, int b_lo, const uint32_t *irt, const uint32_t *irtp,
                                              uint32_t p, uint32_t pinv) {
 #pragma unroll
     for (int lb = 0; lb < NL; lb++) {
-        const int b = b_lo + lb;
+        const int b = LOW0 ? lb : b_lo + lb;
         const int h = 1 << b;
         uint32_t tw[8], twp[8];
         if (b != 0) {
-            if (lb == 0) ff_group_twiddles<0, LDS>(tw, twp, h, low, irt, irtp, p, pinv);
-            else if (lb == 1) ff_group_twiddles<1, LDS>(tw, twp, h, low, irt, irtp, p, pinv);
-            else if (lb == 2) ff_group_twiddles<2, LDS>(tw, twp, h, low, irt, irtp, p, pinv);
-            else ff_group_twiddles<3, LDS>(tw, twp, h, low, irt, irtp, p, pinv);
+            if (lb == 0) ff_group_twiddles<0, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv);
+            else if (lb == 1) ff_group_twiddles<1, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv);
+            else if (lb == 2) ff_group_twiddles<2, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv);
+            else ff_group_twiddles<3, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv);
         }
 #pragma unroll
         for (int j = 0; j < (1 << NL); j++) {
             if (j & (1 << lb)) continue;
             const int k = j | (1 << lb);
             uint32_t x = v[j], y = v[k];
-            if (b != 0) {
-                const int q = j & ((1 << lb) - 1);
-                y = ff_mont_w(y, tw[q], twp[q], p);
-            }
+            const int q = j & ((1 << lb) - 1);
+            if (b != 0 && !(LOW0 && q == 0)) y = ff_mont_w(y, tw[q], twp[q], p);
             v[j] = ff_add(x, y, p);
             v[k] = ff_sub(x, y, p);
         }
@@ -237,15 +248,20 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
 #pragma unroll
             for (int j = 0; j < G; j++) v[j] = ff_mont(v[j], tw[j], p, pinv);
         }
+        // the rounds at row bit 0 (the middle round, the outer passes' HBM
+        // rounds) take the constant-twiddle form
         if (KIND == FF_R_DIF) {
-            ff_dif_group<NL, Gt::lds_tables>(v, low, b_lo, rt, rtp, p, pinv);
+            if (b_lo == 0) ff_dif_group<NL, Gt::lds_tables, true>(v, 0, 0, rt, rtp, p, pinv);
+            else ff_dif_group<NL, Gt::lds_tables>(v, low, b_lo, rt, rtp, p, pinv);
         } else if (KIND == FF_R_DIT) {
-            ff_dit_group<NL, Gt::lds_tables>(v, low, b_lo, irt, irtp, p, pinv);
+            if (b_lo == 0) ff_dit_group<NL, Gt::lds_tables, true>(v, 0, 0, irt, irtp, p, pinv);
+            else ff_dit_group<NL, Gt::lds_tables>(v, low, b_lo, irt, irtp, p, pinv);
         } else {
-            ff_dif_group<NL, Gt::lds_tables>(v, low, b_lo, rt, rtp, p, pinv);
+            // FF_R_MID is only called at b_lo = 0
+            ff_dif_group<NL, Gt::lds_tables, true>(v, 0, 0, rt, rtp, p, pinv);
 #pragma unroll
             for (int j = 0; j < G; j++) v[j] = gt.mid(v[j], B ? B[a0 + off[j]] : v[j], p, pinv);
-            ff_dit_group<NL, Gt::lds_tables>(v, low, b_lo, irt, irtp, p, pinv);
+            ff_dit_group<NL, Gt::lds_tables, true>(v, 0, 0, irt, irtp, p, pinv);
         }
         if constexpr (Gt::on && KIND == FF_R_DIF && TO_G) {
             uint32_t tw[G];

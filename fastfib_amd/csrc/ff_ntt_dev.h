@@ -24,10 +24,16 @@
 // LOW0: the round starts at row bit 0 (low = 0, h = 2^lb), where every
 // twiddle rt[h + k] is one of the first 16 table entries, read as it is (no
 // products) and the same for every group.
+// Tables in HBM (not LDS), b_lo > 0: every twiddle rt[h + low + (k << b_lo)]
+// and its companion are read from the tables by buffer loads whose
+// per-twiddle offset k << b_lo is uniform (a scalar register), so the loads
+// cost the vector unit one address per level, and the products that would
+// derive them are left to the memory pipeline, idle in rounds between two
+// LDS passes.
 template <int LB, bool LDS, bool LOW0 = false>
 __device__ __forceinline__ void ff_group_twiddles(uint32_t tw[1 << LB], uint32_t twp[1 << LB], int h, int low,
                                                   const uint32_t *rt, const uint32_t *rtp, uint32_t p,
-                                                  uint32_t pinv) {
+                                                  uint32_t pinv, int b_lo = 0) {
     // the 16th roots sit at the same address for every lane: from HBM they
     // are read through the constant address space, so they arrive by scalar
     // loads
@@ -37,6 +43,19 @@ __device__ __forceinline__ void ff_group_twiddles(uint32_t tw[1 << LB], uint32_t
         for (int k = 0; k < (1 << LB); k++) {
             tw[k] = LDS ? rt[(1 << LB) + k] : ((cptr)rt)[(1 << LB) + k];
             twp[k] = LDS ? rtp[(1 << LB) + k] : ((cptr)rtp)[(1 << LB) + k];
+        }
+        return;
+    }
+    if constexpr (!LDS) {
+        const __amdgpu_buffer_rsrc_t r0 = __builtin_amdgcn_make_buffer_rsrc((void *)rt, (short)0, FF_TW * 4, 0x00020000);
+        const __amdgpu_buffer_rsrc_t r1 =
+            __builtin_amdgcn_make_buffer_rsrc((void *)rtp, (short)0, FF_TW * 4, 0x00020000);
+        const uint32_t vo = (uint32_t)(h + low) << 2;
+#pragma unroll
+        for (int k = 0; k < (1 << LB); k++) {
+            const int so = (k << b_lo) << 2;
+            tw[k] = __builtin_amdgcn_raw_buffer_load_b32(r0, vo, so, 0);
+            twp[k] = __builtin_amdgcn_raw_buffer_load_b32(r1, vo, so, 0);
         }
         return;
     }
@@ -73,10 +92,10 @@ __device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, con
             continue;
         }
         uint32_t tw[8], twp[8];
-        if (lb == 0) ff_group_twiddles<0, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv);
-        else if (lb == 1) ff_group_twiddles<1, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv);
-        else if (lb == 2) ff_group_twiddles<2, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv);
-        else ff_group_twiddles<3, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv);
+        if (lb == 0) ff_group_twiddles<0, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv, b_lo);
+        else if (lb == 1) ff_group_twiddles<1, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv, b_lo);
+        else if (lb == 2) ff_group_twiddles<2, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv, b_lo);
+        else ff_group_twiddles<3, LDS, LOW0>(tw, twp, h, low, rt, rtp, p, pinv, b_lo);
 #pragma unroll
         for (int j = 0; j < (1 << NL); j++) {
             if (j & (1 << lb)) continue;
@@ -98,10 +117,10 @@ __device__ __forceinline__ void ff_dit_group(uint32_t *v, int low, int b_lo, con
         const int h = 1 << b;
         uint32_t tw[8], twp[8];
         if (b != 0) {
-            if (lb == 0) ff_group_twiddles<0, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv);
-            else if (lb == 1) ff_group_twiddles<1, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv);
-            else if (lb == 2) ff_group_twiddles<2, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv);
-            else ff_group_twiddles<3, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv);
+            if (lb == 0) ff_group_twiddles<0, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv, b_lo);
+            else if (lb == 1) ff_group_twiddles<1, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv, b_lo);
+            else if (lb == 2) ff_group_twiddles<2, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv, b_lo);
+            else ff_group_twiddles<3, LDS, LOW0>(tw, twp, h, low, irt, irtp, p, pinv, b_lo);
         }
 #pragma unroll
         for (int j = 0; j < (1 << NL); j++) {

@@ -994,7 +994,13 @@ extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int 
     const int kind = in.small ? FK_SMALL : FK_INNER;
     const double L4 = 4.0 * pl->L;
     ff_prof_begin(kind, st);
-    const size_t lds = 2 * words * sizeof(uint32_t);
+    // squares use tile A only: half the LDS, more workgroups per CU
+    bool one_tile = true;
+    for (int i = 0; i < nc; i++) {
+        const FFConv &c = cs->c[i];
+        one_tile = one_tile && !c.pair && !c.bspec && (in.small ? c.bsrc == nullptr : c.fb == nullptr);
+    }
+    const size_t lds = (one_tile ? 1 : 2) * words * sizeof(uint32_t);
     if (cs->c[0].pair & 6) {  // two outputs or degree 4: the 512-thread pair kernel
         hipLaunchKernelGGL(k_inner_pair, grid, dim3(2 * FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
     } else if (!in.small)

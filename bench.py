@@ -219,7 +219,7 @@ def cpu_baseline(n: int, seconds: float) -> dict | None:
 BATCH_PRIMES = (2305843009213693951, 1152921504606846883)
 
 
-def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bool = True) -> dict:
+def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bool = True, only=None) -> dict:
     """The uneven batch: count products, operand bits log-uniform in
     [2^12, 2^24], 30% squares, uniform limbs (fastfib_amd.workloads), one
     step = the whole batch. Run forms: one product at a time (plain), grouped
@@ -249,6 +249,8 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bo
              ("graph_learned", ff.BATCH_GRAPH | ff.BATCH_LEARNED, 1),
              ("queue", ff.BATCH_QUEUE, 4), ("graph_queue", ff.BATCH_GRAPH | ff.BATCH_QUEUE, 4),
              ("queue_w1", ff.BATCH_QUEUE, 1), ("queue_w3", ff.BATCH_QUEUE, 3)]
+    if only:
+        forms = [f for f in forms if f[0] in only]
     # the reference's own big_mul / big_sqr over the same products
     want = None
     try:
@@ -285,11 +287,13 @@ def batch_leg(ff, count: int, seed: int, steps: int, cpu_seconds: float, cpu: bo
                      "steps": n_steps}
         bt.close()
     ff.set_option("queue_waves", 4)
-    plain = out["plain"]["gbit_products_per_s"]
     leg = {"workload": f"{count} uneven products, operands 2^12..2^24 bits log-uniform, 30% squares, seed {seed}",
            "operand_gbit": round(gbit, 3), "forms": out, "bit_exact": ok,
-           "checked_against": "reference big_mul/big_sqr, limb for limb" if want is not None else "residues",
-           "speedup_vs_plain": {k: round(v["gbit_products_per_s"] / plain, 2) for k, v in out.items() if k != "plain"}}
+           "checked_against": "reference big_mul/big_sqr, limb for limb" if want is not None else "residues"}
+    if "plain" in out:
+        plain = out["plain"]["gbit_products_per_s"]
+        leg["speedup_vs_plain"] = {k: round(v["gbit_products_per_s"] / plain, 2) for k, v in out.items()
+                                   if k != "plain"}
     if cpu:
         leg["cpu_baseline"] = batch_cpu_baseline(host, gbit, cpu_seconds)
     return leg
@@ -638,7 +642,8 @@ def run_legs_only(ff, args, legs) -> int:
     one JSON line with each leg's result, no CPU baselines."""
     out, ok = {"legs": sorted(legs)}, True
     if "batch" in legs:
-        out["batch"] = batch_leg(ff, args.batch_count, args.batch_seed, args.batch_steps, 0.0, cpu=False)
+        out["batch"] = batch_leg(ff, args.batch_count, args.batch_seed, args.batch_steps, 0.0, cpu=False,
+                                 only=set(args.batch_forms.split(",")) if args.batch_forms else None)
         ok &= out["batch"]["bit_exact"]
     if "regret" in legs:
         out["dispatcher_regret"] = dispatcher_regret()
@@ -670,6 +675,9 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=4096, help="uneven batch size (0: skip the batch leg)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--batch-forms", default="", help="comma list of batch forms for a profiling run "
+                    "(plain, grouped, graph, graph_tuned, graph_learned, queue, graph_queue, queue_w1, queue_w3; "
+                    "default: all)")
     ap.add_argument("--decimal-steps", type=int, default=5, help="warm decimal conversions of F(n) (0: skip)")
     ap.add_argument("--large-n", type=int, default=10 ** 9, help="F(n) leg with dispatcher schedules (0: skip)")
     ap.add_argument("--large-steps", type=int, default=5)

@@ -103,6 +103,18 @@ def int32_probe(ff) -> dict:
             "spec_T_per_s": round(INT32_MUL_SPEC_T, 1)}
 
 
+def copy_probe(ff, mib: int = 256) -> dict:
+    """The measured HBM bandwidth: the library's plain device copy of mib
+    MiB (k_copy_probe, 4 launches, the best of the last three), bytes read
+    plus written per second. Its dispatches are also the calibration of the
+    profiler's FETCH_SIZE / WRITE_SIZE counters."""
+    L = ff.lib()
+    L.ff_probe_copy.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    rate, secs = ctypes.c_double(), ctypes.c_double()
+    ff._check(L.ff_probe_copy(mib << 20, ctypes.byref(rate), ctypes.byref(secs)))
+    return {"measured_GB_per_s": round(rate.value / 1e9, 1), "copy_MiB": mib, "launch_ms": round(secs.value * 1e3, 4)}
+
+
 def profiled_valu(kind: str):
     """VALU wave-instructions per launch of a kernel kind from the committed
     rocprofv3 SQ_INSTS_VALU summary, or None."""
@@ -690,14 +702,9 @@ def main() -> int:
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes), then exit")
     args = ap.parse_args()
     if args.calibrate_copy:
-        import torch
-        a = torch.ones(64 << 20, device="cuda")  # 64 Mi float32 = 256 MiB
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print("copied 256 MiB x3", float(b[0]))
+        import fastfib_amd as ff
+        ff.init()
+        print(json.dumps(copy_probe(ff)))
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (products shard by prime or "
@@ -753,6 +760,8 @@ def main() -> int:
         roofline["int32_mul"] = {"mults_per_launch": round(d["mults"] / d["launches"]), "achieved": round(mr, 2),
                                  "peak": peak, "unit": "T int32 mul/s", "frac": round(mr / peak, 4),
                                  "peak_source": "measured by ff_probe_int32_mul on this GPU", "probe": probe}
+    # the measured HBM bandwidth of a plain copy on this GPU, beside the spec peak
+    roofline["hbm_probe"] = copy_probe(ff)
     vi = profiled_valu(dom)
     if vi:
         # the transform kernels' actual bound: VALU issue (profiled instructions

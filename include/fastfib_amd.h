@@ -381,6 +381,14 @@ FF_API int ff_prof_read_mults(int k, double *mults);
    launch's duration. */
 FF_API int ff_probe_int32_mul(double *mults_per_s, double *seconds);
 
+/* Bandwidth probe (no counterpart in the reference): a plain device copy of
+   `bytes` bytes (16 bytes per lane, grid-stride, 8 workgroups per CU),
+   best of three timed runs; *bytes_per_s counts bytes read plus bytes
+   written. The measured HBM peak beside the 8 TB/s specification, and the
+   dispatch of known size (k_copy_probe) that scales rocprofv3's FETCH_SIZE
+   and WRITE_SIZE counters to bytes. */
+FF_API int ff_probe_copy(size_t bytes, double *bytes_per_s, double *seconds);
+
 /* Allocation counters (no counterpart in the reference, whose arena
    reuses warm pages, alloc.c:19-23,74-140): the device and pinned-host
    allocations the library has made since it started, and the bytes its

@@ -17,17 +17,27 @@ struct FFGarner {
 };
 FFGarner ff_garner_consts(void);  // ff_crt.hip
 
-// X = x1 + x2 p1 + x3 p1 p2 from canonical residues (crt.c:33-64).
+// One reduction of x < 2q below q: x - q wraps above x when x < q.
+__device__ __forceinline__ uint32_t ff_sub_once(uint32_t x, uint32_t q) {
+    const uint32_t d = x - q;
+    return d < x ? d : x;
+}
+
+// X = x1 + x2 p1 + x3 p1 p2 from canonical residues (crt.c:33-64). The last
+// product x3 p1 p2 (31 x 62 bits) is formed from p1 p2's 32-bit halves.
 __device__ __forceinline__ u128 ff_garner(uint32_t r1, uint32_t r2, uint32_t r3, const FFGarner &G) {
     const uint32_t x1 = r1;
-    const uint32_t x1m2 = x1 >= G.p2 ? x1 - G.p2 : x1;  // p1 < 2 p2
+    const uint32_t x1m2 = ff_sub_once(x1, G.p2);  // p1 < 2 p2
     const uint32_t x2 = ff_mont_w(r2 - x1m2 + G.p2, G.i12, G.i12p, G.p2);
-    const uint32_t x1m3 = x1 >= G.p3 ? x1 - G.p3 : x1;  // p1 < 2 p3
+    const uint32_t x1m3 = ff_sub_once(x1, G.p3);  // p1 < 2 p3
     const uint32_t t3 = ff_mont_w(r3 - x1m3 + G.p3, G.i13, G.i13p, G.p3);
-    const uint32_t x2m3 = x2 >= G.p3 ? x2 - G.p3 : x2;  // p2 < 2 p3
+    const uint32_t x2m3 = ff_sub_once(x2, G.p3);  // p2 < 2 p3
     const uint32_t x3 = ff_mont_w(t3 - x2m3 + G.p3, G.i23, G.i23p, G.p3);
-    const uint64_t lo = (uint64_t)x2 * G.p1 + x1;  // < 2^62
-    return (u128)x3 * G.p12 + lo;
+    const uint64_t lo = (uint64_t)x2 * G.p1 + x1;                       // < 2^62
+    const uint64_t a = (uint64_t)x3 * (uint32_t)G.p12 + lo;             // < 2^63 + 2^62
+    const uint64_t b = (uint64_t)x3 * (uint32_t)(G.p12 >> 32);          // < 2^61
+    const uint64_t s = a + (b << 32);
+    return ((u128)((b >> 32) + (s < a ? 1u : 0u)) << 64) | s;
 }
 
 // First coefficient touching limb i: coefficient j spans bits [j bw, j bw + 93).

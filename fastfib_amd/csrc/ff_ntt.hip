@@ -510,14 +510,18 @@ __device__ __forceinline__ void inner_pair_body(const FFInner &in, const FFConv 
         (void)c;
         dst[pos] = v;
     };
-    ff_forward_io(th, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_g, lds_st);
+    // blocks of at most 2^10 words (the small quadruplings' blocks) run
+    // radix-4 rounds: a radix-16 round of 2^10 words has 64 groups for the
+    // half's 256 threads
+    const int rl = in.s <= 10 ? 2 : 4;
+    ff_forward_io(th, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_g, lds_st, FFNoTw{}, rl);
     __syncthreads();
     const int two = (C.pair & 3) == 2;
     const FFPairPoint pt{half ? C.cv[0] : C.cu[0], half ? C.cv[1] : C.cu[1], half ? C.cv[2] : C.cu[2],
                          (C.pair & 4) != 0, ff_mont(P.r2, P.r2, p, pinv)};
     const bool active = !half || two;
-    const int R = ff_nrounds(in.s);
-    const int nl0 = ff_last_nl(in.s);
+    const int R = (in.s + rl - 1) / rl;
+    const int nl0 = in.s % rl == 0 ? rl : in.s % rl;
     const uint32_t *A = lds, *B = lds + words;
     switch (nl0) {
     case 4: pair_mid_round<4>(th, active, A, B, T->irt[pi], T->irtp[pi], p, pinv, pt, R == 1, store_g); break;
@@ -529,9 +533,9 @@ __device__ __forceinline__ void inner_pair_body(const FFInner &in, const FFConv 
     int b = nl0;
     for (int r = 1; r < R; r++) {
         __syncthreads();
-        ff_round_any<FF_R_DIT>(4, false, r == R - 1, th, b, nullptr, nullptr, T->irt[pi], T->irtp[pi], nullptr, p,
+        ff_round_any<FF_R_DIT>(rl, false, r == R - 1, th, b, nullptr, nullptr, T->irt[pi], T->irtp[pi], nullptr, p,
                                pinv, lds_ld, store_g);
-        b += 4;
+        b += rl;
     }
 }
 

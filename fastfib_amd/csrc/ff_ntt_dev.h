@@ -318,15 +318,13 @@ __device__ void ff_round_any(int nl, bool from_g, bool to_g, const FFTile &t, in
 #undef FF_RCASE
 }
 
-// Round sizes: rounds of 4 levels and one of the remainder (rho % 4, or 4).
-// DIF rounds cover the row bits from the top and DIT rounds from the bottom,
-// so the remainder takes the top bits in both: the round at the lowest bits
-// (the forward's last, the inverse's first) is the one that meets HBM in
-// the outer passes and applies their twiddles, two table roots and a
-// geometric sequence per group, which 16-element groups share far better
-// than 2-element ones.
-__device__ __forceinline__ int ff_nrounds(int rho) { return (rho + 3) / 4; }
-__device__ __forceinline__ int ff_last_nl(int rho) { return rho % 4 == 0 ? 4 : rho % 4; }
+// Round sizes: rounds of rl levels (4 by default) and one of the remainder
+// (rho % rl, or rl). DIF rounds cover the row bits from the top and DIT
+// rounds from the bottom, so the remainder takes the top bits in both: the
+// round at the lowest bits (the forward's last, the inverse's first) is the
+// one that meets HBM in the outer passes and applies their twiddles, two
+// table roots and a geometric sequence per group, which 16-element groups
+// share far better than 2-element ones.
 
 // Forward (DIF) over the tile; the first round reads through ld when
 // from_g, the last writes through st when to_g.
@@ -347,17 +345,19 @@ __device__ void ff_pass_through(const FFTile &t, bool from_g, bool to_g, Ld ld, 
     }
 }
 
+// rl: levels per round (1..4); short tiles take fewer, so that more groups
+// (2^(rho - rl)) keep the threads busy.
 template <class Ld, class St, class Gt = FFNoTw>
 __device__ void ff_forward_io(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, uint32_t p, uint32_t pinv,
-                              bool from_g, bool to_g, Ld ld, St st, const Gt &gt = Gt()) {
+                              bool from_g, bool to_g, Ld ld, St st, const Gt &gt = Gt(), int rl = 4) {
     if (t.rho == 0) {
         ff_pass_through(t, from_g, to_g, ld, st, gt, p, pinv);
         return;
     }
-    const int R = ff_nrounds(t.rho);
+    const int R = (t.rho + rl - 1) / rl;
     int b = t.rho;
     for (int r = 0; r < R; r++) {
-        const int nl = r == 0 ? ff_last_nl(t.rho) : 4;
+        const int nl = r == 0 ? (t.rho % rl == 0 ? rl : t.rho % rl) : rl;
         b -= nl;
         if (r) __syncthreads();
         ff_round_any<FF_R_DIF>(nl, from_g && r == 0, to_g && r == R - 1, t, b, rt, rtp, nullptr, nullptr, nullptr,
@@ -368,15 +368,15 @@ __device__ void ff_forward_io(const FFTile &t, const uint32_t *rt, const uint32_
 // Inverse (DIT) over the tile: the rounds of ff_forward_io in reverse.
 template <class Ld, class St, class Gt = FFNoTw>
 __device__ void ff_inverse_io(const FFTile &t, const uint32_t *irt, const uint32_t *irtp, uint32_t p, uint32_t pinv,
-                              bool from_g, bool to_g, Ld ld, St st, const Gt &gt = Gt()) {
+                              bool from_g, bool to_g, Ld ld, St st, const Gt &gt = Gt(), int rl = 4) {
     if (t.rho == 0) {
         ff_pass_through(t, from_g, to_g, ld, st, gt, p, pinv);
         return;
     }
-    const int R = ff_nrounds(t.rho);
+    const int R = (t.rho + rl - 1) / rl;
     int b = 0;
     for (int r = 0; r < R; r++) {
-        const int nl = r == R - 1 ? ff_last_nl(t.rho) : 4;
+        const int nl = r == R - 1 ? (t.rho % rl == 0 ? rl : t.rho % rl) : rl;
         if (r) __syncthreads();
         ff_round_any<FF_R_DIT>(nl, from_g && r == 0, to_g && r == R - 1, t, b, nullptr, nullptr, irt, irtp, nullptr,
                                p, pinv, ld, st, gt);

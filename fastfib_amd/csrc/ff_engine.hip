@@ -373,6 +373,14 @@ static void mul_many(const DNum *const *A, const DNum *const *B, DNum *out, int 
 
 // ------------------------------------------------------------------ fast doubling
 
+// Outer tile of F(n)'s own transforms of L words (0: the library default):
+// fib_tile_words up to 2^20 words, where a step's passes are a few hundred
+// workgroups and latency-bound, so smaller tiles (more workgroups) win; the
+// longer transforms' passes keep the default (narrower rows of their
+// multi-pass plans coalesce worse: F(10^9) 5.15 -> 6.04 ms with 2048 there).
+static int fib_tile_words(uint64_t L) { return L <= (1u << 20) ? g_fib_tile_words : 0; }
+
+
 // One linear step (fib.c:78-109) in one launch from s1 = F(k)^2 and
 // s2 = F(k-1)^2, given in J as residues or limbs: (lo, hi) = bit ?
 // (F(2k), F(2k+1)) : (F(2k-1), F(2k)) on len limbs.
@@ -436,6 +444,8 @@ static void square_step(const DNum &fk, const DNum &fkm1, int k_odd, int bit, DN
             const uint64_t bb = x[j]->bits;
             const uint8_t sq1 = 1;
             ff_pick_scheds(&sh[j], &bb, &bb, &sq1, 1, 2, g_dispatch, &sc[j]);
+        } else {
+            sc[j].tile_words = fib_tile_words(sh[j].L);
         }
         sh[j] = ff_shape_with(sh[j], sc[j]);
     }
@@ -509,7 +519,8 @@ static bool pair_run(const DNum &a, const DNum &b, const ProdShape &s, int deg4,
     FFPlan pl;
     // quadruplings of few inner blocks: smaller blocks, more workgroups
     const int il = deg4 && g_quad_inner_log && s.L <= FF_QUAD_SMALL_L ? g_quad_inner_log : g_inner_log;
-    if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, il, g_tile_words) != 0 ||
+    const int tw = fib_tile_words(s.L);
+    if (ff_plan_build(&pl, (uint32_t)s.L, (int)s.bw, s.lg, s.r3, s.conv, il, tw ? tw : g_tile_words) != 0 ||
         pl.npass == 0)
         return false;
     const int two = cv != nullptr;

@@ -11,6 +11,10 @@ extern hipStream_t g_st;
 extern std::recursive_mutex g_mu;
 extern int g_inner_log;
 extern int g_tile_words;
+// outer tile of F(n)'s own steps (doubling squares under the reference rule,
+// quadruplings, pair products; 0: tile_words): their small passes are
+// latency, where smaller tiles mean more workgroups
+extern int g_fib_tile_words;
 extern uint64_t g_force_len;
 extern int g_max_lg;    // longest single transform 2^max_lg; longer products split
 extern int g_pair_steps;   // bit 0: doubling steps as weighted pairs; bit 1: the last product as one

@@ -22,6 +22,7 @@ static char g_err[512];
 
 int g_inner_log = 12;
 int g_tile_words = 4096;
+int g_fib_tile_words = 2048;
 uint64_t g_force_len = 0;
 int g_dispatch = 0;
 int g_max_lg = 25;
@@ -414,7 +415,8 @@ extern "C" int ff_get_option(const char *name, long long *v) {
                 {"max_lg", g_max_lg},             {"pair_steps", g_pair_steps},     {"quad_max_lg", g_quad_max_lg},
                 {"quad_inner_log", g_quad_inner_log}, {"host_bits", g_host_bits}, {"school_limbs", g_school_limbs},
                 {"queue_waves", g_queue_waves},   {"dec_queue", g_dec_queue},
-                {"queue_parts", g_queue_parts},   {"queue_chunk", g_queue_chunk},       {"dispatch", g_dispatch}};
+                {"queue_parts", g_queue_parts},   {"queue_chunk", g_queue_chunk},       {"dispatch", g_dispatch},
+                {"fib_tile_words", g_fib_tile_words}};
     for (const auto &o : opts) {
         if (!strcmp(name, o.name)) {
             *v = o.value;
@@ -433,6 +435,9 @@ extern "C" int ff_set_option(const char *name, long long v) {
     } else if (!strcmp(name, "tile_words")) {
         if (v < 1024 || v > 16384) return FF_ERR_ARG;
         g_tile_words = (int)v;
+    } else if (!strcmp(name, "fib_tile_words")) {
+        if (v != 0 && (v < 1024 || v > 16384)) return FF_ERR_ARG;
+        g_fib_tile_words = (int)v;
     } else if (!strcmp(name, "force_len")) {
         if (v < 0) return FF_ERR_ARG;
         g_force_len = (uint64_t)v;

@@ -17,7 +17,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfastfib_amd.so")
-SOURCES = ["ff_ntt.hip", "ff_crt.hip", "ff_limbs.hip", "ff_runtime.hip", "ff_engine.hip", "ff_batch.hip", "ff_tune.hip", "ff_mlp.hip", "ff_bigops.hip",
+SOURCES = ["ff_ntt.hip", "ff_ntt_list.hip", "ff_queue.hip", "ff_crt.hip", "ff_limbs.hip", "ff_runtime.hip", "ff_engine.hip", "ff_batch.hip", "ff_tune.hip", "ff_mlp.hip", "ff_bigops.hip",
            "ff_decimal.hip", "ff_split.hip", "ff_early.hip", "ff_noalloc.hip", "ff_probe.hip", "ff_school.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

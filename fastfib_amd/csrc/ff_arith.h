@@ -107,6 +107,20 @@ __device__ __forceinline__ uint32_t ff_mont_w(uint32_t a, uint32_t w, uint32_t w
     return ff_mont_fix(hi, __umulhi(m, p), p);
 }
 
+// (x - y) * w * 2^-32 mod p for x, y < p, with wp = w * pinv: the
+// difference d = x - y in (-p, p) enters as a signed 32-bit value (p < 2^31),
+// so it needs no "+ p". With m = lo(d * wp) taken signed, d*w - m*p is
+// divisible by 2^32 (its low halves agree) and the signed high halves give
+// r = (d*w - m*p) / 2^32 exactly, |r| < (p^2 + 2^31 p) / 2^32 < p; the
+// correction of ff_mont_fix then lands in [0, p). One instruction fewer than
+// ff_mont_w(x - y + p, w, wp, p), with the same (canonical) result.
+__device__ __forceinline__ uint32_t ff_mont_w_diff(uint32_t x, uint32_t y, uint32_t w, uint32_t wp, uint32_t p) {
+    const int32_t d = (int32_t)(x - y);
+    const int32_t hi = __mulhi(d, (int32_t)w);
+    const int32_t m = (int32_t)((uint32_t)d * wp);
+    return ff_mont_fix((uint32_t)hi, (uint32_t)__mulhi(m, (int32_t)p), p);
+}
+
 __device__ __forceinline__ uint32_t ff_add(uint32_t x, uint32_t y, uint32_t p) {
     uint32_t s = x + y;  // < 2p < 2^32
     uint32_t t = s - p;

@@ -63,6 +63,7 @@ struct FFPass {
     int r3;
     int C, logC, ld;
     int top;            // forward: read limbs; inverse: apply the CRT scale
+    int rl;             // levels per round (2..4)
     int bw;             // coefficient width (forward top pass)
     uint32_t sc[3];     // R^2 / L mod p (inverse top pass)
 };

@@ -1,0 +1,421 @@
+// Device bodies of the transform passes (outer passes, inner blocks, the
+// pair kernel), shared by the list and grid kernels (ff_ntt.hip) and the
+// batch work-queue kernel (ff_queue.hip).
+#pragma once
+
+#include "ff_internal.h"
+#include "ff_ntt_dev.h"
+
+
+#define FF_THREADS 256
+
+// Four-step twiddles of an outer pass: element (row position pos, tile
+// column c), column col = c0 + c, is scaled by w^(col * f(pos) * rootstep),
+// f the row's frequency (ff_row_freq), w the 3 * 2^25-th root; INV: by the
+// inverse root, times the CRT scale on the top pass. group() serves the 2^NL
+// consecutive rows of one butterfly group: their frequencies are
+// f0 + step * rev(j), so the twiddles are base * g^rev(j) from two table
+// roots (base = w^(col f0 rootstep), g = w^(col step rootstep)).
+template <bool INV>
+struct FFOuterTw {
+    static constexpr bool on = true;
+    static constexpr bool lds_tables = false;
+    // outer passes have no middle round
+    __device__ uint32_t mid(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) const { return ff_mont(x, y, p, pinv); }
+    const FFTables *T;
+    int pi, rho, r3;
+    bool top;
+    uint32_t c0, rootstep, scm, p, pinv;
+
+    __device__ __forceinline__ uint32_t root(uint32_t e) const {
+        if (INV) e = e ? FF_NBASE - e : 0;
+        return ff_root(T, pi, e, p, pinv);
+    }
+    __device__ __forceinline__ uint32_t one(int pos, int c) const {
+        uint32_t tw = root((c0 + c) * ff_row_freq(pos, rho, r3) * rootstep);
+        if (INV && top) tw = ff_mont(tw, scm, p, pinv);
+        return tw;
+    }
+    template <int NL>
+    __device__ __forceinline__ void group(int pos, int c, uint32_t *tw) const {
+        const uint32_t cs = (c0 + c) * rootstep;
+        const uint32_t step = (r3 ? 3u : 1u) << (rho - NL);
+        uint32_t base = root(cs * ff_row_freq(pos, rho, r3));
+        if (INV && top) base = ff_mont(base, scm, p, pinv);
+        uint32_t pw[1 << NL];
+        ff_geo<NL>(base, root(cs * step), pw, p, pinv);
+#pragma unroll
+        for (int j = 0; j < (1 << NL); j++) tw[j] = pw[ff_rev_c(j, NL)];
+    }
+};
+
+// The bw-bit coefficients of an operand of nw 32-bit words at src, reduced
+// mod p (src_load8, ntt_kernels.c:40-74, reference): element (row pos,
+// column c) of a forward top pass is coefficient pos * Sb + c0 + c.
+// Coefficient positions fit 32 bits (L <= 3 * 2^25 coefficients of at most
+// 35 bits). The three words holding a coefficient are read by buffer loads,
+// which return zero past the operand's last word, so coefficients past the
+// operand need no bounds test. The high bits of a coefficient wider than 32
+// bits add hi * 2^32 mod p, a Montgomery product by R^2.
+struct FFCoefSrc {
+    __amdgpu_buffer_rsrc_t rs;
+    uint32_t Sb, c0, lomask, himask, p, r2, r2p;
+    int bw;
+    __device__ __forceinline__ FFCoefSrc(const uint32_t *src, uint32_t nw, uint32_t Sb_, uint32_t c0_, int bw_,
+                                         const FFPrime &P)
+        : rs(__builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)(nw * 4u), 0x00020000)),
+          Sb(Sb_), c0(c0_), lomask(bw_ >= 32 ? ~0u : ((1u << bw_) - 1)),
+          himask(bw_ > 32 ? ((1u << (bw_ - 32)) - 1) : 0u), p(P.p), r2(P.r2), r2p(P.r2 * P.pinv), bw(bw_) {}
+    __device__ __forceinline__ uint32_t bit(int pos, int c) const {
+        return ((uint32_t)pos * Sb + c0 + (uint32_t)c) * (uint32_t)bw;
+    }
+    __device__ __forceinline__ void words(uint32_t bit, uint32_t *w) const {
+        const uint32_t off = (bit >> 5) << 2;
+        w[0] = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+        w[1] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 4, 0, 0);
+        w[2] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 8, 0, 0);
+    }
+    __device__ __forceinline__ uint32_t value(const uint32_t *w, uint32_t bit) const {
+        const uint32_t sh = bit & 31;
+        const uint32_t lo = __builtin_amdgcn_alignbit(w[1], w[0], sh) & lomask;
+        const uint32_t hi = __builtin_amdgcn_alignbit(w[2], w[1], sh) & himask;
+        uint32_t r = lo, q = r - p;
+        r = q < r ? q : r;
+        q = r - p;
+        r = q < r ? q : r;  // lo < 2^32 < 3p
+        if (bw > 32) r = ff_add(r, ff_mont_w(hi, r2, r2p, p), p);
+        return r;
+    }
+};
+
+// The tile of a forward top pass from the operand's limbs (FFCoefSrc): each
+// thread issues the loads of FF_UNPACK_B elements before it uses the first
+// of them.
+#define FF_UNPACK_B 8
+__device__ __forceinline__ void ff_unpack_tile(const FFTile &t, const FFCoefSrc &cs) {
+    const int n = (t.nsub << t.rho) << t.logC;
+    const int tid = ff_tile_tid(t), nthr = ff_tile_nthr(t);
+    for (int k0 = 0; k0 * nthr < n; k0 += FF_UNPACK_B) {
+        uint32_t w[FF_UNPACK_B][3], b[FF_UNPACK_B];
+#pragma unroll
+        for (int e = 0; e < FF_UNPACK_B; e++) {
+            const int i = tid + (k0 + e) * nthr;
+            b[e] = cs.bit(i >> t.logC, i & (t.C - 1));
+            // past the tile (i >= n) the loads are harmless and their values unused
+            cs.words(b[e], w[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < FF_UNPACK_B; e++) {
+            const int i = tid + (k0 + e) * nthr;
+            const uint32_t r = cs.value(w[e], b[e]);
+            if (i < n) t.s[ff_pos_addr(t, i >> t.logC, i & (t.C - 1))] = r;
+        }
+    }
+}
+
+// Outer pass over one tile of R rows x C columns (tile index `tile` among
+// the pass's nblocks * tpb) of array X, prime pi. COH: the array is read and
+// written by coherent accesses (ff_ld / ff_st).
+template <bool INV, bool TOP, bool R3, bool COH = false>
+__device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, int pi, uint32_t tile,
+                                           const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
+    const FFPrime P = PR.p[pi];
+    const uint32_t p = P.p;
+    const uint32_t blk = tile / ps.tpb, ct = tile - blk * ps.tpb;
+    const uint32_t c0 = ct << ps.logC;
+    const ff_gptr<uint32_t> data = ff_g(X.data) + (size_t)pi * ps.L + (size_t)blk * ps.Lb;
+    FFTile t{lds, ps.ld, ps.C, ps.logC, ps.rho, R3 ? 3 : 1, 0};
+    const int n = (int)ps.R << ps.logC;
+
+    const uint32_t pinv = P.pinv;
+    const uint32_t scm = (INV && TOP) ? ps.sc[pi] : 0;
+    const bool top = TOP;
+    const int bw = ps.bw;
+    const uint32_t Sb = ps.Sb, rootstep = ps.rootstep;
+    const int rho = ps.rho, r3 = R3;
+    const uint32_t nw = X.nw;
+    // element (row position pos, tile column c) <-> data[pos * Sb + c0 + c]
+    // (a top pass reads the limbs instead, through FFCoefSrc)
+    auto load_fwd = [&](int pos, int c) -> uint32_t { return ff_ld<COH>(data + ((size_t)pos * Sb + c0 + c)); };
+    // the twiddles are applied by the rounds (tw), not by these accessors
+    auto store_fwd = [&](int pos, int c, uint32_t v) { ff_st<COH>(data + ((size_t)pos * Sb + c0 + c), v); };
+    auto load_inv = [&](int pos, int c) -> uint32_t { return ff_ld<COH>(data + ((size_t)pos * Sb + c0 + c)); };
+    const FFOuterTw<INV> tw{T, pi, rho, r3, top, c0, rootstep, scm, p, pinv};
+    auto store_inv = [&](int pos, int c, uint32_t v) { ff_st<COH>(data + ((size_t)pos * Sb + c0 + c), v); };
+    auto lds_load = [&](int pos, int c) -> uint32_t { return lds[ff_pos_addr(t, pos, c)]; };
+    auto lds_store = [&](int pos, int c, uint32_t v) { lds[ff_pos_addr(t, pos, c)] = v; };
+
+    if (!INV) {
+        if (!R3 && !TOP) {
+            ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, true, true, load_fwd, store_fwd, tw, ps.rl);
+        } else {
+            // the coefficients are unpacked into the LDS tile (the radix-3
+            // layer works there). A first round reading them straight into
+            // registers through FFCoefSrc measured 12% slower.
+            ff_unpack_tile(t, FFCoefSrc(X.src, nw, Sb, c0, bw, P));
+            if (R3) ff_radix3_layer<false>(t, T, pi, P, ps.rstep3);
+            else __syncthreads();
+            ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, false, true, lds_load, store_fwd, tw, ps.rl);
+        }
+    } else {
+        if (!R3) {
+            ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, true, load_inv, store_inv, tw, ps.rl);
+        } else {
+            ff_inverse_io(t, T->irt[pi], T->irtp[pi], p, pinv, true, false, load_inv, lds_store, tw, ps.rl);
+            ff_radix3_layer<true>(t, T, pi, P, ps.rstep3);
+            for (int i = threadIdx.x; i < n; i += blockDim.x) store_inv(i >> ps.logC, i & (ps.C - 1),
+                                                                        lds_load(i >> ps.logC, i & (ps.C - 1)));
+        }
+    }
+}
+
+// The grid position of a work-list workgroup: item k owns blocks
+// [start[k], start[k+1]).
+__device__ __forceinline__ int ff_find_item(const uint32_t *start, int n, uint32_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Inner blocks: forward transform of a (and b), pointwise product, inverse,
+// all in LDS. small = 1: the whole transform, read from the limbs and
+// scaled for the CRT at the end.
+template <bool SMALL, bool R3, bool COH = false>
+__device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, int pi, uint32_t blkx,
+                                           const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
+    const FFPrime P = PR.p[pi];
+    const uint32_t p = P.p;
+    const bool square = SMALL ? (C.bsrc == nullptr) : (C.fb == nullptr);
+    const uint32_t n = SMALL ? in.L : in.S;
+    const int words = (int)(n + (n >> 4) + 16);
+    uint32_t *A = lds, *B = lds + words;
+    FFTile ta{A, 1, 1, 0, in.s, R3 ? 3 : 1, 1};
+    FFTile tb = ta;
+    tb.s = B;
+    const size_t off = (size_t)pi * in.L + (SMALL ? 0 : (size_t)blkx * in.S);
+
+    const uint32_t pinv = P.pinv;
+    const bool small = SMALL;
+    const int bw = in.bw;
+    const uint32_t sc = in.sc[pi];
+    const ff_gptr<uint32_t> fa = ff_g(C.fa), dst = fa + off;
+    const ff_gptr<const uint32_t> fb = ff_g(C.fb), asrc = ff_g(C.asrc), bsrc = ff_g(C.bsrc);
+    // element i of the block <-> limbs' coefficient i (one-block form) or
+    // the forward transform's word off + i
+    auto load_a = [&](int pos, int c) -> uint32_t {
+        (void)c;
+        return small ? ff_load_coef(asrc, C.anw, (uint64_t)pos, bw, P) : ff_ld<COH>(fa + (off + pos));
+    };
+    auto load_b = [&](int pos, int c) -> uint32_t {
+        (void)c;
+        return small ? ff_load_coef(bsrc, C.bnw, (uint64_t)pos, bw, P) : ff_ld<COH>(fb + (off + pos));
+    };
+    auto store_a = [&](int pos, int c, uint32_t v) {
+        (void)c;
+        ff_st<COH>(dst + pos, small ? ff_mont(v, sc, p, pinv) : v);
+    };
+    auto lds_a = [&](int pos, int c) -> uint32_t { return A[ff_pos_addr(ta, pos, c)]; };
+    auto lds_a_st = [&](int pos, int c, uint32_t v) { A[ff_pos_addr(ta, pos, c)] = v; };
+    auto lds_b_st = [&](int pos, int c, uint32_t v) { B[ff_pos_addr(tb, pos, c)] = v; };
+
+    if (!SMALL) {
+        if (!square) {
+            ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_b, lds_b_st);
+            __syncthreads();
+        }
+        if (C.pair == 1) {
+            // a weighted pair with one output: the middle round forms
+            // cu0 a^2 + cu1 ab + cu2 b^2 from a's group and B
+            FFPairMid pm;
+            pm.w0 = C.cu[0], pm.w1 = C.cu[1], pm.w2 = C.cu[2];
+            ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], B, p, pinv, true, true, load_a, store_a,
+                       pm);
+        } else {
+            ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, true,
+                       true, load_a, store_a);
+        }
+        return;
+    }
+    const bool bgiven = C.bspec == 1;  // B's spectrum read, not computed
+    // rounds of two levels up to 2^10 points: a radix-16 round of a short
+    // transform keeps only n / 16 of the threads busy (16 at n = 256)
+    const int small_rl = ta.rho <= 10 ? 2 : 4;
+    if (C.bspec == 2) {
+        // a's spectrum alone, stored by tile position for the products sharing it
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) lds_a_st((int)i, 0, load_a((int)i, 0));
+        if (R3) ff_radix3_layer<false>(ta, T, pi, P, in.rstep3);
+        else __syncthreads();
+        ff_forward_io(ta, T->rt[pi], T->rtp[pi], p, pinv, false, false, lds_a, lds_a_st);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) ff_st<COH>(dst + i, lds_a((int)i, 0));
+        return;
+    }
+    if (!R3) {
+        // one block read from limbs: unpack element by element into LDS
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            lds_a_st((int)i, 0, load_a((int)i, 0));
+            if (bgiven) lds_b_st((int)i, 0, ff_ld<COH>(fb + (off + i)));
+            else if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
+        }
+        __syncthreads();
+        if (!square && !bgiven) {
+            ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, false, false, lds_a, lds_b_st);
+            __syncthreads();
+        }
+        ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, false, true,
+                   lds_a, store_a, FFNoTw{}, small_rl);
+        return;
+    }
+    // 3 * 2^k in one block: the radix-3 layers work on the LDS tiles
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        lds_a_st((int)i, 0, load_a((int)i, 0));
+        if (bgiven) lds_b_st((int)i, 0, ff_ld<COH>(fb + (off + i)));
+        else if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
+    }
+    ff_radix3_layer<false>(ta, T, pi, P, in.rstep3);
+    if (!square && !bgiven) {
+        ff_radix3_layer<false>(tb, T, pi, P, in.rstep3);
+        ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, false, false, lds_a, lds_b_st);
+        __syncthreads();
+    }
+    ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, false, false,
+               lds_a, lds_a_st, FFNoTw{}, small_rl);
+    ff_radix3_layer<true>(ta, T, pi, P, in.rstep3);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) store_a((int)i, 0, lds_a((int)i, 0));
+}
+
+// Pair mode (multi-pass plans): 2 * FF_THREADS threads, the first half on
+// the block of a, the second on the block of b, each through the forward
+// levels into its LDS tile (tile A, tile B). The middle round then fuses the
+// pointwise combination with the first inverse levels: each thread of the
+// first half forms u = cu0 X + cu1 Y + cu2 Z of its group (X = a^2, Y = ab,
+// Z = b^2 pointwise), each of the second half v likewise, both from tiles A
+// and B; a barrier between those reads and the writes back keeps the two
+// halves off each other's operands. The remaining inverse rounds end in
+// HBM. With u alone the second half runs the rounds on an empty tile
+// (nsub = 0), keeping the barrier counts equal.
+//
+// Degree 4 (pair & 4): the weights apply to X = x^2, Y = xy, Z = y^2 for
+// x = a^2 + 2ab, y = a^2 + b^2 (F(2k), F(2k-1) from F(k), F(k-1)), and one
+// product by R^3 restores the single R^-1 of a plain pointwise product that
+// the inverse pass's CRT scale expects.
+
+// w x mod p for a weight w < 8.
+__device__ __forceinline__ uint32_t ff_wmul(uint32_t x, int w, uint32_t p) {
+    const uint32_t x2 = ff_add(x, x, p), x4 = ff_add(x2, x2, p);
+    uint32_t r = (w & 1) ? x : 0u;
+    if (w & 2) r = ff_add(r, x2, p);
+    if (w & 4) r = ff_add(r, x4, p);
+    return r;
+}
+
+struct FFPairPoint {
+    int w0, w1, w2;
+    int deg4;
+    uint32_t r3;  // R^3 mod p
+    __device__ uint32_t operator()(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv) const {
+        if (deg4) {
+            // x' = a (a + 2b), y' = a^2 + b^2
+            const uint32_t aa = ff_mont(x, x, p, pinv), bb = ff_mont(y, y, p, pinv);
+            const uint32_t y2 = ff_add(y, y, p);
+            x = ff_mont(x, ff_add(x, y2, p), p, pinv);
+            y = ff_add(aa, bb, p);
+        }
+        // w0 x^2 + w1 xy + w2 y^2 = x (w0 x + w1 y) + w2 y^2
+        uint32_t u = ff_mont(x, ff_add(ff_wmul(x, w0, p), ff_wmul(y, w1, p), p), p, pinv);
+        if (w2) u = ff_add(u, ff_wmul(ff_mont(y, y, p, pinv), w2, p), p);
+        if (deg4) u = ff_mont(u, r3, p, pinv);
+        return u;
+    }
+};
+
+// The middle round on row bits [0, NL) of a one-column tile: this thread's
+// output from tiles A and B, the first NL inverse levels, then (after a
+// barrier) its own tile, or HBM through st when it is the last round. Every
+// thread of the workgroup runs the same number of iterations.
+template <int NL, class St>
+__device__ __forceinline__ void pair_mid_round(const FFTile &t, bool active, const uint32_t *A, const uint32_t *B,
+                                               const uint32_t *irt, const uint32_t *irtp, uint32_t p, uint32_t pinv,
+                                               const FFPairPoint &pt, bool to_g, St st) {
+    const int G = 1 << NL;
+    const int ngroups = 1 << (t.rho - NL);
+    const int nthr = ff_tile_nthr(t);
+    const int iters = (ngroups + nthr - 1) / nthr;
+    for (int it = 0; it < iters; it++) {
+        const int g = ff_tile_tid(t) + it * nthr;
+        const bool act = active && g < ngroups;
+        uint32_t v[G];
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                const int ad = ff_pos_addr(t, (g << NL) | j, 0);
+                v[j] = pt(A[ad], B[ad], p, pinv);
+            }
+            ff_dit_group<NL, false, true>(v, 0, 0, irt, irtp, p, pinv);
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                const int pos = (g << NL) | j;
+                if (to_g) st(pos, 0, v[j]);
+                else t.s[ff_pos_addr(t, pos, 0)] = v[j];
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void inner_pair_body(const FFInner &in, const FFConv &C, int pi, uint32_t blkx,
+                                                const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
+    const FFPrime P = PR.p[pi];
+    const uint32_t p = P.p, pinv = P.pinv;
+    const uint32_t n = in.S;
+    const int words = (int)(n + (n >> 4) + 16);
+    const int half = threadIdx.x >= FF_THREADS;
+    FFTile th{lds + half * words, 1, 1, 0, in.s, 1, 1, half * FF_THREADS, FF_THREADS};
+    const size_t off = (size_t)pi * in.L + (size_t)blkx * in.S;
+    const uint32_t *src = (half ? C.fb : C.fa) + off;
+    uint32_t *dst = (half ? const_cast<uint32_t *>(C.fb) : C.fa) + off;
+    auto load_g = [&](int pos, int c) -> uint32_t {
+        (void)c;
+        return src[pos];
+    };
+    auto lds_ld = [&](int pos, int c) -> uint32_t { return th.s[ff_pos_addr(th, pos, c)]; };
+    auto lds_st = [&](int pos, int c, uint32_t v) { th.s[ff_pos_addr(th, pos, c)] = v; };
+    auto store_g = [&](int pos, int c, uint32_t v) {
+        (void)c;
+        dst[pos] = v;
+    };
+    // blocks of at most 2^10 words (the small quadruplings' blocks) run
+    // radix-4 rounds: a radix-16 round of 2^10 words has 64 groups for the
+    // half's 256 threads
+    const int rl = in.s <= 10 ? 2 : 4;
+    ff_forward_io(th, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_g, lds_st, FFNoTw{}, rl);
+    __syncthreads();
+    const int two = (C.pair & 3) == 2;
+    const FFPairPoint pt{half ? C.cv[0] : C.cu[0], half ? C.cv[1] : C.cu[1], half ? C.cv[2] : C.cu[2],
+                         (C.pair & 4) != 0, ff_mont(P.r2, P.r2, p, pinv)};
+    const bool active = !half || two;
+    const int R = (in.s + rl - 1) / rl;
+    const int nl0 = in.s % rl == 0 ? rl : in.s % rl;
+    const uint32_t *A = lds, *B = lds + words;
+    switch (nl0) {
+    case 4: pair_mid_round<4>(th, active, A, B, T->irt[pi], T->irtp[pi], p, pinv, pt, R == 1, store_g); break;
+    case 3: pair_mid_round<3>(th, active, A, B, T->irt[pi], T->irtp[pi], p, pinv, pt, R == 1, store_g); break;
+    case 2: pair_mid_round<2>(th, active, A, B, T->irt[pi], T->irtp[pi], p, pinv, pt, R == 1, store_g); break;
+    default: pair_mid_round<1>(th, active, A, B, T->irt[pi], T->irtp[pi], p, pinv, pt, R == 1, store_g); break;
+    }
+    if (!active) th.nsub = 0;
+    int b = nl0;
+    for (int r = 1; r < R; r++) {
+        __syncthreads();
+        ff_round_any<FF_R_DIT>(rl, false, r == R - 1, th, b, nullptr, nullptr, T->irt[pi], T->irtp[pi], nullptr, p,
+                               pinv, lds_ld, store_g);
+        b += rl;
+    }
+}
+

@@ -103,7 +103,7 @@ __device__ __forceinline__ void ff_dif_group(uint32_t *v, int low, int b_lo, con
             const uint32_t x = v[j], y = v[k];
             const int q = j & ((1 << lb) - 1);
             v[j] = ff_add(x, y, p);
-            v[k] = (LOW0 && q == 0) ? ff_sub(x, y, p) : ff_mont_w(x - y + p, tw[q], twp[q], p);
+            v[k] = (LOW0 && q == 0) ? ff_sub(x, y, p) : ff_mont_w_diff(x, y, tw[q], twp[q], p);
         }
     }
 }
@@ -442,7 +442,7 @@ __device__ void ff_radix3_layer(const FFTile &t, const FFTables *T, int pi, cons
         const uint32_t a0 = t.s[a0i], a1 = t.s[a1i], a2 = t.s[a2i];
         if (!INV) {
             const uint32_t y0 = ff_add(ff_add(a0, a1, p), a2, p);
-            const uint32_t d = ff_mont_w(a1 - a2 + p, P.w3, P.w3p, p);
+            const uint32_t d = ff_mont_w_diff(a1, a2, P.w3, P.w3p, p);
             const uint32_t y1 = ff_add(ff_sub(a0, a2, p), d, p);
             const uint32_t y2 = ff_sub(ff_sub(a0, a1, p), d, p);
             t.s[a0i] = y0;
@@ -450,7 +450,7 @@ __device__ void ff_radix3_layer(const FFTile &t, const FFTables *T, int pi, cons
             t.s[a2i] = ff_mont(y2, tw2, p, P.pinv);
         } else {
             const uint32_t t1 = ff_mont(a1, tw1, p, P.pinv), t2 = ff_mont(a2, tw2, p, P.pinv);
-            const uint32_t d = ff_mont_w(t1 - t2 + p, P.iw3, P.iw3p, p);
+            const uint32_t d = ff_mont_w_diff(t1, t2, P.iw3, P.iw3p, p);
             t.s[a0i] = ff_add(ff_add(a0, t1, p), t2, p);
             t.s[a1i] = ff_add(ff_sub(a0, t2, p), d, p);
             t.s[a2i] = ff_sub(ff_sub(a0, t1, p), d, p);

@@ -15,6 +15,11 @@ extern int g_tile_words;
 // quadruplings, pair products; 0: tile_words): their small passes are
 // latency, where smaller tiles mean more workgroups
 extern int g_fib_tile_words;
+// Levels per round of the outer passes: 0 = by the pass's row count 2^rho
+// (three where bit rho of g_outer_rl3 is set, else four), 2..4 = that many
+// in every outer pass.
+extern int g_outer_rl;
+extern int g_outer_rl3;
 extern uint64_t g_force_len;
 extern int g_max_lg;    // longest single transform 2^max_lg; longer products split
 extern int g_pair_steps;   // bit 0: doubling steps as weighted pairs; bit 1: the last product as one

@@ -79,6 +79,7 @@ struct FFInner {
     uint32_t rstep3;
     int bw;         // small only: coefficient width
     uint32_t sc[3]; // small only: CRT scale
+    int rl;         // levels per round of the pair kernel (k_inner_pair)
 };
 
 // A product's full schedule.

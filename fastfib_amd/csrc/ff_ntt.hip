@@ -225,6 +225,10 @@ extern "C" int ff_plan_build(FFPlan *pl, uint32_t L, int bw, int lg, int r3, uin
     const int s = k < S_log ? k : S_log;
     in.s = s;
     in.S = 1u << s;
+    // blocks of at most 2^10 words (the small quadruplings' blocks) run
+    // radix-4 rounds in the pair kernel: a radix-16 round of 2^10 words has
+    // 64 groups for the half's 256 threads
+    in.rl = g_pair_rl ? g_pair_rl : (s <= 10 ? 2 : 4);
     const int rbits = k - s;  // L / S = 3^r3 * 2^rbits
     // at most 2^10 rows per pass; the top pass carries the factor 3 and then
     // at most 2^8 power-of-2 rows; the top pass gets the smaller share

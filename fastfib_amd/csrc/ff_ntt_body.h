@@ -12,7 +12,10 @@
 // Four-step twiddles of an outer pass: element (row position pos, tile
 // column c), column col = c0 + c, is scaled by w^(col * f(pos) * rootstep),
 // f the row's frequency (ff_row_freq), w the 3 * 2^25-th root; INV: by the
-// inverse root, times the CRT scale on the top pass. group() serves the 2^NL
+// inverse root, times the CRT scale on the top pass (scm); forward top pass
+// with scm != 0: times scm * 2^-32 (2^32 mod p for scm = R^2 mod p, undoing
+// the 2^-32 of coefficients unpacked by FFCoefSrc in Montgomery-reduced
+// form; the rounds between are linear). group() serves the 2^NL
 // consecutive rows of one butterfly group: their frequencies are
 // f0 + step * rev(j), so the twiddles are base * g^rev(j) from two table
 // roots (base = w^(col f0 rootstep), g = w^(col step rootstep)).
@@ -33,7 +36,7 @@ struct FFOuterTw {
     }
     __device__ __forceinline__ uint32_t one(int pos, int c) const {
         uint32_t tw = root((c0 + c) * ff_row_freq(pos, rho, r3) * rootstep);
-        if (INV && top) tw = ff_mont(tw, scm, p, pinv);
+        if (top && (INV || scm)) tw = ff_mont(tw, scm, p, pinv);
         return tw;
     }
     template <int NL>
@@ -41,7 +44,7 @@ struct FFOuterTw {
         const uint32_t cs = (c0 + c) * rootstep;
         const uint32_t step = (r3 ? 3u : 1u) << (rho - NL);
         uint32_t base = root(cs * ff_row_freq(pos, rho, r3));
-        if (INV && top) base = ff_mont(base, scm, p, pinv);
+        if (top && (INV || scm)) base = ff_mont(base, scm, p, pinv);
         uint32_t pw[1 << NL];
         ff_geo<NL>(base, root(cs * step), pw, p, pinv);
 #pragma unroll
@@ -55,17 +58,21 @@ struct FFOuterTw {
 // Coefficient positions fit 32 bits (L <= 3 * 2^25 coefficients of at most
 // 35 bits). The three words holding a coefficient are read by buffer loads,
 // which return zero past the operand's last word, so coefficients past the
-// operand need no bounds test. The high bits of a coefficient wider than 32
-// bits add hi * 2^32 mod p, a Montgomery product by R^2.
+// operand need no bounds test. A coefficient wider than 32 bits, hi * 2^32 +
+// lo (hi < 8), is taken in Montgomery-reduced form instead, its value times
+// 2^-32: with m = lo * p^-1 mod 2^32, hi * 2^32 + lo - m p is divisible by
+// 2^32 with quotient hi - hi(m p) in (-p, 8), one correction from [0, p).
+// The forward top pass's twiddles then carry the factor 2^32 back
+// (FFOuterTw with scm = R^2 mod p).
 struct FFCoefSrc {
     __amdgpu_buffer_rsrc_t rs;
-    uint32_t Sb, c0, lomask, himask, p, r2, r2p;
+    uint32_t Sb, c0, lomask, himask, p, pinv;
     int bw;
     __device__ __forceinline__ FFCoefSrc(const uint32_t *src, uint32_t nw, uint32_t Sb_, uint32_t c0_, int bw_,
                                          const FFPrime &P)
         : rs(__builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)(nw * 4u), 0x00020000)),
           Sb(Sb_), c0(c0_), lomask(bw_ >= 32 ? ~0u : ((1u << bw_) - 1)),
-          himask(bw_ > 32 ? ((1u << (bw_ - 32)) - 1) : 0u), p(P.p), r2(P.r2), r2p(P.r2 * P.pinv), bw(bw_) {}
+          himask(bw_ > 32 ? ((1u << (bw_ - 32)) - 1) : 0u), p(P.p), pinv(P.pinv), bw(bw_) {}
     __device__ __forceinline__ uint32_t bit(int pos, int c) const {
         return ((uint32_t)pos * Sb + c0 + (uint32_t)c) * (uint32_t)bw;
     }
@@ -75,16 +82,20 @@ struct FFCoefSrc {
         w[1] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 4, 0, 0);
         w[2] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 8, 0, 0);
     }
+    // WIDE = (bw > 32)
+    template <bool WIDE>
     __device__ __forceinline__ uint32_t value(const uint32_t *w, uint32_t bit) const {
         const uint32_t sh = bit & 31;
         const uint32_t lo = __builtin_amdgcn_alignbit(w[1], w[0], sh) & lomask;
-        const uint32_t hi = __builtin_amdgcn_alignbit(w[2], w[1], sh) & himask;
+        if (WIDE) {
+            // (hi * 2^32 + lo) * 2^-32 mod p
+            const uint32_t hi = __builtin_amdgcn_alignbit(w[2], w[1], sh) & himask;
+            return ff_mont_fix(hi, __umulhi(lo * pinv, p), p);
+        }
         uint32_t r = lo, q = r - p;
         r = q < r ? q : r;
         q = r - p;
-        r = q < r ? q : r;  // lo < 2^32 < 3p
-        if (bw > 32) r = ff_add(r, ff_mont_w(hi, r2, r2p, p), p);
-        return r;
+        return q < r ? q : r;  // lo < 2^32 < 3p
     }
 };
 
@@ -92,9 +103,36 @@ struct FFCoefSrc {
 // thread issues the loads of FF_UNPACK_B elements before it uses the first
 // of them.
 #define FF_UNPACK_B 8
-__device__ __forceinline__ void ff_unpack_tile(const FFTile &t, const FFCoefSrc &cs) {
+template <bool WIDE>
+__device__ __forceinline__ void ff_unpack_tile_w(const FFTile &t, const FFCoefSrc &cs) {
     const int n = (t.nsub << t.rho) << t.logC;
     const int tid = ff_tile_tid(t), nthr = ff_tile_nthr(t);
+    if (t.C <= nthr && !t.pad) {
+        // nthr is a multiple of C (both powers of 2): a thread keeps its
+        // column and steps down rs rows per element, so the bit position and
+        // the LDS address advance by uniform strides
+        const int rs = nthr >> t.logC, rows = n >> t.logC;
+        const uint32_t dbit = (uint32_t)rs * cs.Sb * (uint32_t)cs.bw;
+        const int dadr = rs * t.ld;
+        int row = tid >> t.logC;
+        const int col = tid & (t.C - 1);
+        uint32_t bit = cs.bit(row, col);
+        int adr = row * t.ld + col;
+        for (int k0 = 0; k0 * nthr < n; k0 += FF_UNPACK_B) {
+            uint32_t w[FF_UNPACK_B][3];
+#pragma unroll
+            for (int e = 0; e < FF_UNPACK_B; e++) cs.words(bit + (uint32_t)e * dbit, w[e]);
+#pragma unroll
+            for (int e = 0; e < FF_UNPACK_B; e++) {
+                const uint32_t r = cs.value<WIDE>(w[e], bit + (uint32_t)e * dbit);
+                if (row + e * rs < rows) t.s[adr + e * dadr] = r;
+            }
+            bit += FF_UNPACK_B * dbit;
+            adr += FF_UNPACK_B * dadr;
+            row += FF_UNPACK_B * rs;
+        }
+        return;
+    }
     for (int k0 = 0; k0 * nthr < n; k0 += FF_UNPACK_B) {
         uint32_t w[FF_UNPACK_B][3], b[FF_UNPACK_B];
 #pragma unroll
@@ -107,10 +145,15 @@ __device__ __forceinline__ void ff_unpack_tile(const FFTile &t, const FFCoefSrc 
 #pragma unroll
         for (int e = 0; e < FF_UNPACK_B; e++) {
             const int i = tid + (k0 + e) * nthr;
-            const uint32_t r = cs.value(w[e], b[e]);
+            const uint32_t r = cs.value<WIDE>(w[e], b[e]);
             if (i < n) t.s[ff_pos_addr(t, i >> t.logC, i & (t.C - 1))] = r;
         }
     }
+}
+
+__device__ __forceinline__ void ff_unpack_tile(const FFTile &t, const FFCoefSrc &cs) {
+    if (cs.bw > 32) ff_unpack_tile_w<true>(t, cs);
+    else ff_unpack_tile_w<false>(t, cs);
 }
 
 // Outer pass over one tile of R rows x C columns (tile index `tile` among
@@ -128,7 +171,9 @@ __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, i
     const int n = (int)ps.R << ps.logC;
 
     const uint32_t pinv = P.pinv;
-    const uint32_t scm = (INV && TOP) ? ps.sc[pi] : 0;
+    // inverse top pass: the CRT scale; forward top pass at widths above 32
+    // bits: R^2 mod p, for the coefficients' factor 2^-32 (FFCoefSrc)
+    const uint32_t scm = TOP ? (INV ? ps.sc[pi] : (ps.bw > 32 ? P.r2 : 0u)) : 0u;
     const bool top = TOP;
     const int bw = ps.bw;
     const uint32_t Sb = ps.Sb, rootstep = ps.rootstep;
@@ -390,10 +435,7 @@ __device__ __forceinline__ void inner_pair_body(const FFInner &in, const FFConv 
         (void)c;
         dst[pos] = v;
     };
-    // blocks of at most 2^10 words (the small quadruplings' blocks) run
-    // radix-4 rounds: a radix-16 round of 2^10 words has 64 groups for the
-    // half's 256 threads
-    const int rl = in.s <= 10 ? 2 : 4;
+    const int rl = in.rl;
     ff_forward_io(th, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_g, lds_st, FFNoTw{}, rl);
     __syncthreads();
     const int two = (C.pair & 3) == 2;

@@ -20,6 +20,9 @@ extern int g_fib_tile_words;
 // in every outer pass.
 extern int g_outer_rl;
 extern int g_outer_rl3;
+// Levels per round of the pair kernel: 0 = two for blocks of at most 2^10
+// words, else four; 2..4 = that many.
+extern int g_pair_rl;
 extern uint64_t g_force_len;
 extern int g_max_lg;    // longest single transform 2^max_lg; longer products split
 extern int g_pair_steps;   // bit 0: doubling steps as weighted pairs; bit 1: the last product as one

@@ -52,6 +52,34 @@ struct FFOuterTw {
     }
 };
 
+// Row access to a block of n words of a transform array in HBM: tile
+// element (pos, c) is word pos * Sb + c0 + c (< n). Reads and writes go
+// through a buffer descriptor of the block (scalar registers: a lane
+// supplies only its 32-bit byte offset), or with COH through the coherent
+// accesses of ff_ld / ff_st. A row accessor for the rounds (ff_rows_io).
+template <bool COH>
+struct FFRows {
+    static constexpr bool rows = true;
+    ff_gptr<uint32_t> p;
+    __amdgpu_buffer_rsrc_t rs;
+    uint32_t Sb, c0;
+    __device__ __forceinline__ FFRows(ff_gptr<uint32_t> p_, uint32_t n, uint32_t Sb_, uint32_t c0_)
+        : p(p_), rs(__builtin_amdgcn_make_buffer_rsrc((void *)(uint32_t *)p_, (short)0, (int)(n * 4u), 0x00020000)),
+          Sb(Sb_), c0(c0_) {}
+    __device__ __forceinline__ uint32_t row0(int pos, int c) const { return ((uint32_t)pos * Sb + c0 + (uint32_t)c) << 2; }
+    __device__ __forceinline__ uint32_t rstep() const { return Sb << 2; }
+    __device__ __forceinline__ uint32_t ldb(uint32_t ob) const {
+        if (COH) return ff_ld<COH>(p + (ob >> 2));
+        return __builtin_amdgcn_raw_buffer_load_b32(rs, ob, 0, 0);
+    }
+    __device__ __forceinline__ void stb(uint32_t ob, uint32_t v) const {
+        if (COH) ff_st<COH>(p + (ob >> 2), v);
+        else __builtin_amdgcn_raw_buffer_store_b32(v, rs, ob, 0, 0);
+    }
+    __device__ __forceinline__ uint32_t operator()(int pos, int c) const { return ldb(row0(pos, c)); }
+    __device__ __forceinline__ void operator()(int pos, int c, uint32_t v) const { stb(row0(pos, c), v); }
+};
+
 // The bw-bit coefficients of an operand of nw 32-bit words at src, reduced
 // mod p (src_load8, ntt_kernels.c:40-74, reference): element (row pos,
 // column c) of a forward top pass is coefficient pos * Sb + c0 + c.
@@ -181,12 +209,11 @@ __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, i
     const uint32_t nw = X.nw;
     // element (row position pos, tile column c) <-> data[pos * Sb + c0 + c]
     // (a top pass reads the limbs instead, through FFCoefSrc)
-    auto load_fwd = [&](int pos, int c) -> uint32_t { return ff_ld<COH>(data + ((size_t)pos * Sb + c0 + c)); };
-    // the twiddles are applied by the rounds (tw), not by these accessors
-    auto store_fwd = [&](int pos, int c, uint32_t v) { ff_st<COH>(data + ((size_t)pos * Sb + c0 + c), v); };
-    auto load_inv = [&](int pos, int c) -> uint32_t { return ff_ld<COH>(data + ((size_t)pos * Sb + c0 + c)); };
+    // (offsets inside the block fit 32 bits: Lb <= 3 * 2^25); the twiddles
+    // are applied by the rounds (tw), not by these accessors
+    const FFRows<COH> rows(data, ps.Lb, Sb, c0);
+    const FFRows<COH> &load_fwd = rows, &store_fwd = rows, &load_inv = rows, &store_inv = rows;
     const FFOuterTw<INV> tw{T, pi, rho, r3, top, c0, rootstep, scm, p, pinv};
-    auto store_inv = [&](int pos, int c, uint32_t v) { ff_st<COH>(data + ((size_t)pos * Sb + c0 + c), v); };
     auto lds_load = [&](int pos, int c) -> uint32_t { return lds[ff_pos_addr(t, pos, c)]; };
     auto lds_store = [&](int pos, int c, uint32_t v) { lds[ff_pos_addr(t, pos, c)] = v; };
 
@@ -249,27 +276,32 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
     const uint32_t sc = in.sc[pi];
     const ff_gptr<uint32_t> fa = ff_g(C.fa), dst = fa + off;
     const ff_gptr<const uint32_t> fb = ff_g(C.fb), asrc = ff_g(C.asrc), bsrc = ff_g(C.bsrc);
+    const ff_gptr<const uint32_t> fbo = C.fb ? fb + off : fb;
     // element i of the block <-> limbs' coefficient i (one-block form) or
-    // the forward transform's word off + i
+    // the forward transform's word off + i: a uniform base plus a 32-bit
+    // lane offset
     auto load_a = [&](int pos, int c) -> uint32_t {
         (void)c;
-        return small ? ff_load_coef(asrc, C.anw, (uint64_t)pos, bw, P) : ff_ld<COH>(fa + (off + pos));
+        return small ? ff_load_coef(asrc, C.anw, (uint64_t)pos, bw, P) : ff_ld<COH>(dst + (uint32_t)pos);
     };
     auto load_b = [&](int pos, int c) -> uint32_t {
         (void)c;
-        return small ? ff_load_coef(bsrc, C.bnw, (uint64_t)pos, bw, P) : ff_ld<COH>(fb + (off + pos));
+        return small ? ff_load_coef(bsrc, C.bnw, (uint64_t)pos, bw, P) : ff_ld<COH>(fbo + (uint32_t)pos);
     };
     auto store_a = [&](int pos, int c, uint32_t v) {
         (void)c;
-        ff_st<COH>(dst + pos, small ? ff_mont(v, sc, p, pinv) : v);
+        ff_st<COH>(dst + (uint32_t)pos, small ? ff_mont(v, sc, p, pinv) : v);
     };
     auto lds_a = [&](int pos, int c) -> uint32_t { return A[ff_pos_addr(ta, pos, c)]; };
     auto lds_a_st = [&](int pos, int c, uint32_t v) { A[ff_pos_addr(ta, pos, c)] = v; };
     auto lds_b_st = [&](int pos, int c, uint32_t v) { B[ff_pos_addr(tb, pos, c)] = v; };
 
     if (!SMALL) {
+        // the block's words as rows of one word (row accessors for the rounds)
+        const FFRows<COH> ra(dst, n, 1, 0);
         if (!square) {
-            ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, true, false, load_b, lds_b_st);
+            const FFRows<COH> rb(ff_gptr<uint32_t>(const_cast<uint32_t *>((const uint32_t *)fbo)), n, 1, 0);
+            ff_forward_io(tb, T->rt[pi], T->rtp[pi], p, pinv, true, false, rb, lds_b_st);
             __syncthreads();
         }
         if (C.pair == 1) {
@@ -277,11 +309,10 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
             // cu0 a^2 + cu1 ab + cu2 b^2 from a's group and B
             FFPairMid pm;
             pm.w0 = C.cu[0], pm.w1 = C.cu[1], pm.w2 = C.cu[2];
-            ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], B, p, pinv, true, true, load_a, store_a,
-                       pm);
+            ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], B, p, pinv, true, true, ra, ra, pm);
         } else {
             ff_conv_io(ta, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, true,
-                       true, load_a, store_a);
+                       true, ra, ra);
         }
         return;
     }
@@ -303,7 +334,7 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
         // one block read from limbs: unpack element by element into LDS
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
             lds_a_st((int)i, 0, load_a((int)i, 0));
-            if (bgiven) lds_b_st((int)i, 0, ff_ld<COH>(fb + (off + i)));
+            if (bgiven) lds_b_st((int)i, 0, ff_ld<COH>(fbo + i));
             else if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
         }
         __syncthreads();
@@ -318,7 +349,7 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
     // 3 * 2^k in one block: the radix-3 layers work on the LDS tiles
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         lds_a_st((int)i, 0, load_a((int)i, 0));
-        if (bgiven) lds_b_st((int)i, 0, ff_ld<COH>(fb + (off + i)));
+        if (bgiven) lds_b_st((int)i, 0, ff_ld<COH>(fbo + i));
         else if (!square) lds_b_st((int)i, 0, load_b((int)i, 0));
     }
     ff_radix3_layer<false>(ta, T, pi, P, in.rstep3);

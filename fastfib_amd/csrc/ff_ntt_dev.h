@@ -174,6 +174,17 @@ __device__ __forceinline__ int ff_tile_addr(const FFTile &t, int sub, int row, i
 // round, which cover the same row bits and therefore the same groups.
 enum { FF_R_DIF = 0, FF_R_DIT = 1, FF_R_MID = 2 };
 
+// Loaders and storers of the rounds are callables ld(pos, c) -> value and
+// st(pos, c, value). A row accessor (ff_rows_io: a member `rows`) also
+// gives the byte offset of element (pos, c), row0(pos, c), and the byte
+// stride of one row, rstep(), with ldb / stb at byte offsets: a group's
+// elements then cost one addition each (offsets advance by uniform
+// strides) instead of an address computation.
+template <class T, class = void>
+struct ff_rows_io : std::false_type {};
+template <class T>
+struct ff_rows_io<T, std::void_t<decltype(T::rows)>> : std::true_type {};
+
 // Twiddles of the outer four-step passes, applied in the round that touches
 // HBM (the forward's last, the inverse's first). Those rounds cover the
 // lowest row bits, so a group holds 2^NL consecutive rows, whose twiddles
@@ -257,8 +268,14 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
         uint32_t v[G];
 #pragma unroll
         for (int j = 0; j < G; j++) {
-            const int pos = pos0 + (base | (j << b_lo));
-            v[j] = FROM_G ? ld(pos, c) : t.s[a0 + off[j]];
+            // base has no bits in [b_lo, b_lo + NL): the sum is the OR
+            const int pos = pos0 + base + (j << b_lo);
+            if constexpr (FROM_G && ff_rows_io<Ld>::value) {
+                // row j at the group's first byte offset plus j uniform strides
+                v[j] = ld.ldb(ld.row0(pos0 + base, c) + (uint32_t)j * (ld.rstep() << b_lo));
+            } else {
+                v[j] = FROM_G ? ld(pos, c) : t.s[a0 + off[j]];
+            }
         }
         if constexpr (Gt::on && KIND == FF_R_DIT && FROM_G) {
             // b_lo = 0 here: rows pos0 + base + j
@@ -290,9 +307,13 @@ __device__ __forceinline__ void ff_round_io(const FFTile &t, int b_lo, const uin
         }
 #pragma unroll
         for (int j = 0; j < G; j++) {
-            const int pos = pos0 + (base | (j << b_lo));
-            if (TO_G) st(pos, c, v[j]);
-            else t.s[a0 + off[j]] = v[j];
+            const int pos = pos0 + base + (j << b_lo);
+            if constexpr (TO_G && ff_rows_io<St>::value) {
+                st.stb(st.row0(pos0 + base, c) + (uint32_t)j * (st.rstep() << b_lo), v[j]);
+            } else {
+                if (TO_G) st(pos, c, v[j]);
+                else t.s[a0 + off[j]] = v[j];
+            }
         }
     }
 }

@@ -296,6 +296,24 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
     auto lds_a_st = [&](int pos, int c, uint32_t v) { A[ff_pos_addr(ta, pos, c)] = v; };
     auto lds_b_st = [&](int pos, int c, uint32_t v) { B[ff_pos_addr(tb, pos, c)] = v; };
 
+    if (!SMALL && in.s == 12) {
+        // the common block of 2^12 words: its geometry as literals
+        const FFTile a12{A, 1, 1, 0, 12, 1, 1}, b12{B, 1, 1, 0, 12, 1, 1};
+        const FFRows<COH> ra(dst, 4096, 1, 0);
+        if (!square) {
+            const FFRows<COH> rb(ff_gptr<uint32_t>(const_cast<uint32_t *>((const uint32_t *)fbo)), 4096, 1, 0);
+            ff_forward12_io(b12, T->rt[pi], T->rtp[pi], p, pinv, rb);
+            __syncthreads();
+        }
+        if (C.pair == 1) {
+            FFPairMid pm;
+            pm.w0 = C.cu[0], pm.w1 = C.cu[1], pm.w2 = C.cu[2];
+            ff_conv12_io(a12, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], B, p, pinv, ra, ra, pm);
+        } else {
+            ff_conv12_io(a12, T->rt[pi], T->rtp[pi], T->irt[pi], T->irtp[pi], square ? nullptr : B, p, pinv, ra, ra);
+        }
+        return;
+    }
     if (!SMALL) {
         // the block's words as rows of one word (row accessors for the rounds)
         const FFRows<COH> ra(dst, n, 1, 0);

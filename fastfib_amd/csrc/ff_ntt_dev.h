@@ -435,6 +435,38 @@ __device__ void ff_conv_io(const FFTile &t, const uint32_t *rt, const uint32_t *
     }
 }
 
+// The same for a block of 2^12 words (a one-column tile, rho = 12, built by
+// the caller from literals): rounds of four levels at row bits 8, 4, 0, 4, 8
+// written out, so that with everything inlined each round's LDS offsets and
+// twiddle offsets are compile-time constants (immediate offsets of the LDS
+// and buffer instructions). The first round reads through ld, the last
+// writes through st.
+template <class Ld, class St, class Gt = FFNoTw>
+__device__ __forceinline__ void ff_conv12_io(const FFTile &t, const uint32_t *rt, const uint32_t *rtp,
+                                             const uint32_t *irt, const uint32_t *irtp, const uint32_t *B,
+                                             uint32_t p, uint32_t pinv, Ld ld, St st, const Gt &gt = Gt()) {
+    ff_round_io<4, FF_R_DIF, true, false>(t, 8, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, st, gt);
+    __syncthreads();
+    ff_round_io<4, FF_R_DIF, false, false>(t, 4, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, st, gt);
+    __syncthreads();
+    ff_round_io<4, FF_R_MID, false, false>(t, 0, rt, rtp, irt, irtp, B, p, pinv, ld, st, gt);
+    __syncthreads();
+    ff_round_io<4, FF_R_DIT, false, false>(t, 4, nullptr, nullptr, irt, irtp, nullptr, p, pinv, ld, st, gt);
+    __syncthreads();
+    ff_round_io<4, FF_R_DIT, false, true>(t, 8, nullptr, nullptr, irt, irtp, nullptr, p, pinv, ld, st, gt);
+}
+
+// Forward transform of a 2^12-word block into its LDS tile (as ff_conv12_io).
+template <class Ld>
+__device__ __forceinline__ void ff_forward12_io(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, uint32_t p,
+                                                uint32_t pinv, Ld ld) {
+    auto none = [](int, int, uint32_t) {};
+    ff_round_io<4, FF_R_DIF, true, false>(t, 8, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, none);
+    __syncthreads();
+    ff_round_io<4, FF_R_DIF, false, false>(t, 4, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, none);
+    __syncthreads();
+    ff_round_io<4, FF_R_DIF, false, false>(t, 0, rt, rtp, nullptr, nullptr, nullptr, p, pinv, ld, none);
+}
 
 // Radix-3 layer over a tile of nsub = 3 stacked thirds (ntt_radix3.c:8-22,
 // reference): forward, for n1 < M with a_r = (third r, row n1):

@@ -120,12 +120,41 @@ extern "C" int ff_tables_init(void) {
 
 #include "ff_ntt_body.h"
 
-template <bool INV, bool TOP, bool R3>
+template <bool INV, bool TOP, bool R3, int SH = 0>
 __global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, const FFTables *T,
                                                       FFPrimes PR) {
     extern __shared__ uint32_t lds[];
     const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
-    outer_body<INV, TOP, R3>(ps, xs.x[item], pi, blockIdx.x, T, PR, lds);
+    outer_body<INV, TOP, R3, false, SH>(ps, xs.x[item], pi, blockIdx.x, T, PR, lds);
+}
+
+// The fixed-geometry shape of a pass (ff_shape), 0 for none.
+static int outer_shape(const FFPass &ps) {
+    if (ps.r3) return 0;
+    for (int sh = 1; sh <= FF_NSHAPES; sh++) {
+        const FFShape s = ff_shape(sh);
+        if (ps.rho == s.rho && ps.logC == s.logC && ps.ld == s.ld && ps.rl == s.rl) return sh;
+    }
+    return 0;
+}
+
+// A top pass (INV: inverse) of a power-of-2 length, by its shape.
+template <bool INV>
+static void launch_top(const FFPass &ps, const FFXformSet *xs, dim3 grid, hipStream_t st, const FFTables *T,
+                       FFPrimes PR) {
+    const size_t lds = (size_t)ps.R * ps.ld * sizeof(uint32_t);
+    switch (outer_shape(ps)) {
+#define FF_SHCASE(SH)                                                                                            \
+    case SH: hipLaunchKernelGGL((k_outer<INV, true, false, SH>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR); \
+        break;
+        FF_SHCASE(1)
+        FF_SHCASE(2)
+        FF_SHCASE(3)
+        FF_SHCASE(4)
+        FF_SHCASE(5)
+#undef FF_SHCASE
+    default: hipLaunchKernelGGL((k_outer<INV, true, false>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR);
+    }
 }
 
 
@@ -282,8 +311,7 @@ extern "C" hipError_t ff_launch_forward(const FFPlan *pl, const FFXformSet *xs, 
             hipLaunchKernelGGL((k_outer<false, true, true>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
                                g_tables, g_primes_arg);
         else if (i == 0)
-            hipLaunchKernelGGL((k_outer<false, true, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
-                               g_tables, g_primes_arg);
+            launch_top<false>(ps, xs, grid, st, g_tables, g_primes_arg);
         else
             hipLaunchKernelGGL((k_outer<false, false, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
                                g_tables, g_primes_arg);
@@ -368,8 +396,7 @@ extern "C" hipError_t ff_launch_inverse(const FFPlan *pl, const FFXformSet *xs, 
             hipLaunchKernelGGL((k_outer<true, true, true>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
                                g_tables, g_primes_arg);
         else if (i == 0)
-            hipLaunchKernelGGL((k_outer<true, true, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
-                               g_tables, g_primes_arg);
+            launch_top<true>(ps, xs, grid, st, g_tables, g_primes_arg);
         else
             hipLaunchKernelGGL((k_outer<true, false, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
                                g_tables, g_primes_arg);

@@ -184,10 +184,28 @@ __device__ __forceinline__ void ff_unpack_tile(const FFTile &t, const FFCoefSrc 
     else ff_unpack_tile_w<false>(t, cs);
 }
 
+// Tile shapes of the outer passes with a fixed-geometry path (rows 2^rho,
+// 2^logC columns, row stride ld words, rounds of rl levels): the top passes
+// of products of 2^17 .. 2^21 words per prime, as fast doubling plans them.
+// Shape 0: the general path.
+struct FFShape {
+    int rho, logC, ld, rl;
+};
+#define FF_NSHAPES 5
+__host__ __device__ constexpr FFShape ff_shape(int sh) {
+    return sh == 1 ? FFShape{5, 6, 64, 3}
+         : sh == 2 ? FFShape{6, 5, 33, 3}
+         : sh == 3 ? FFShape{7, 4, 17, 4}
+         : sh == 4 ? FFShape{8, 4, 17, 4}
+         : sh == 5 ? FFShape{9, 4, 17, 3}
+                   : FFShape{0, 0, 0, 0};
+}
+
 // Outer pass over one tile of R rows x C columns (tile index `tile` among
 // the pass's nblocks * tpb) of array X, prime pi. COH: the array is read and
-// written by coherent accesses (ff_ld / ff_st).
-template <bool INV, bool TOP, bool R3, bool COH = false>
+// written by coherent accesses (ff_ld / ff_st). SH > 0: the pass has tile
+// shape ff_shape(SH) (no radix-3 layer), its rounds at literal positions.
+template <bool INV, bool TOP, bool R3, bool COH = false, int SH = 0>
 __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, int pi, uint32_t tile,
                                            const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
     const FFPrime P = PR.p[pi];
@@ -217,6 +235,26 @@ __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, i
     auto lds_load = [&](int pos, int c) -> uint32_t { return lds[ff_pos_addr(t, pos, c)]; };
     auto lds_store = [&](int pos, int c, uint32_t v) { lds[ff_pos_addr(t, pos, c)] = v; };
 
+    if constexpr (SH > 0 && !R3) {
+        constexpr FFShape S = ff_shape(SH);
+        const FFTile tf{lds, S.ld, 1 << S.logC, S.logC, S.rho, 1, 0};
+        const FFOuterTw<INV> twf{T, pi, S.rho, 0, top, c0, rootstep, scm, p, pinv};
+        if constexpr (!INV) {
+            if constexpr (TOP) {
+                ff_unpack_tile(tf, FFCoefSrc(X.src, nw, Sb, c0, bw, P));
+                __syncthreads();
+                ff_forward_fixed<S.rho, S.rl, 0, false, true>(tf, T->rt[pi], T->rtp[pi], p, pinv, lds_load, store_fwd,
+                                                             twf);
+            } else {
+                ff_forward_fixed<S.rho, S.rl, 0, true, true>(tf, T->rt[pi], T->rtp[pi], p, pinv, load_fwd, store_fwd,
+                                                            twf);
+            }
+        } else {
+            ff_inverse_fixed<S.rho, S.rl, 0, true, true>(tf, T->irt[pi], T->irtp[pi], p, pinv, load_inv, store_inv,
+                                                        twf);
+        }
+        return;
+    }
     if (!INV) {
         if (!R3 && !TOP) {
             ff_forward_io(t, T->rt[pi], T->rtp[pi], p, pinv, true, true, load_fwd, store_fwd, tw, ps.rl);

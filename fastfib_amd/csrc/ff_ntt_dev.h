@@ -456,6 +456,36 @@ __device__ __forceinline__ void ff_conv12_io(const FFTile &t, const uint32_t *rt
     ff_round_io<4, FF_R_DIT, false, true>(t, 8, nullptr, nullptr, irt, irtp, nullptr, p, pinv, ld, st, gt);
 }
 
+// ff_forward_io / ff_inverse_io with the row count 2^RHO and the round size
+// RL as template parameters: round R covers literal row bits, so that with
+// a tile built from literals (and everything inlined) each round's LDS and
+// twiddle offsets are compile-time constants.
+template <int RHO, int RL, int R, bool FROM_G, bool TO_G, class Ld, class St, class Gt>
+__device__ __forceinline__ void ff_forward_fixed(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, uint32_t p,
+                                                 uint32_t pinv, Ld ld, St st, const Gt &gt) {
+    constexpr int NR = (RHO + RL - 1) / RL;
+    constexpr int NL0 = RHO % RL ? RHO % RL : RL;  // the remainder takes the top bits
+    constexpr int NL = R == 0 ? NL0 : RL;
+    constexpr int B = RHO - NL0 - R * RL;
+    if constexpr (R > 0) __syncthreads();
+    ff_round_io<NL, FF_R_DIF, FROM_G && R == 0, TO_G && R == NR - 1>(t, B, rt, rtp, nullptr, nullptr, nullptr, p,
+                                                                      pinv, ld, st, gt);
+    if constexpr (R + 1 < NR) ff_forward_fixed<RHO, RL, R + 1, FROM_G, TO_G>(t, rt, rtp, p, pinv, ld, st, gt);
+}
+
+template <int RHO, int RL, int R, bool FROM_G, bool TO_G, class Ld, class St, class Gt>
+__device__ __forceinline__ void ff_inverse_fixed(const FFTile &t, const uint32_t *irt, const uint32_t *irtp,
+                                                 uint32_t p, uint32_t pinv, Ld ld, St st, const Gt &gt) {
+    constexpr int NR = (RHO + RL - 1) / RL;
+    constexpr int NL0 = RHO % RL ? RHO % RL : RL;  // the last round takes the top bits
+    constexpr int NL = R == NR - 1 ? NL0 : RL;
+    constexpr int B = R * RL;
+    if constexpr (R > 0) __syncthreads();
+    ff_round_io<NL, FF_R_DIT, FROM_G && R == 0, TO_G && R == NR - 1>(t, B, nullptr, nullptr, irt, irtp, nullptr, p,
+                                                                      pinv, ld, st, gt);
+    if constexpr (R + 1 < NR) ff_inverse_fixed<RHO, RL, R + 1, FROM_G, TO_G>(t, irt, irtp, p, pinv, ld, st, gt);
+}
+
 // Forward transform of a 2^12-word block into its LDS tile (as ff_conv12_io).
 template <class Ld>
 __device__ __forceinline__ void ff_forward12_io(const FFTile &t, const uint32_t *rt, const uint32_t *rtp, uint32_t p,

@@ -130,31 +130,52 @@ __global__ __launch_bounds__(FF_THREADS) void k_outer(FFPass ps, FFXformSet xs, 
 
 // The fixed-geometry shape of a pass (ff_shape), 0 for none.
 static int outer_shape(const FFPass &ps) {
-    if (ps.r3) return 0;
     for (int sh = 1; sh <= FF_NSHAPES; sh++) {
         const FFShape s = ff_shape(sh);
-        if (ps.rho == s.rho && ps.logC == s.logC && ps.ld == s.ld && ps.rl == s.rl) return sh;
+        if (ps.rho == s.rho && ps.logC == s.logC && ps.ld == s.ld && ps.rl == s.rl && ps.r3 == s.r3) return sh;
     }
     return 0;
 }
 
-// A top pass (INV: inverse) of a power-of-2 length, by its shape.
-template <bool INV>
-static void launch_top(const FFPass &ps, const FFXformSet *xs, dim3 grid, hipStream_t st, const FFTables *T,
-                       FFPrimes PR) {
+// An outer pass (INV: inverse; TOP: the top pass), by its shape: the
+// fixed-geometry kernel where one is instantiated for it, else the general
+// one.
+template <bool INV, bool TOP>
+static void launch_outer(const FFPass &ps, const FFXformSet *xs, dim3 grid, hipStream_t st, const FFTables *T,
+                         FFPrimes PR) {
     const size_t lds = (size_t)ps.R * ps.ld * sizeof(uint32_t);
-    switch (outer_shape(ps)) {
-#define FF_SHCASE(SH)                                                                                            \
-    case SH: hipLaunchKernelGGL((k_outer<INV, true, false, SH>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR); \
-        break;
-        FF_SHCASE(1)
-        FF_SHCASE(2)
-        FF_SHCASE(3)
-        FF_SHCASE(4)
-        FF_SHCASE(5)
-#undef FF_SHCASE
-    default: hipLaunchKernelGGL((k_outer<INV, true, false>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR);
+#define FF_SHCASE(R3, SH)                                                                                       \
+    case SH: hipLaunchKernelGGL((k_outer<INV, TOP, R3, SH>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR); \
+        return;
+    const int sh = outer_shape(ps);
+    if constexpr (TOP) {
+        if (ps.r3) {
+            switch (sh) {
+                FF_SHCASE(true, 9)
+                FF_SHCASE(true, 10)
+            }
+            hipLaunchKernelGGL((k_outer<INV, TOP, true>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR);
+            return;
+        }
+        switch (sh) {
+            FF_SHCASE(false, 1)
+            FF_SHCASE(false, 2)
+            FF_SHCASE(false, 3)
+            FF_SHCASE(false, 4)
+            FF_SHCASE(false, 5)
+            FF_SHCASE(false, 6)
+            FF_SHCASE(false, 7)
+            FF_SHCASE(false, 8)
+        }
+        hipLaunchKernelGGL((k_outer<INV, TOP, false>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR);
+    } else {
+        switch (sh) {
+            FF_SHCASE(false, 6)
+            FF_SHCASE(false, 7)
+        }
+        hipLaunchKernelGGL((k_outer<INV, TOP, false>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR);
     }
+#undef FF_SHCASE
 }
 
 
@@ -307,14 +328,8 @@ extern "C" hipError_t ff_launch_forward(const FFPlan *pl, const FFXformSet *xs, 
         dim3 grid(ps.nblocks * ps.tpb, 3 * nx);
         const int kind = i == 0 ? FK_FWD_TOP : FK_FWD;
         ff_prof_begin(kind, st);
-        if (i == 0 && ps.r3)
-            hipLaunchKernelGGL((k_outer<false, true, true>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
-                               g_tables, g_primes_arg);
-        else if (i == 0)
-            launch_top<false>(ps, xs, grid, st, g_tables, g_primes_arg);
-        else
-            hipLaunchKernelGGL((k_outer<false, false, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
-                               g_tables, g_primes_arg);
+        if (i == 0) launch_outer<false, true>(ps, xs, grid, st, g_tables, g_primes_arg);
+        else launch_outer<false, false>(ps, xs, grid, st, g_tables, g_primes_arg);
         const double L4 = 4.0 * pl->L;
         ff_prof_end(kind, i == 0 ? 3.0 * (operand_bytes + nx * L4) : 3.0 * nx * 2 * L4, st);
     }
@@ -392,14 +407,8 @@ extern "C" hipError_t ff_launch_inverse(const FFPlan *pl, const FFXformSet *xs, 
         dim3 grid(ps.nblocks * ps.tpb, 3 * nx);
         const int kind = i == 0 ? FK_INV_TOP : FK_INV;
         ff_prof_begin(kind, st);
-        if (i == 0 && ps.r3)
-            hipLaunchKernelGGL((k_outer<true, true, true>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
-                               g_tables, g_primes_arg);
-        else if (i == 0)
-            launch_top<true>(ps, xs, grid, st, g_tables, g_primes_arg);
-        else
-            hipLaunchKernelGGL((k_outer<true, false, false>), grid, dim3(FF_THREADS), outer_lds(ps), st, ps, *xs,
-                               g_tables, g_primes_arg);
+        if (i == 0) launch_outer<true, true>(ps, xs, grid, st, g_tables, g_primes_arg);
+        else launch_outer<true, false>(ps, xs, grid, st, g_tables, g_primes_arg);
         ff_prof_end(kind, 3.0 * nx * 8.0 * pl->L, st);
     }
     return hipGetLastError();

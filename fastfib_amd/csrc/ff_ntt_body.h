@@ -185,26 +185,34 @@ __device__ __forceinline__ void ff_unpack_tile(const FFTile &t, const FFCoefSrc 
 }
 
 // Tile shapes of the outer passes with a fixed-geometry path (rows 2^rho,
-// 2^logC columns, row stride ld words, rounds of rl levels): the top passes
-// of products of 2^17 .. 2^21 words per prime, as fast doubling plans them.
+// times 3 with r3, 2^logC columns, row stride ld words, rounds of rl
+// levels), as the plans of products of 2^17 .. 3 * 2^24 words per prime
+// give them: top passes of 2^17 .. 2^21 words (shapes 1-5, fast doubling's
+// tiles), of 2^22 .. 2^25 (6-8) and of 3 * 2^21 .. 3 * 2^23 (9, 10, with
+// the radix-3 layer); second passes of 3 * 2^21 .. 3 * 2^24 words (6, 7).
 // Shape 0: the general path.
 struct FFShape {
-    int rho, logC, ld, rl;
+    int rho, logC, ld, rl, r3;
 };
-#define FF_NSHAPES 5
+#define FF_NSHAPES 10
 __host__ __device__ constexpr FFShape ff_shape(int sh) {
-    return sh == 1 ? FFShape{5, 6, 64, 3}
-         : sh == 2 ? FFShape{6, 5, 33, 3}
-         : sh == 3 ? FFShape{7, 4, 17, 4}
-         : sh == 4 ? FFShape{8, 4, 17, 4}
-         : sh == 5 ? FFShape{9, 4, 17, 3}
-                   : FFShape{0, 0, 0, 0};
+    return sh == 1    ? FFShape{5, 6, 64, 3, 0}
+           : sh == 2  ? FFShape{6, 5, 33, 3, 0}
+           : sh == 3  ? FFShape{7, 4, 17, 4, 0}
+           : sh == 4  ? FFShape{8, 4, 17, 4, 0}
+           : sh == 5  ? FFShape{9, 4, 17, 3, 0}
+           : sh == 6  ? FFShape{6, 6, 64, 3, 0}
+           : sh == 7  ? FFShape{5, 7, 128, 3, 0}
+           : sh == 8  ? FFShape{10, 4, 16, 4, 0}
+           : sh == 9  ? FFShape{5, 5, 33, 3, 1}
+           : sh == 10 ? FFShape{4, 6, 64, 4, 1}
+                      : FFShape{0, 0, 0, 0, 0};
 }
 
 // Outer pass over one tile of R rows x C columns (tile index `tile` among
 // the pass's nblocks * tpb) of array X, prime pi. COH: the array is read and
 // written by coherent accesses (ff_ld / ff_st). SH > 0: the pass has tile
-// shape ff_shape(SH) (no radix-3 layer), its rounds at literal positions.
+// shape ff_shape(SH), its rounds at literal positions.
 template <bool INV, bool TOP, bool R3, bool COH = false, int SH = 0>
 __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, int pi, uint32_t tile,
                                            const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
@@ -235,23 +243,32 @@ __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, i
     auto lds_load = [&](int pos, int c) -> uint32_t { return lds[ff_pos_addr(t, pos, c)]; };
     auto lds_store = [&](int pos, int c, uint32_t v) { lds[ff_pos_addr(t, pos, c)] = v; };
 
-    if constexpr (SH > 0 && !R3) {
+    if constexpr (SH > 0) {
         constexpr FFShape S = ff_shape(SH);
-        const FFTile tf{lds, S.ld, 1 << S.logC, S.logC, S.rho, 1, 0};
-        const FFOuterTw<INV> twf{T, pi, S.rho, 0, top, c0, rootstep, scm, p, pinv};
+        static_assert((S.r3 != 0) == R3 && (!R3 || TOP), "shape and kernel agree");
+        const FFTile tf{lds, S.ld, 1 << S.logC, S.logC, S.rho, R3 ? 3 : 1, 0};
+        const FFOuterTw<INV> twf{T, pi, S.rho, R3, top, c0, rootstep, scm, p, pinv};
         if constexpr (!INV) {
             if constexpr (TOP) {
                 ff_unpack_tile(tf, FFCoefSrc(X.src, nw, Sb, c0, bw, P));
-                __syncthreads();
+                if constexpr (R3) ff_radix3_layer<false>(tf, T, pi, P, ps.rstep3);
+                else __syncthreads();
                 ff_forward_fixed<S.rho, S.rl, 0, false, true>(tf, T->rt[pi], T->rtp[pi], p, pinv, lds_load, store_fwd,
                                                              twf);
             } else {
                 ff_forward_fixed<S.rho, S.rl, 0, true, true>(tf, T->rt[pi], T->rtp[pi], p, pinv, load_fwd, store_fwd,
                                                             twf);
             }
-        } else {
+        } else if constexpr (!R3) {
             ff_inverse_fixed<S.rho, S.rl, 0, true, true>(tf, T->irt[pi], T->irtp[pi], p, pinv, load_inv, store_inv,
                                                         twf);
+        } else {
+            ff_inverse_fixed<S.rho, S.rl, 0, true, false>(tf, T->irt[pi], T->irtp[pi], p, pinv, load_inv, lds_store,
+                                                         twf);
+            ff_radix3_layer<true>(tf, T, pi, P, ps.rstep3);
+            constexpr int NT = (3 << S.rho) << S.logC;
+            for (int i = threadIdx.x; i < NT; i += blockDim.x)
+                store_inv(i >> S.logC, i & ((1 << S.logC) - 1), tf.s[ff_pos_addr(tf, i >> S.logC, i & ((1 << S.logC) - 1))]);
         }
         return;
     }

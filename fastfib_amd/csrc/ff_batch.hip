@@ -32,6 +32,7 @@ namespace {
 struct BLaunch {
     int kind;  // 0 outer list, 1 inner list, 2 crt list, 3 zero fill
     bool inv, top, r3, small;
+    int sh;    // outer list: the items' fixed-geometry shape (ff_outer_shape), or 0
     const void *items;
     const uint32_t *start;
     int n;
@@ -237,23 +238,24 @@ static void build_lists(ff_batch *B) {
     std::vector<Pending> pend;
 
     // A launch allocates its largest item's LDS for every workgroup, so
-    // items are grouped by how many workgroups their LDS lets a CU hold.
+    // items are grouped by how many workgroups their LDS lets a CU hold;
+    // outer passes also by their fixed-geometry shape (ff_outer_shape).
     auto occ = [](size_t lds) -> int {
         const size_t k = lds ? (size_t)163840 / lds : 8;
         return (int)(k < 8 ? k : 8);
     };
     auto outer_occs = [&](int pass, int r3sel) {
-        std::set<int> out;
+        std::set<std::pair<int, int>> out;
         for (size_t i : order) {
             const FFPlan &pl = *plan[i];
             if (pl.npass <= pass) continue;
             const FFPass &ps = pl.pass[pass];
             if (pass == 0 && (int)ps.r3 != r3sel) continue;
-            out.insert(occ(ff_outer_lds_bytes(&ps)));
+            out.insert({occ(ff_outer_lds_bytes(&ps)), ff_outer_shape(&ps, pass == 0)});
         }
         return out;
     };
-    auto outer_list = [&](int pass, bool inv, int r3sel, int oc) {
+    auto outer_list = [&](int pass, bool inv, int r3sel, int oc, int sh) {
         std::vector<FFOuterItem> its;
         std::vector<uint32_t> start;
         std::vector<int> who;
@@ -265,7 +267,7 @@ static void build_lists(ff_batch *B) {
             if (pl.npass <= pass) continue;
             const FFPass &ps = pl.pass[pass];
             if (pass == 0 && (int)ps.r3 != r3sel) continue;
-            if (occ(ff_outer_lds_bytes(&ps)) != oc) continue;
+            if (occ(ff_outer_lds_bytes(&ps)) != oc || ff_outer_shape(&ps, pass == 0) != sh) continue;
             const ff_prod &p = B->items[i];
             const bool sq = !p.b || p.b == p.a;
             for (int op = 0; op < (sq ? 1 : 2); op++) {
@@ -290,7 +292,7 @@ static void build_lists(ff_batch *B) {
         }
         if (its.empty()) return;
         Pending pd{};
-        pd.L.kind = 0, pd.L.inv = inv, pd.L.top = pass == 0, pd.L.r3 = r3sel, pd.L.n = (int)its.size();
+        pd.L.kind = 0, pd.L.inv = inv, pd.L.top = pass == 0, pd.L.r3 = r3sel, pd.L.sh = sh, pd.L.n = (int)its.size();
         pd.L.blocks = blocks, pd.L.lds = lds, pd.L.bytes = bytes;
         pd.items_off = put(its.data(), its.size() * sizeof(FFOuterItem));
         pd.start_off = put(start.data(), start.size() * 4);
@@ -378,7 +380,7 @@ static void build_lists(ff_batch *B) {
 
     // every (pass, radix-3) list once per occupancy class
     auto outer_all = [&](int pass, bool inv, int r3sel) {
-        for (int oc : outer_occs(pass, r3sel)) outer_list(pass, inv, r3sel, oc);
+        for (const auto &os : outer_occs(pass, r3sel)) outer_list(pass, inv, r3sel, os.first, os.second);
     };
     auto inner_all = [&](int small, int r3sel) {
         for (int oc : inner_occs(small, r3sel)) inner_list(small, r3sel, oc);
@@ -555,7 +557,7 @@ static void enqueue_grouped(ff_batch *B) {
     for (const BLaunch &L : B->launches) {
         switch (L.kind) {
         case 0:
-            FF_CHECK(ff_launch_outer_list(L.inv, L.top, L.r3, (const FFOuterItem *)L.items, L.start, L.n, L.blocks,
+            FF_CHECK(ff_launch_outer_list(L.inv, L.top, L.r3, L.sh, (const FFOuterItem *)L.items, L.start, L.n, L.blocks,
                                           L.lds, L.bytes, g_st));
             break;
         case 1:

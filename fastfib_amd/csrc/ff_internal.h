@@ -203,8 +203,12 @@ hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int nc, int nge
 hipError_t ff_launch_inverse(const FFPlan *pl, const FFXformSet *xs, int nx, hipStream_t st);
 
 // Work-list launches (one launch for a heterogeneous list of items).
-hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, const FFOuterItem *items, const uint32_t *start, int n,
-                                uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
+// sh: the items' common fixed-geometry shape (ff_outer_shape), or 0.
+hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, int sh, const FFOuterItem *items, const uint32_t *start,
+                                int n, uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
+// The fixed-geometry kernel shape of an outer pass (top: the top pass), 0
+// where it has none.
+int ff_outer_shape(const FFPass *ps, bool top);
 hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerItem *items, const uint32_t *start, int n,
                                 uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
 // A doubling step whose squarings are one-block transforms, fused: the CRT

@@ -137,6 +137,11 @@ static int outer_shape(const FFPass &ps) {
     return 0;
 }
 
+int ff_outer_shape(const FFPass *ps, bool top) {
+    const int sh = outer_shape(*ps);
+    return top || sh == 6 || sh == 7 ? sh : 0;
+}
+
 // An outer pass (INV: inverse; TOP: the top pass), by its shape: the
 // fixed-geometry kernel where one is instantiated for it, else the general
 // one.

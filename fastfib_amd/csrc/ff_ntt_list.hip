@@ -7,7 +7,7 @@
 
 // Work-list form: item k owns blocks [start[k], start[k+1]) of the flat
 // grid, 3 * nblocks * tpb of them (the prime is the major index).
-template <bool INV, bool TOP, bool R3>
+template <bool INV, bool TOP, bool R3, int SH = 0>
 __global__ __launch_bounds__(FF_THREADS) void k_outer_list(const FFOuterItem *items, const uint32_t *start, int n,
                                                            const FFTables *T, FFPrimes PR) {
     extern __shared__ uint32_t lds[];
@@ -16,7 +16,7 @@ __global__ __launch_bounds__(FF_THREADS) void k_outer_list(const FFOuterItem *it
     const uint32_t local = blockIdx.x - start[k];
     const uint32_t ntile = it.ps.nblocks * it.ps.tpb;
     const int pi = (int)(local / ntile);
-    outer_body<INV, TOP, R3>(it.ps, it.x, pi, local - (uint32_t)pi * ntile, T, PR, lds);
+    outer_body<INV, TOP, R3, false, SH>(it.ps, it.x, pi, local - (uint32_t)pi * ntile, T, PR, lds);
 }
 
 // Work-list form: item k owns blocks [start[k], start[k+1]), 3 * nblk of them.
@@ -60,19 +60,58 @@ __global__ __launch_bounds__(FF_THREADS, 4) void k_inner_part_list(const FFInner
     else ff_inverse_io(ta, T->irt[pi], T->irtp[pi], P.p, P.pinv, true, true, ld, st);
 }
 
-extern "C" hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, const FFOuterItem *items,
+// A list of one pass kind, by the items' shape (sh > 0: every item has
+// that fixed-geometry shape).
+template <bool INV, bool TOP>
+static void launch_outer_list(bool r3, int sh, const FFOuterItem *items, const uint32_t *start, int n, dim3 g,
+                              size_t lds, hipStream_t st) {
+    const FFTables *T = ff_tables_dev();
+    const FFPrimes PR = ff_primes();
+    const dim3 b(FF_THREADS);
+#define FF_SHCASE(R3, SH)                                                                                      \
+    case SH: hipLaunchKernelGGL((k_outer_list<INV, TOP, R3, SH>), g, b, lds, st, items, start, n, T, PR); \
+        return;
+    if constexpr (TOP) {
+        if (r3) {
+            switch (sh) {
+                FF_SHCASE(true, 9)
+                FF_SHCASE(true, 10)
+            }
+            hipLaunchKernelGGL((k_outer_list<INV, TOP, true>), g, b, lds, st, items, start, n, T, PR);
+            return;
+        }
+        switch (sh) {
+            FF_SHCASE(false, 1)
+            FF_SHCASE(false, 2)
+            FF_SHCASE(false, 3)
+            FF_SHCASE(false, 4)
+            FF_SHCASE(false, 5)
+            FF_SHCASE(false, 6)
+            FF_SHCASE(false, 7)
+            FF_SHCASE(false, 8)
+        }
+        hipLaunchKernelGGL((k_outer_list<INV, TOP, false>), g, b, lds, st, items, start, n, T, PR);
+    } else {
+        switch (sh) {
+            FF_SHCASE(false, 6)
+            FF_SHCASE(false, 7)
+        }
+        hipLaunchKernelGGL((k_outer_list<INV, TOP, false>), g, b, lds, st, items, start, n, T, PR);
+    }
+#undef FF_SHCASE
+}
+
+extern "C" hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, int sh, const FFOuterItem *items,
                                            const uint32_t *start, int n, uint32_t blocks, size_t lds, double bytes,
                                            hipStream_t st) {
     if (!blocks) return hipSuccess;
     const int kind = inv ? (top ? FK_INV_TOP : FK_INV) : (top ? FK_FWD_TOP : FK_FWD);
-    const dim3 g(blocks), b(FF_THREADS);
+    const dim3 g(blocks);
     ff_prof_begin(kind, st);
-    if (!inv && top && r3) hipLaunchKernelGGL((k_outer_list<false, true, true>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
-    else if (!inv && top) hipLaunchKernelGGL((k_outer_list<false, true, false>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
-    else if (!inv) hipLaunchKernelGGL((k_outer_list<false, false, false>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
-    else if (top && r3) hipLaunchKernelGGL((k_outer_list<true, true, true>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
-    else if (top) hipLaunchKernelGGL((k_outer_list<true, true, false>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
-    else hipLaunchKernelGGL((k_outer_list<true, false, false>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
+    if (!inv && top) launch_outer_list<false, true>(r3, sh, items, start, n, g, lds, st);
+    else if (!inv) launch_outer_list<false, false>(r3, sh, items, start, n, g, lds, st);
+    else if (top) launch_outer_list<true, true>(r3, sh, items, start, n, g, lds, st);
+    else launch_outer_list<true, false>(r3, sh, items, start, n, g, lds, st);
     ff_prof_end(kind, bytes, st);
     return hipGetLastError();
 }

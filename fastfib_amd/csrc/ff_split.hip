@@ -215,7 +215,7 @@ void ff_mul_split(const MulItem &it, int force_lg) {
         const FFOuterItem *di;
         const uint32_t *ds;
         lists.push_back(upload_list(its, start, &di, &ds));
-        FF_CHECK(ff_launch_outer_list(inv, pass == 0, false, di, ds, (int)its.size(), blocks,
+        FF_CHECK(ff_launch_outer_list(inv, pass == 0, false, ff_outer_shape(&ps, pass == 0), di, ds, (int)its.size(), blocks,
                                       ff_outer_lds_bytes(&ps), 3.0 * 8.0 * Lp * its.size(), g_st));
     };
     auto inner = [&](bool inv, const std::vector<uint32_t *> &arrays) {

@@ -158,6 +158,9 @@ static void launch_outer(const FFPass &ps, const FFXformSet *xs, dim3 grid, hipS
             switch (sh) {
                 FF_SHCASE(true, 9)
                 FF_SHCASE(true, 10)
+                FF_SHCASE(true, 11)
+                FF_SHCASE(true, 13)
+                FF_SHCASE(true, 14)
             }
             hipLaunchKernelGGL((k_outer<INV, TOP, true>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR);
             return;
@@ -171,6 +174,7 @@ static void launch_outer(const FFPass &ps, const FFXformSet *xs, dim3 grid, hipS
             FF_SHCASE(false, 6)
             FF_SHCASE(false, 7)
             FF_SHCASE(false, 8)
+            FF_SHCASE(false, 12)
         }
         hipLaunchKernelGGL((k_outer<INV, TOP, false>), grid, dim3(FF_THREADS), lds, st, ps, *xs, T, PR);
     } else {

@@ -189,12 +189,14 @@ __device__ __forceinline__ void ff_unpack_tile(const FFTile &t, const FFCoefSrc 
 // levels), as the plans of products of 2^17 .. 3 * 2^24 words per prime
 // give them: top passes of 2^17 .. 2^21 words (shapes 1-5, fast doubling's
 // tiles), of 2^22 .. 2^25 (6-8) and of 3 * 2^21 .. 3 * 2^23 (9, 10, with
-// the radix-3 layer); second passes of 3 * 2^21 .. 3 * 2^24 words (6, 7).
+// the radix-3 layer); second passes of 3 * 2^21 .. 3 * 2^24 words (6, 7);
+// top passes of batch items with 4096-word tiles: 3 * 2^17 .. 3 * 2^20
+// words with the radix-3 layer (11, 13, 14), 2^17 .. 2^19 words (12).
 // Shape 0: the general path.
 struct FFShape {
     int rho, logC, ld, rl, r3;
 };
-#define FF_NSHAPES 10
+#define FF_NSHAPES 14
 __host__ __device__ constexpr FFShape ff_shape(int sh) {
     return sh == 1    ? FFShape{5, 6, 64, 3, 0}
            : sh == 2  ? FFShape{6, 5, 33, 3, 0}
@@ -206,6 +208,10 @@ __host__ __device__ constexpr FFShape ff_shape(int sh) {
            : sh == 8  ? FFShape{10, 4, 16, 4, 0}
            : sh == 9  ? FFShape{5, 5, 33, 3, 1}
            : sh == 10 ? FFShape{4, 6, 64, 4, 1}
+           : sh == 11 ? FFShape{6, 4, 17, 3, 1}
+           : sh == 12 ? FFShape{7, 5, 33, 4, 0}
+           : sh == 13 ? FFShape{7, 4, 17, 4, 1}
+           : sh == 14 ? FFShape{8, 4, 17, 4, 1}
                       : FFShape{0, 0, 0, 0, 0};
 }
 

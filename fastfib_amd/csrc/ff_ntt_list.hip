@@ -76,6 +76,9 @@ static void launch_outer_list(bool r3, int sh, const FFOuterItem *items, const u
             switch (sh) {
                 FF_SHCASE(true, 9)
                 FF_SHCASE(true, 10)
+                FF_SHCASE(true, 11)
+                FF_SHCASE(true, 13)
+                FF_SHCASE(true, 14)
             }
             hipLaunchKernelGGL((k_outer_list<INV, TOP, true>), g, b, lds, st, items, start, n, T, PR);
             return;
@@ -89,6 +92,7 @@ static void launch_outer_list(bool r3, int sh, const FFOuterItem *items, const u
             FF_SHCASE(false, 6)
             FF_SHCASE(false, 7)
             FF_SHCASE(false, 8)
+            FF_SHCASE(false, 12)
         }
         hipLaunchKernelGGL((k_outer_list<INV, TOP, false>), g, b, lds, st, items, start, n, T, PR);
     } else {

@@ -189,11 +189,12 @@ static void launch_outer(const FFPass &ps, const FFXformSet *xs, dim3 grid, hipS
 
 
 
+template <int PM>
 __global__ __launch_bounds__(2 * FF_THREADS, 2) void k_inner_pair(FFInner in, FFConvSet cs, const FFTables *T,
                                                                   FFPrimes PR) {
     extern __shared__ uint32_t lds[];
     const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
-    inner_pair_body(in, cs.c[item], pi, blockIdx.x, T, PR, lds);
+    inner_pair_body<PM>(in, cs.c[item], pi, blockIdx.x, T, PR, lds);
 }
 
 template <bool SMALL, bool R3>
@@ -398,7 +399,13 @@ extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int 
     }
     const size_t lds = (one_tile ? 1 : 2) * words * sizeof(uint32_t);
     if (cs->c[0].pair & 6) {  // two outputs or degree 4: the 512-thread pair kernel
-        hipLaunchKernelGGL(k_inner_pair, grid, dim3(2 * FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
+        const int pm = ff_pair_mode(in);
+        if (pm == 1)
+            hipLaunchKernelGGL(k_inner_pair<1>, grid, dim3(2 * FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
+        else if (pm == 2)
+            hipLaunchKernelGGL(k_inner_pair<2>, grid, dim3(2 * FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
+        else
+            hipLaunchKernelGGL(k_inner_pair<0>, grid, dim3(2 * FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
     } else if (!in.small)
         hipLaunchKernelGGL((k_inner<false, false>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
     else if (in.r3)

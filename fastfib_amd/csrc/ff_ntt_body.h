@@ -524,8 +524,67 @@ __device__ __forceinline__ void pair_mid_round(const FFTile &t, bool active, con
     }
 }
 
+// The inverse rounds of a pair block above its middle round: rounds of RL
+// levels from row bit B up to RHO, the last one to HBM, at literal positions.
+template <int RHO, int RL, int B, class St>
+__device__ __forceinline__ void pair_dit_fixed(const FFTile &t, const uint32_t *irt, const uint32_t *irtp, uint32_t p,
+                                               uint32_t pinv, St st) {
+    if constexpr (B < RHO) {
+        __syncthreads();
+        ff_round_io<RL, FF_R_DIT, false, B + RL >= RHO>(t, B, nullptr, nullptr, irt, irtp, nullptr, p, pinv, st, st);
+        pair_dit_fixed<RHO, RL, B + RL>(t, irt, irtp, p, pinv, st);
+    }
+}
+
+// inner_pair_body for blocks of 2^RHO words in rounds of RL levels, the
+// tile geometry and every round position literal; each half addresses its
+// operand's block through a buffer descriptor (the half is the same for a
+// whole wavefront, so the block's address is read from the first lane).
+template <int RHO, int RL>
+__device__ __forceinline__ void inner_pair_fixed(const FFConv &C, int pi, size_t off, const FFTables *T,
+                                                 const FFPrimes &PR, uint32_t *lds) {
+    constexpr int N = 1 << RHO, WORDS = N + (N >> 4) + 16;
+    constexpr int NL0 = RHO % RL ? RHO % RL : RL;
+    const FFPrime P = PR.p[pi];
+    const uint32_t p = P.p, pinv = P.pinv;
+    const int half = threadIdx.x >= FF_THREADS;
+    const FFTile tf{lds + half * WORDS, 1, 1, 0, RHO, 1, 1, half * FF_THREADS, FF_THREADS};
+    // (readfirstlane returns an int: each half goes back through uint32_t,
+    // or a low half with bit 31 set would sign-extend over the high one)
+    const uint64_t bp = (uint64_t)((half ? const_cast<uint32_t *>(C.fb) : C.fa) + off);
+    const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(bp >> 32));
+    const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)bp);
+    const uint64_t bu = ((uint64_t)bhi << 32) | blo;
+    const FFRows<false> rows(ff_g((uint32_t *)bu), N, 1, 0);
+    ff_forward_fixed<RHO, RL, 0, true, false>(tf, T->rt[pi], T->rtp[pi], p, pinv, rows, rows, FFNoTw{});
+    __syncthreads();
+    const int two = (C.pair & 3) == 2;
+    const FFPairPoint pt{half ? C.cv[0] : C.cu[0], half ? C.cv[1] : C.cu[1], half ? C.cv[2] : C.cu[2],
+                         (C.pair & 4) != 0, ff_mont(P.r2, P.r2, p, pinv)};
+    const bool active = !half || two;
+    pair_mid_round<NL0>(tf, active, lds, lds + WORDS, T->irt[pi], T->irtp[pi], p, pinv, pt, false, rows);
+    FFTile td = tf;
+    if (!active) td.nsub = 0;
+    pair_dit_fixed<RHO, RL, NL0>(td, T->irt[pi], T->irtp[pi], p, pinv, rows);
+}
+
+// PM: 0 the general body, 1 blocks of 2^12 words in rounds of four levels,
+// 2 blocks of 2^10 words in rounds of two (ff_pair_mode).
+__host__ __device__ inline int ff_pair_mode(const FFInner &in) {
+    return in.s == 12 && in.rl == 4 ? 1 : in.s == 10 && in.rl == 2 ? 2 : 0;
+}
+
+template <int PM>
 __device__ __forceinline__ void inner_pair_body(const FFInner &in, const FFConv &C, int pi, uint32_t blkx,
                                                 const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
+    if constexpr (PM == 1) {
+        inner_pair_fixed<12, 4>(C, pi, (size_t)pi * in.L + (size_t)blkx * in.S, T, PR, lds);
+        return;
+    }
+    if constexpr (PM == 2) {
+        inner_pair_fixed<10, 2>(C, pi, (size_t)pi * in.L + (size_t)blkx * in.S, T, PR, lds);
+        return;
+    }
     const FFPrime P = PR.p[pi];
     const uint32_t p = P.p, pinv = P.pinv;
     const uint32_t n = in.S;

@@ -197,11 +197,11 @@ __global__ __launch_bounds__(2 * FF_THREADS, 2) void k_inner_pair(FFInner in, FF
     inner_pair_body<PM>(in, cs.c[item], pi, blockIdx.x, T, PR, lds);
 }
 
-template <bool SMALL, bool R3>
+template <bool SMALL, bool R3, bool FIX12 = false>
 __global__ __launch_bounds__(FF_THREADS, 4) void k_inner(FFInner in, FFConvSet cs, const FFTables *T, FFPrimes PR) {
     extern __shared__ uint32_t lds[];
     const int item = blockIdx.y / 3, pi = blockIdx.y - 3 * (blockIdx.y / 3);
-    inner_body<SMALL, R3>(in, cs.c[item], pi, blockIdx.x, T, PR, lds);
+    inner_body<SMALL, R3, false, FIX12>(in, cs.c[item], pi, blockIdx.x, T, PR, lds);
 }
 
 
@@ -406,6 +406,9 @@ extern "C" hipError_t ff_launch_conv(const FFPlan *pl, const FFConvSet *cs, int 
             hipLaunchKernelGGL(k_inner_pair<2>, grid, dim3(2 * FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
         else
             hipLaunchKernelGGL(k_inner_pair<0>, grid, dim3(2 * FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
+    } else if (!in.small && in.s == 12) {  // 2^12-word blocks: the fixed-path kernel
+        hipLaunchKernelGGL((k_inner<false, false, true>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables,
+                           g_primes_arg);
     } else if (!in.small)
         hipLaunchKernelGGL((k_inner<false, false>), grid, dim3(FF_THREADS), lds, st, in, *cs, g_tables, g_primes_arg);
     else if (in.r3)

@@ -317,7 +317,9 @@ __device__ __forceinline__ int ff_find_item(const uint32_t *start, int n, uint32
 // Inner blocks: forward transform of a (and b), pointwise product, inverse,
 // all in LDS. small = 1: the whole transform, read from the limbs and
 // scaled for the CRT at the end.
-template <bool SMALL, bool R3, bool COH = false>
+// FIX12: every block is 2^12 words (the fixed path only, in a kernel of
+// its own, whose registers are the fixed path's alone).
+template <bool SMALL, bool R3, bool COH = false, bool FIX12 = false>
 __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, int pi, uint32_t blkx,
                                            const FFTables *T, const FFPrimes &PR, uint32_t *lds) {
     const FFPrime P = PR.p[pi];
@@ -357,7 +359,7 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
     auto lds_a_st = [&](int pos, int c, uint32_t v) { A[ff_pos_addr(ta, pos, c)] = v; };
     auto lds_b_st = [&](int pos, int c, uint32_t v) { B[ff_pos_addr(tb, pos, c)] = v; };
 
-    if (!SMALL && in.s == 12) {
+    if (!SMALL && (FIX12 || in.s == 12)) {
         // the common block of 2^12 words: its geometry as literals
         const FFTile a12{A, 1, 1, 0, 12, 1, 1}, b12{B, 1, 1, 0, 12, 1, 1};
         const FFRows<COH> ra(dst, 4096, 1, 0);
@@ -375,6 +377,7 @@ __device__ __forceinline__ void inner_body(const FFInner &in, const FFConv &C, i
         }
         return;
     }
+    if constexpr (FIX12) return;
     if (!SMALL) {
         // the block's words as rows of one word (row accessors for the rounds)
         const FFRows<COH> ra(dst, n, 1, 0);

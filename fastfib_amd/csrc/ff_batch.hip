@@ -32,7 +32,8 @@ namespace {
 struct BLaunch {
     int kind;  // 0 outer list, 1 inner list, 2 crt list, 3 zero fill
     bool inv, top, r3, small;
-    int sh;    // outer list: the items' fixed-geometry shape (ff_outer_shape), or 0
+    int sh;    // outer list: the items' fixed-geometry shape (ff_outer_shape), or 0;
+               // inner list: 1 where every block is 2^12 words (the fixed-path kernel)
     const void *items;
     const uint32_t *start;
     int n;
@@ -299,16 +300,19 @@ static void build_lists(ff_batch *B) {
         pd.item_size = sizeof(FFOuterItem), pd.start = start, pd.who = who;
         pend.push_back(pd);
     };
+    // multi-pass items' inner lists also by whether their blocks are 2^12
+    // words (the fixed-path kernel)
+    auto fix12 = [](const FFPlan &pl) { return !pl.inner.small && pl.inner.s == 12 ? 1 : 0; };
     auto inner_occs = [&](int small, int r3sel) {
-        std::set<int> out;
+        std::set<std::pair<int, int>> out;
         for (size_t i : order) {
             const FFPlan &pl = *plan[i];
             if (pl.inner.small != small || (small && pl.inner.r3 != r3sel)) continue;
-            out.insert(occ(ff_inner_lds_bytes(&pl.inner)));
+            out.insert({occ(ff_inner_lds_bytes(&pl.inner)), fix12(pl)});
         }
         return out;
     };
-    auto inner_list = [&](int small, int r3sel, int oc) {
+    auto inner_list = [&](int small, int r3sel, int oc, int fx) {
         std::vector<FFInnerItem> its;
         std::vector<uint32_t> start;
         std::vector<int> who;
@@ -318,7 +322,7 @@ static void build_lists(ff_batch *B) {
         for (size_t i : order) {
             const FFPlan &pl = *plan[i];
             if (pl.inner.small != small || (small && pl.inner.r3 != r3sel)) continue;
-            if (occ(ff_inner_lds_bytes(&pl.inner)) != oc) continue;
+            if (occ(ff_inner_lds_bytes(&pl.inner)) != oc || fix12(pl) != fx) continue;
             const ff_prod &p = B->items[i];
             const bool sq = !p.b || p.b == p.a;
             FFInnerItem ii;
@@ -337,7 +341,7 @@ static void build_lists(ff_batch *B) {
         }
         if (its.empty()) return;
         Pending pd{};
-        pd.L.kind = 1, pd.L.small = small, pd.L.r3 = r3sel, pd.L.n = (int)its.size();
+        pd.L.kind = 1, pd.L.small = small, pd.L.r3 = r3sel, pd.L.sh = fx, pd.L.n = (int)its.size();
         pd.L.blocks = blocks, pd.L.lds = lds, pd.L.bytes = bytes;
         pd.items_off = put(its.data(), its.size() * sizeof(FFInnerItem));
         pd.start_off = put(start.data(), start.size() * 4);
@@ -383,7 +387,7 @@ static void build_lists(ff_batch *B) {
         for (const auto &os : outer_occs(pass, r3sel)) outer_list(pass, inv, r3sel, os.first, os.second);
     };
     auto inner_all = [&](int small, int r3sel) {
-        for (int oc : inner_occs(small, r3sel)) inner_list(small, r3sel, oc);
+        for (const auto &os : inner_occs(small, r3sel)) inner_list(small, r3sel, os.first, os.second);
     };
     for (int pass = 0; pass < 3; pass++) {
         if (pass == 0) {
@@ -561,7 +565,7 @@ static void enqueue_grouped(ff_batch *B) {
                                           L.lds, L.bytes, g_st));
             break;
         case 1:
-            FF_CHECK(ff_launch_inner_list(L.small, L.r3, (const FFInnerItem *)L.items, L.start, L.n, L.blocks, L.lds,
+            FF_CHECK(ff_launch_inner_list(L.small, L.r3, L.sh != 0, (const FFInnerItem *)L.items, L.start, L.n, L.blocks, L.lds,
                                           L.bytes, g_st));
             break;
         case 2:

@@ -209,7 +209,8 @@ hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, int sh, const FFOut
 // The fixed-geometry kernel shape of an outer pass (top: the top pass), 0
 // where it has none.
 int ff_outer_shape(const FFPass *ps, bool top);
-hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerItem *items, const uint32_t *start, int n,
+// fix12: every item's blocks are 2^12 words (the fixed-path kernel).
+hipError_t ff_launch_inner_list(bool small, bool r3, bool fix12, const FFInnerItem *items, const uint32_t *start, int n,
                                 uint32_t blocks, size_t lds_bytes, double bytes, hipStream_t st);
 // A doubling step whose squarings are one-block transforms, fused: the CRT
 // of both squares and the linear step (fib.c:36-62) in one workgroup.

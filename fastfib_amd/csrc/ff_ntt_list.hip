@@ -20,7 +20,7 @@ __global__ __launch_bounds__(FF_THREADS) void k_outer_list(const FFOuterItem *it
 }
 
 // Work-list form: item k owns blocks [start[k], start[k+1]), 3 * nblk of them.
-template <bool SMALL, bool R3>
+template <bool SMALL, bool R3, bool FIX12 = false>
 __global__ __launch_bounds__(FF_THREADS, 4) void k_inner_list(const FFInnerItem *items, const uint32_t *start, int n,
                                                               const FFTables *T, FFPrimes PR) {
     extern __shared__ uint32_t lds[];
@@ -29,7 +29,7 @@ __global__ __launch_bounds__(FF_THREADS, 4) void k_inner_list(const FFInnerItem 
     const uint32_t local = blockIdx.x - start[k];
     const uint32_t nblk = SMALL ? 1 : it.in.L / it.in.S;
     const int pi = (int)(local / nblk);
-    inner_body<SMALL, R3>(it.in, it.c, pi, local - (uint32_t)pi * nblk, T, PR, lds);
+    inner_body<SMALL, R3, false, FIX12>(it.in, it.c, pi, local - (uint32_t)pi * nblk, T, PR, lds);
 }
 
 // Inner blocks of a transform on their own, in place: the forward levels
@@ -120,8 +120,9 @@ extern "C" hipError_t ff_launch_outer_list(bool inv, bool top, bool r3, int sh, 
     return hipGetLastError();
 }
 
-extern "C" hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerItem *items, const uint32_t *start,
-                                           int n, uint32_t blocks, size_t lds, double bytes, hipStream_t st) {
+extern "C" hipError_t ff_launch_inner_list(bool small, bool r3, bool fix12, const FFInnerItem *items,
+                                           const uint32_t *start, int n, uint32_t blocks, size_t lds, double bytes,
+                                           hipStream_t st) {
     if (!blocks) return hipSuccess;
     const int kind = small ? FK_SMALL : FK_INNER;
     // one-block lists of short transforms: n / 4 threads (what radix-4
@@ -136,7 +137,8 @@ extern "C" hipError_t ff_launch_inner_list(bool small, bool r3, const FFInnerIte
     }
     const dim3 g(blocks), b(threads);
     ff_prof_begin(kind, st);
-    if (!small) hipLaunchKernelGGL((k_inner_list<false, false>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
+    if (!small && fix12) hipLaunchKernelGGL((k_inner_list<false, false, true>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
+    else if (!small) hipLaunchKernelGGL((k_inner_list<false, false>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
     else if (r3) hipLaunchKernelGGL((k_inner_list<true, true>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
     else hipLaunchKernelGGL((k_inner_list<true, false>), g, b, lds, st, items, start, n, ff_tables_dev(), ff_primes());
     ff_prof_end(kind, bytes, st);

@@ -104,6 +104,45 @@ __device__ void early_pair_conv(const FFTile &t, const uint32_t *const *tw, uint
     }
 }
 
+// early_pair_conv for 2^LG-point sub-transforms with every round's row
+// bits a literal (as the tile's geometry): the LDS offsets of each round are
+// compile-time constants.
+template <int LG, int R, class Ld, class St>
+__device__ __forceinline__ void early_fwd_fixed(const FFTile &t, const uint32_t *const *tw, uint32_t p, uint32_t pinv,
+                                                Ld ld, St st) {
+    constexpr int NR = (LG + 1) / 2;
+    if constexpr (R < NR - 1) {
+        constexpr int B = LG - 2 * (R + 1);
+        if constexpr (R > 0) __syncthreads();
+        ff_round_io<2, FF_R_DIF, R == 0, false>(t, B, tw[0], tw[1], nullptr, nullptr, nullptr, p, pinv, ld, st,
+                                                FFLdsTables{});
+        early_fwd_fixed<LG, R + 1>(t, tw, p, pinv, ld, st);
+    }
+}
+
+template <int LG, int B, class Ld, class St>
+__device__ __forceinline__ void early_inv_fixed(const FFTile &t, const uint32_t *const *tw, uint32_t p, uint32_t pinv,
+                                                Ld ld, St st) {
+    if constexpr (B < LG) {
+        __syncthreads();
+        ff_round_io<2, FF_R_DIT, false, B + 2 >= LG>(t, B, nullptr, nullptr, tw[2], tw[3], nullptr, p, pinv, ld, st,
+                                                     FFLdsTables{});
+        early_inv_fixed<LG, B + 2>(t, tw, p, pinv, ld, st);
+    }
+}
+
+template <int LG, class Ld, class St>
+__device__ __forceinline__ void early_pair_conv_fixed(const FFTile &t0, const uint32_t *const *tw, uint32_t p,
+                                                      uint32_t pinv, int bit, Ld ld, St st) {
+    static_assert(LG >= 4, "a forward and an inverse round apart from the middle one");
+    const FFTile t{t0.s, 1, 1, 0, LG, 2, 1, t0.tid0, t0.nthr};
+    constexpr int NL0 = LG % 2 ? 1 : 2;
+    early_fwd_fixed<LG, 0>(t, tw, p, pinv, ld, st);
+    __syncthreads();
+    early_pair_mid<NL0>(t, tw, p, pinv, bit);
+    early_inv_fixed<LG, NL0>(t, tw, p, pinv, ld, st);
+}
+
 // Coefficients of a product of len <= FF_EARLY_LIMBS limbs at bw >= 32, as
 // SSShared::MAXC for that many limbs.
 #define EARLY_MAXC (((FF_EARLY_LIMBS + 1) * 64 + 93) / 32 + 64)
@@ -171,7 +210,11 @@ __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTa
             E.y[pos >> lg][pi * L + (pos & (L - 1))] = ff_mont(v, sc, p, pinv);
         };
         const uint32_t *tw[4] = {E.tw[pi][0], E.tw[pi][1], E.tw[pi][2], E.tw[pi][3]};
-        early_pair_conv(tl, tw, p, pinv, st.bit, ld, sto);
+        // the chain's longest transforms with literal round positions
+        if (lg == 10) early_pair_conv_fixed<10>(tl, tw, p, pinv, st.bit, ld, sto);
+        else if (lg == 9) early_pair_conv_fixed<9>(tl, tw, p, pinv, st.bit, ld, sto);
+        else if (lg == 8) early_pair_conv_fixed<8>(tl, tw, p, pinv, st.bit, ld, sto);
+        else early_pair_conv(tl, tw, p, pinv, st.bit, ld, sto);
         __syncthreads();
         // both new numbers, sums of products of F(k) and F(k - 1): conv1
         // coefficients each, at most (64 len + bw - 1) / bw <= EARLY_MAXC

@@ -107,7 +107,7 @@ extern "C" hipError_t ff_launch_crt(const FFCrtSet *js, int njobs, uint64_t max_
     uint64_t total = 0;
     int wide = 1;
     for (int k = 0; k < njobs; k++) total += js->j[k].nout, wide &= js->j[k].bw >= 32;
-    const int lpt = crt_lpt(wide ? 32 : 0, total);
+    const int lpt = wide && total > (uint64_t)g_crt_wave_limbs ? 4 : 1;
     const FFCrtSet &set = *js;
     const uint64_t max_blk = (max_nout + (uint64_t)CRT_LIMBS * lpt - 1) / ((uint64_t)CRT_LIMBS * lpt);
     ff_prof_begin(FK_CRT_PACK, st);
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(CRT_LIMBS) void k_step_fused(FFStepJob J, FFGarner 
 
 // Limbs per thread of a step: as for the CRT (one while a workgroup per
 // 256 limbs fits one wave of residency).
-static inline int step_lpt(uint64_t len) { return len > CRT_ONE_WAVE_LIMBS ? 4 : 1; }
+static inline int step_lpt(uint64_t len) { return len > (uint64_t)g_crt_wave_limbs ? 4 : 1; }
 
 size_t ff_step_lb_words(uint64_t len) {
     const uint64_t per = (uint64_t)CRT_LIMBS * step_lpt(len);

@@ -23,6 +23,10 @@ extern int g_outer_rl3;
 // Levels per round of the pair kernel: 0 = two for blocks of at most 2^10
 // words, else four; 2..4 = that many.
 extern int g_pair_rl;
+// Output limbs above which a single CRT launch and a fused doubling step
+// take four limbs per thread (default 1280 workgroups of 256: one wave of
+// residency).
+extern long long g_crt_wave_limbs;
 extern uint64_t g_force_len;
 extern int g_max_lg;    // longest single transform 2^max_lg; longer products split
 extern int g_pair_steps;   // bit 0: doubling steps as weighted pairs; bit 1: the last product as one

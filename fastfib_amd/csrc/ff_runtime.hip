@@ -25,6 +25,7 @@ int g_tile_words = 4096;
 int g_fib_tile_words = 2048;
 int g_outer_rl = 0;
 int g_pair_rl = 0;
+long long g_crt_wave_limbs = 1280ll * 256;
 int g_outer_rl3 = (1 << 5) | (1 << 6) | (1 << 9);
 uint64_t g_force_len = 0;
 int g_dispatch = 0;
@@ -419,7 +420,7 @@ extern "C" int ff_get_option(const char *name, long long *v) {
                 {"quad_inner_log", g_quad_inner_log}, {"host_bits", g_host_bits}, {"school_limbs", g_school_limbs},
                 {"queue_waves", g_queue_waves},   {"dec_queue", g_dec_queue},
                 {"queue_parts", g_queue_parts},   {"queue_chunk", g_queue_chunk},       {"dispatch", g_dispatch},
-                {"fib_tile_words", g_fib_tile_words}, {"outer_rl", g_outer_rl}, {"outer_rl3", g_outer_rl3}, {"pair_rl", g_pair_rl}};
+                {"fib_tile_words", g_fib_tile_words}, {"outer_rl", g_outer_rl}, {"outer_rl3", g_outer_rl3}, {"pair_rl", g_pair_rl}, {"crt_wave_limbs", g_crt_wave_limbs}};
     for (const auto &o : opts) {
         if (!strcmp(name, o.name)) {
             *v = o.value;
@@ -444,6 +445,9 @@ extern "C" int ff_set_option(const char *name, long long v) {
     } else if (!strcmp(name, "outer_rl")) {
         if (v != 0 && (v < 2 || v > 4)) return FF_ERR_ARG;
         g_outer_rl = (int)v;
+    } else if (!strcmp(name, "crt_wave_limbs")) {
+        if (v < 0 || v > (1ll << 40)) return FF_ERR_ARG;
+        g_crt_wave_limbs = v;
     } else if (!strcmp(name, "pair_rl")) {
         if (v != 0 && (v < 2 || v > 4)) return FF_ERR_ARG;
         g_pair_rl = (int)v;

@@ -94,12 +94,12 @@ struct FFRows {
 // (FFOuterTw with scm = R^2 mod p).
 struct FFCoefSrc {
     __amdgpu_buffer_rsrc_t rs;
-    uint32_t Sb, c0, lomask, himask, p, pinv;
+    uint32_t nw, Sb, c0, lomask, himask, p, pinv;
     int bw;
-    __device__ __forceinline__ FFCoefSrc(const uint32_t *src, uint32_t nw, uint32_t Sb_, uint32_t c0_, int bw_,
+    __device__ __forceinline__ FFCoefSrc(const uint32_t *src, uint32_t nw_, uint32_t Sb_, uint32_t c0_, int bw_,
                                          const FFPrime &P)
-        : rs(__builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)(nw * 4u), 0x00020000)),
-          Sb(Sb_), c0(c0_), lomask(bw_ >= 32 ? ~0u : ((1u << bw_) - 1)),
+        : rs(__builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)(nw_ * 4u), 0x00020000)),
+          nw(nw_), Sb(Sb_), c0(c0_), lomask(bw_ >= 32 ? ~0u : ((1u << bw_) - 1)),
           himask(bw_ > 32 ? ((1u << (bw_ - 32)) - 1) : 0u), p(P.p), pinv(P.pinv), bw(bw_) {}
     __device__ __forceinline__ uint32_t bit(int pos, int c) const {
         return ((uint32_t)pos * Sb + c0 + (uint32_t)c) * (uint32_t)bw;
@@ -146,7 +146,23 @@ __device__ __forceinline__ void ff_unpack_tile_w(const FFTile &t, const FFCoefSr
         const int col = tid & (t.C - 1);
         uint32_t bit = cs.bit(row, col);
         int adr = row * t.ld + col;
+        // rows from rows_nz on hold coefficients pos * Sb + c0 + c >= ncoef,
+        // past the operand's last bit: zero (a product's operands fill at
+        // most half its transform, so the tile's upper half is usually
+        // zero). A batch whose first row (k0 rs, that of thread 0) is past
+        // them, uniform across the workgroup, stores zeros with no loads.
+        const uint32_t ncoef = (uint32_t)(((uint64_t)cs.nw * 32u + (uint32_t)cs.bw - 1) / (uint32_t)cs.bw);
+        const uint32_t rows_nz = (ncoef + cs.Sb - 1) / cs.Sb;
         for (int k0 = 0; k0 * nthr < n; k0 += FF_UNPACK_B) {
+            if ((uint32_t)(k0 * rs) >= rows_nz) {
+#pragma unroll
+                for (int e = 0; e < FF_UNPACK_B; e++)
+                    if (row + e * rs < rows) t.s[adr + e * dadr] = 0;
+                bit += FF_UNPACK_B * dbit;
+                adr += FF_UNPACK_B * dadr;
+                row += FF_UNPACK_B * rs;
+                continue;
+            }
             uint32_t w[FF_UNPACK_B][3];
 #pragma unroll
             for (int e = 0; e < FF_UNPACK_B; e++) cs.words(bit + (uint32_t)e * dbit, w[e]);

@@ -170,28 +170,41 @@ static_assert(sizeof(EarlyCrt2) <= sizeof(uint32_t) * 3 * EARLY_TILE, "second ch
 
 __global__ __launch_bounds__(EARLY_T) void k_fib_early(FFEarlyArgs a, const FFTables *T, FFPrimes PR, FFGarner G) {
     __shared__ EarlyShared E;
+    static_assert(EARLY_T == EARLY_L, "thread t copies entry t of each table");
     const int t = threadIdx.x;
+    const int pi = t < 640 ? t / 320 : 2;
+    const int tid0 = 320 * pi, nthr = pi < 2 ? 320 : 384;
+    // every load of the set-up (the twelve table entries of this thread,
+    // the prime, the first step) issued before the first LDS store: the
+    // tables are read through a generic pointer, which the compiler cannot
+    // tell apart from LDS, so stores between them would order the loads
+    const FFPrime P = PR.p[pi];
+    const uint32_t p = P.p, pinv = P.pinv;
+    FFEarlyStep st_next = a.st[0];
+    uint32_t tv[3][4];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        tv[q][0] = T->rt[q][t];
+        tv[q][1] = T->rtp[q][t];
+        tv[q][2] = T->irt[q][t];
+        tv[q][3] = T->irtp[q][t];
+    }
+    if (t < 2 * a.nsteps) atomicExch(a.keys + t, 0ull);
     for (int i = t; i < FF_EARLY_LIMBS; i += EARLY_T) {
         E.num[0][i] = i < FF_EARLY_F0 ? a.f[0][i] : 0;
         E.num[1][i] = i < FF_EARLY_F0 ? a.f[1][i] : 0;
     }
-    for (int i = t; i < 3 * EARLY_L; i += EARLY_T) {
-        const int q = i / EARLY_L, j = i - q * EARLY_L;
-        E.tw[q][0][j] = T->rt[q][j];
-        E.tw[q][1][j] = T->rtp[q][j];
-        E.tw[q][2][j] = T->irt[q][j];
-        E.tw[q][3][j] = T->irtp[q][j];
-    }
-    if (t < 2 * a.nsteps) atomicExch(a.keys + t, 0ull);
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) E.tw[q][k][t] = tv[q][k];
     __threadfence();
     __syncthreads();
-    const int pi = t < 640 ? t / 320 : 2;
-    const int tid0 = 320 * pi, nthr = pi < 2 ? 320 : 384;
-    const FFPrime P = PR.p[pi];
-    const uint32_t p = P.p, pinv = P.pinv;
     SSRun R{0, 0};
     for (int s = 0; s < a.nsteps; s++) {
-        const FFEarlyStep st = a.st[s];
+        // this step's sizes; the next step's load overlaps this step
+        const FFEarlyStep st = st_next;
+        if (s + 1 < a.nsteps) st_next = a.st[s + 1];
         const int lg = st.lg;
         const int L = 1 << lg;
         const FFTile tl{E.tile[pi], 1, 1, 0, lg, 2, 1, tid0, nthr};

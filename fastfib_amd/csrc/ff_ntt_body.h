@@ -128,10 +128,12 @@ struct FFCoefSrc {
 };
 
 // The tile of a forward top pass from the operand's limbs (FFCoefSrc): each
-// thread issues the loads of FF_UNPACK_B elements before it uses the first
-// of them.
+// thread issues the loads of UB elements before it uses the first of them.
+// Batches of 4 skip the zero rows at a finer grain; the 2^10-row tiles keep
+// 8 loads in flight (measured: 4 is faster for tiles of at most 2^9 rows,
+// 8 for 2^10).
 #define FF_UNPACK_B 8
-template <bool WIDE>
+template <bool WIDE, int UB = FF_UNPACK_B>
 __device__ __forceinline__ void ff_unpack_tile_w(const FFTile &t, const FFCoefSrc &cs) {
     const int n = (t.nsub << t.rho) << t.logC;
     const int tid = ff_tile_tid(t), nthr = ff_tile_nthr(t);
@@ -153,41 +155,41 @@ __device__ __forceinline__ void ff_unpack_tile_w(const FFTile &t, const FFCoefSr
         // them, uniform across the workgroup, stores zeros with no loads.
         const uint32_t ncoef = (uint32_t)(((uint64_t)cs.nw * 32u + (uint32_t)cs.bw - 1) / (uint32_t)cs.bw);
         const uint32_t rows_nz = (ncoef + cs.Sb - 1) / cs.Sb;
-        for (int k0 = 0; k0 * nthr < n; k0 += FF_UNPACK_B) {
+        for (int k0 = 0; k0 * nthr < n; k0 += UB) {
             if ((uint32_t)(k0 * rs) >= rows_nz) {
 #pragma unroll
-                for (int e = 0; e < FF_UNPACK_B; e++)
+                for (int e = 0; e < UB; e++)
                     if (row + e * rs < rows) t.s[adr + e * dadr] = 0;
-                bit += FF_UNPACK_B * dbit;
-                adr += FF_UNPACK_B * dadr;
-                row += FF_UNPACK_B * rs;
+                bit += UB * dbit;
+                adr += UB * dadr;
+                row += UB * rs;
                 continue;
             }
-            uint32_t w[FF_UNPACK_B][3];
+            uint32_t w[UB][3];
 #pragma unroll
-            for (int e = 0; e < FF_UNPACK_B; e++) cs.words(bit + (uint32_t)e * dbit, w[e]);
+            for (int e = 0; e < UB; e++) cs.words(bit + (uint32_t)e * dbit, w[e]);
 #pragma unroll
-            for (int e = 0; e < FF_UNPACK_B; e++) {
+            for (int e = 0; e < UB; e++) {
                 const uint32_t r = cs.value<WIDE>(w[e], bit + (uint32_t)e * dbit);
                 if (row + e * rs < rows) t.s[adr + e * dadr] = r;
             }
-            bit += FF_UNPACK_B * dbit;
-            adr += FF_UNPACK_B * dadr;
-            row += FF_UNPACK_B * rs;
+            bit += UB * dbit;
+            adr += UB * dadr;
+            row += UB * rs;
         }
         return;
     }
-    for (int k0 = 0; k0 * nthr < n; k0 += FF_UNPACK_B) {
-        uint32_t w[FF_UNPACK_B][3], b[FF_UNPACK_B];
+    for (int k0 = 0; k0 * nthr < n; k0 += UB) {
+        uint32_t w[UB][3], b[UB];
 #pragma unroll
-        for (int e = 0; e < FF_UNPACK_B; e++) {
+        for (int e = 0; e < UB; e++) {
             const int i = tid + (k0 + e) * nthr;
             b[e] = cs.bit(i >> t.logC, i & (t.C - 1));
             // past the tile (i >= n) the loads are harmless and their values unused
             cs.words(b[e], w[e]);
         }
 #pragma unroll
-        for (int e = 0; e < FF_UNPACK_B; e++) {
+        for (int e = 0; e < UB; e++) {
             const int i = tid + (k0 + e) * nthr;
             const uint32_t r = cs.value<WIDE>(w[e], b[e]);
             if (i < n) t.s[ff_pos_addr(t, i >> t.logC, i & (t.C - 1))] = r;
@@ -195,9 +197,10 @@ __device__ __forceinline__ void ff_unpack_tile_w(const FFTile &t, const FFCoefSr
     }
 }
 
+template <int UB = FF_UNPACK_B>
 __device__ __forceinline__ void ff_unpack_tile(const FFTile &t, const FFCoefSrc &cs) {
-    if (cs.bw > 32) ff_unpack_tile_w<true>(t, cs);
-    else ff_unpack_tile_w<false>(t, cs);
+    if (cs.bw > 32) ff_unpack_tile_w<true, UB>(t, cs);
+    else ff_unpack_tile_w<false, UB>(t, cs);
 }
 
 // Tile shapes of the outer passes with a fixed-geometry path (rows 2^rho,
@@ -272,7 +275,7 @@ __device__ __forceinline__ void outer_body(const FFPass &ps, const FFXform &X, i
         const FFOuterTw<INV> twf{T, pi, S.rho, R3, top, c0, rootstep, scm, p, pinv};
         if constexpr (!INV) {
             if constexpr (TOP) {
-                ff_unpack_tile(tf, FFCoefSrc(X.src, nw, Sb, c0, bw, P));
+                ff_unpack_tile<S.rho >= 10 ? 8 : 4>(tf, FFCoefSrc(X.src, nw, Sb, c0, bw, P));
                 if constexpr (R3) ff_radix3_layer<false>(tf, T, pi, P, ps.rstep3);
                 else __syncthreads();
                 ff_forward_fixed<S.rho, S.rl, 0, false, true>(tf, T->rt[pi], T->rtp[pi], p, pinv, lds_load, store_fwd,

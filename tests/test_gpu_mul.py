@@ -92,6 +92,21 @@ def test_large_products_vs_reference(ff, ref, na):
     assert (ff.big_mul_ntt(a, b) == ref.mul_ntt(a, b)).all()
 
 
+@pytest.mark.parametrize("nb", [1, 2, 65, 4097, 65537, (1 << 18) - 1, (1 << 18) + 1, 400000])
+def test_uneven_products_vs_reference(ff, ref, nb):
+    """A 2^19-limb operand times operands of many sizes, against the
+    reference's big_mul_ntt: the smaller operand's forward top-pass tile is
+    zero from some row on, and those rows are stored without loads in batches
+    of rows; the sizes put that row at many places against the batches, at
+    widths from 32 to 35 bits (the narrower the smaller operand, the wider)."""
+    rng = random.Random(nb + 7)
+    na = 1 << 19
+    a = to_limbs(rng.getrandbits(64 * na) | (1 << (64 * na - 1)))
+    b = to_limbs(rng.getrandbits(64 * nb) | (1 << (64 * nb - 1)))
+    assert (ff.big_mul_ntt(a, b) == ref.mul_ntt(a, b)).all()
+    assert (ff.big_mul_ntt(b, a) == ref.mul_ntt(b, a)).all()
+
+
 def xorshift_limbs(rng, n):
     """n random limbs from the generator, top bit set (nttfuzz.c:22-30)."""
     out = np.fromiter((rng() for _ in range(n)), dtype=np.uint64, count=n)
